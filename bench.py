@@ -1,0 +1,120 @@
+#!/usr/bin/env python3
+"""Benchmark: forward Euler steps of the 2-D shallow-water solver on the
+~1M-triangle channel (10 km x 1 km, seeded Delaunay mesh, bumpy bed, 5 Manning
+zones, INLET-Q 500 m^3/s, EXIT-H 3 m, walls), FP64, dt from CFL 0.5.
+
+A step is one fused RHS + Euler update of every cell (custom_ODE_update_cells);
+the state and all mesh data are resident in HBM before the timed region.
+Prints one JSON line (see the keys below).  Single GPU only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hydrograd.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def bytes_per_cell_step(mesh):
+    """Algorithmic HBM bytes of one forward cell-step: read xi, q_x, q_y (24),
+    write them (24), per-cell constants h_still, zb, n, S0x, S0y, area (48),
+    per cell-face neighbour index (4) + n_x, n_y (16) + length (8)."""
+    avg_deg = float(mesh.face_off[-1]) / mesh.n_cells
+    return 24 + 24 + 48 + 28 * avg_deg
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the step kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when not measured."""
+    p = os.path.join(ROOT, "profiles", "k_swe_step_channel1M_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(case, seconds):
+    """The oracle (C++ restatement of the reference's loop structure) on the
+    host, one thread like the single-threaded reference, on a bounded sample
+    of the same workload: a few Euler steps of the same mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    orc = pyoracle.Oracle(case)
+    pyoracle.set_threads(1)
+    t0 = time.perf_counter()
+    orc.run(case.Q0, case.dt, 1)
+    t1 = time.perf_counter()
+    n = max(1, min(50, int(seconds / max(t1 - t0, 1e-3))))
+    t0 = time.perf_counter()
+    orc.run(case.Q0, case.dt, n)
+    el = time.perf_counter() - t0
+    return {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} Euler steps of the same {case.N}-cell mesh, C++ oracle -O3, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU domain decomposition "
+              "is design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    from hgswe import capi, cases
+
+    t0 = time.perf_counter()
+    case = cases.channel(args.cells, seed=0, nsteps=args.steps)
+    print(f"# case {case.name}: N={case.N} faces={case.mesh.n_faces} ghosts={case.G} dt={case.dt:.6g} "
+          f"(built in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx = capi.Context(case)
+    print(f"# {capi.info()}", file=sys.stderr, flush=True)
+    ctx.upload(case.Q0)
+    ctx.advance(case.dt, args.warmup)
+    # timed region: K device-resident steps, synchronized on both sides
+    t0 = time.perf_counter()
+    ctx.advance(case.dt, args.steps)
+    el = time.perf_counter() - t0
+    value = case.N * args.steps / el
+    ms_step, ms_kernel = ctx.time_advance(case.dt, 100, 2)
+    bpc = bytes_per_cell_step(case.mesh)
+    achieved = bpc * case.N / (ms_kernel * 1e-3) / 1e9
+    out = {
+        "metric": "cell-steps/s fwd and fwd+adjoint gradient, 1M-cell tri mesh FP64; % HBM roof",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "channel_1M_tri_forward_euler", "cells": case.N, "faces": int(case.mesh.n_faces),
+                   "ghosts": case.G, "cfl": 0.5, "dt": case.dt, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_swe_step", "kernel_ms": ms_kernel, "bytes_per_cell_step": bpc},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+// hg_ctx.hpp -- the context behind an hg_ctx* handle (host-side layout of
+// every device buffer) and the kernel argument blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "hgswe.h"
+
+namespace hg {
+
+// Arguments of the fused RHS / Euler-step kernel.  Cell data are in the
+// library's (possibly renumbered) order; faces in ELL form, slot k of cell i
+// at k * N + i, so every face load of a wavefront is coalesced.
+struct StepArgs {
+  int32_t N;
+  int32_t maxdeg;
+  const double* xi;   // state in
+  const double* qx;
+  const double* qy;
+  double* xo;         // state out (step) or dQdt (rhs)
+  double* qxo;
+  double* qyo;
+  const double* hstill;
+  const double* zb;
+  const double* n;
+  const double* S0x;
+  const double* S0y;
+  const double* area;
+  const int32_t* deg;
+  const int32_t* nbr;  // >= 0 cell, < 0 ~ghost
+  const double* fnx;
+  const double* fny;
+  const double* flen;
+  const int32_t* g_kind;
+  const int32_t* g_bc;
+  const double* g_hstill;
+  const double* g_zb;
+  const double* g_L23;   // L^(2/3) of inlet ghosts
+  const double* inletQ;
+  const double* exitWSE;
+  const double* totalA;  // per inlet, this step
+  double g, half_g, kn2, hsmall, dt;
+};
+
+struct InletArgs {
+  int32_t n_inlet;
+  const int32_t* off;
+  const int32_t* cell;
+  const double* L53;
+  const double* xi;
+  const double* hstill;
+  const double* n;
+  double hsmall;
+  double* totalA;
+  int32_t* flag;  // set to 1 when a conveyance is <= 1e-10
+};
+
+template <class T>
+struct DBuf {  // owning device buffer
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace hg
+
+struct hg_ctx {
+  std::string err;
+  int64_t N = 0, G = 0;
+  int32_t maxdeg = 0;
+  int32_t n_inlet = 0, n_exit = 0, n_inlet_faces = 0;
+  hg_consts c{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  bool fields_set = false;
+
+  // renumbering: lib index -> caller index, and inverse
+  std::vector<int32_t> perm, iperm;
+  // host copies needed by parameter application
+  std::vector<int32_t> h_zone;  // lib order
+  int32_t n_zones = 0;
+  int32_t active = HG_PARAM_NONE;
+
+  // device buffers (lib order)
+  double* st[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  int cur = 0;
+  double *hstill = nullptr, *zb = nullptr, *n = nullptr, *S0x = nullptr, *S0y = nullptr, *area = nullptr;
+  double *zb0 = nullptr, *n0 = nullptr, *S0x0 = nullptr, *S0y0 = nullptr;  // as set by hg_set_fields
+  int32_t *deg = nullptr, *nbr = nullptr;
+  double *fnx = nullptr, *fny = nullptr, *flen = nullptr;
+  int32_t *g_kind = nullptr, *g_bc = nullptr, *g_cell = nullptr;
+  double *g_hstill = nullptr, *g_zb = nullptr, *g_zb0 = nullptr, *g_L23 = nullptr;
+  int32_t *in_off = nullptr, *in_cell = nullptr;
+  double* in_L53 = nullptr;
+  double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
+  int32_t* flags = nullptr;  // [0] conveyance error
+  int32_t* zone = nullptr;
+  double* scratch = nullptr;  // 3N doubles for host transfers / params
+  size_t scratch_n = 0;
+  double* red = nullptr;      // reduction partials
+
+  // cached graph of two steps (A->B->A) per dt
+  hipGraphExec_t gexec = nullptr;
+  double gdt = 0.0;
+
+  std::vector<void*> allocs;
+};

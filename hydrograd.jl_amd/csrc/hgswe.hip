@@ -1,0 +1,903 @@
+// hgswe.hip -- forward path of libhgswe: context set-up, the fused
+// RHS / Euler-step kernel, inlet conveyance, bed data, zone gather, CFL, and
+// the C ABI declared in include/hgswe.h.
+//
+// Reference path (Hydrograd.jl): swe_2d_rhs, semi_discretize_swe_2D.jl:18-277
+// -> process_all_boundaries_2d (bc_2D.jl:575-875), compute_inviscid_fluxes
+// (:281-434) -> Riemann_2D_Roe (swe_2D_solvers.jl:4-164),
+// compute_source_terms (:449-559); custom_ODE_update_cells
+// (custom_ODE_solvers.jl:5-33).
+//
+// Design: one work-item per cell gathers its own faces in the reference's
+// local order (no atomics, deterministic, every internal face evaluated from
+// both sides exactly like the reference).  Ghost states are formed inline
+// because a ghost's internal cell is the cell that owns the boundary face;
+// only the inlet conveyance (a sequential sum over an inlet's faces) needs a
+// separate tiny kernel per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hg_ctx.hpp"
+#include "hg_device.hpp"
+#include "hgswe.h"
+
+using hg::Side;
+using hg::StepArgs;
+
+static std::string g_create_err;
+
+#define HG_TRY(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                    \
+      return HG_ERR_HIP;                                                               \
+    }                                                                                  \
+  } while (0)
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <class T>
+int dalloc(hg_ctx* ctx, T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return HG_ERR_HIP;
+  }
+  ctx->allocs.push_back(*p);
+  return HG_OK;
+}
+
+template <class T>
+int upload(hg_ctx* ctx, T* dst, const T* src, size_t n) {
+  if (n == 0) return HG_OK;
+  HG_TRY(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+// --------------------------------------------------------------- kernels ---
+
+// Inlet conveyance (bc_2D.jl:640-680): total_A_b = sum over the inlet's
+// faces, in NodeString order, of ((L^(5/3) h_c) / n_c) * wet_c.  The terms
+// are formed in parallel, the sum is a left fold by one lane (the
+// reference's sequential order).
+__global__ __launch_bounds__(kBlock) void k_inlet_conveyance(hg::InletArgs a) {
+  __shared__ double term[kBlock];
+  const int b = blockIdx.x;
+  const int f0 = a.off[b], f1 = a.off[b + 1];
+  double acc = 0.0;
+  for (int base = f0; base < f1; base += kBlock) {
+    const int f = base + threadIdx.x;
+    if (f < f1) {
+      const int c = a.cell[f];
+      double h = a.xi[c] + a.hstill[c];
+      if (h <= a.hsmall) h = a.hsmall;
+      const double w = (h > a.hsmall) ? 1.0 : 0.0;
+      term[threadIdx.x] = a.L53[f] * h / a.n[c] * w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int cnt = min(kBlock, f1 - base);
+      for (int k = 0; k < cnt; ++k) acc = (base + k == f0) ? term[k] : acc + term[k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.totalA[b] = acc;
+    if (!(acc > 1e-10)) a.flag[0] = 1;
+  }
+}
+
+// Fused RHS (+ Euler step and dry clamp when STEP).  One work-item per cell.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const double hsmall = a.hsmall;
+  const double xi = a.xi[i], hs = a.hstill[i];
+  const double qx0 = a.qx[i], qy0 = a.qy[i];
+  double h = xi + hs, qx = qx0, qy = qy0;  // semi_discretize_swe_2D.jl:101-106
+  if (h <= hsmall) {
+    h = hsmall;
+    qx = 0.0;
+    qy = 0.0;
+  }
+  const double zb = a.zb[i];
+  const double nman = a.n[i];
+  const Side Ls{xi, hs, h, qx, qy, zb};
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  const int d = a.deg[i];
+  for (int k = 0; k < d; ++k) {
+    const int s = k * a.N + i;
+    const int j = a.nbr[s];
+    const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
+    Side Rs;
+    if (j >= 0) {
+      const double xj = a.xi[j], hsj = a.hstill[j];
+      double hj = xj + hsj, qxj = a.qx[j], qyj = a.qy[j];
+      if (hj <= hsmall) {
+        hj = hsmall;
+        qxj = 0.0;
+        qyj = 0.0;
+      }
+      Rs = Side{xj, hsj, hj, qxj, qyj, a.zb[j]};
+    } else {  // ghost (bc_2D.jl:640-834), internal cell = this cell
+      const int gi = ~j;
+      double hg, qxg, qyg;
+      switch (a.g_kind[gi]) {
+        case HG_BC_INLETQ: {
+          const int b = a.g_bc[gi];
+          const double w = (h > hsmall) ? 1.0 : 0.0;
+          const double v = a.inletQ[b] / a.totalA[b] * a.g_L23[gi] / nman;
+          hg = h;
+          qxg = -h * v * nx * w;
+          qyg = -h * v * ny * w;
+          break;
+        }
+        case HG_BC_EXITH: {
+          const double hx = a.exitWSE[a.g_bc[gi]] - zb;
+          hg = (hx > hsmall) ? hx : hsmall;
+          qxg = qx;
+          qyg = qy;
+          break;
+        }
+        case HG_BC_WALL:
+          hg = h;
+          qxg = -qx;
+          qyg = -qy;
+          break;
+        default: {  // HG_BC_SYMM
+          const double vdn = qx * nx + qy * ny;
+          hg = h;
+          qxg = qx - 2.0 * vdn * nx;
+          qyg = qy - 2.0 * vdn * ny;
+          break;
+        }
+      }
+      const double hsg = a.g_hstill[gi];
+      Rs = Side{hg - hsg, hsg, hg, qxg, qyg, a.g_zb[gi]};
+    }
+    double f0, f1, f2;
+    hg::roe_flux(Ls, Rs, nx, ny, a.g, a.half_g, hsmall, f0, f1, f2);
+    acc0 = acc0 + f0 * L;
+    acc1 = acc1 + f1 * L;
+    acc2 = acc2 + f2 * L;
+  }
+  const double A = a.area[i];
+  const double r0 = -acc0 / A, r1 = -acc1 / A, r2 = -acc2 / A;
+  double s1 = 0.0, s2 = 0.0;
+  if (h > hsmall) {  // :463-475
+    double fx, fy;
+    hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
+    s1 = a.g * xi * a.S0x[i] - fx;
+    s2 = a.g * xi * a.S0y[i] - fy;
+  }
+  const double d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;  // :264
+  if (STEP) {  // custom_ODE_solvers.jl:16-26
+    double xn = xi + a.dt * d0;
+    double qxn = qx0 + a.dt * d1;
+    double qyn = qy0 + a.dt * d2;
+    if (xn < hsmall) {
+      xn = hsmall;
+      qxn = 0.0;
+      qyn = 0.0;
+    }
+    a.xo[i] = xn;
+    a.qxo[i] = qxn;
+    a.qyo[i] = qyn;
+  } else {
+    a.xo[i] = d0;
+    a.qxo[i] = d1;
+    a.qyo[i] = d2;
+  }
+}
+
+// update_bed_data (process_bed_2D.jl:46-66): zb_ghost and S0 from zb.
+__global__ __launch_bounds__(kBlock) void k_bed_data(int N, int G, const double* zb, const int32_t* deg,
+                                                    const int32_t* nbr, const double* fnx, const double* fny,
+                                                    const double* flen, const double* area, const int32_t* g_cell,
+                                                    double* S0x, double* S0y, double* g_zb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < G) g_zb[i] = zb[g_cell[i]];
+  if (i >= N) return;
+  const double z = zb[i];
+  double gx = 0.0, gy = 0.0;
+  for (int k = 0; k < deg[i]; ++k) {
+    const int s = k * N + i;
+    const int j = nbr[s];
+    const double zf = (j >= 0) ? (z + zb[j]) * 0.5 : z;
+    gx = gx + fnx[s] * zf * flen[s];
+    gy = gy + fny[s] * zf * flen[s];
+  }
+  S0x[i] = -(gx / area[i]);
+  S0y[i] = -(gy / area[i]);
+}
+
+// n_cell = params[zone] (process_ManningN_2D.jl:71-98).
+__global__ __launch_bounds__(kBlock) void k_zone_gather(int N, const int32_t* zone, const double* zn, double* n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) n[i] = zn[zone[i]];
+}
+
+// CFL: per-block minimum of (2A/P)/(|u| + sqrt(g h)) over wet cells.
+__global__ __launch_bounds__(kBlock) void k_cfl(int N, const double* xi, const double* qx, const double* qy,
+                                               const double* hstill, const int32_t* deg, const double* flen,
+                                               const double* area, double g, double hsmall, double* part) {
+  __shared__ double red[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = INFINITY;
+  if (i < N) {
+    const double h = xi[i] + hstill[i];
+    if (h > hsmall) {
+      double P = 0.0;
+      for (int k = 0; k < deg[i]; ++k) P = P + flen[k * N + i];
+      const double u = qx[i] / h, w = qy[i] / h;
+      const double spd = sqrt(u * u + w * w) + sqrt(g * h);
+      v = ((2.0 * area[i]) / P) / spd;
+    }
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------ host helpers --
+
+uint64_t spread21(uint64_t v) {
+  v &= 0x1FFFFFull;
+  v = (v | (v << 32)) & 0x1F00000000FFFFull;
+  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+std::vector<int32_t> morton_perm(int64_t N, const double* xy) {
+  double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+  for (int64_t i = 0; i < N; ++i)
+    for (int d = 0; d < 2; ++d) {
+      lo[d] = std::min(lo[d], xy[2 * i + d]);
+      hi[d] = std::max(hi[d], xy[2 * i + d]);
+    }
+  const double span = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+  std::vector<std::pair<uint64_t, int32_t>> key(N);
+  const double scale = double((1u << 21) - 1) / span;
+  for (int64_t i = 0; i < N; ++i) {
+    uint64_t qx = uint64_t(std::min(std::max((xy[2 * i] - lo[0]) * scale, 0.0), double((1u << 21) - 1)));
+    uint64_t qy = uint64_t(std::min(std::max((xy[2 * i + 1] - lo[1]) * scale, 0.0), double((1u << 21) - 1)));
+    key[i] = {spread21(qx) | (spread21(qy) << 1), int32_t(i)};
+  }
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
+                     return a.first < b.first;
+                   });
+  std::vector<int32_t> perm(N);
+  for (int64_t i = 0; i < N; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
+StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = nullptr, double* out1 = nullptr,
+                   double* out2 = nullptr) {
+  StepArgs a{};
+  a.N = int32_t(ctx->N);
+  a.maxdeg = ctx->maxdeg;
+  a.xi = ctx->st[src][0];
+  a.qx = ctx->st[src][1];
+  a.qy = ctx->st[src][2];
+  a.xo = out0 ? out0 : ctx->st[dst][0];
+  a.qxo = out1 ? out1 : ctx->st[dst][1];
+  a.qyo = out2 ? out2 : ctx->st[dst][2];
+  a.hstill = ctx->hstill;
+  a.zb = ctx->zb;
+  a.n = ctx->n;
+  a.S0x = ctx->S0x;
+  a.S0y = ctx->S0y;
+  a.area = ctx->area;
+  a.deg = ctx->deg;
+  a.nbr = ctx->nbr;
+  a.fnx = ctx->fnx;
+  a.fny = ctx->fny;
+  a.flen = ctx->flen;
+  a.g_kind = ctx->g_kind;
+  a.g_bc = ctx->g_bc;
+  a.g_hstill = ctx->g_hstill;
+  a.g_zb = ctx->g_zb;
+  a.g_L23 = ctx->g_L23;
+  a.inletQ = ctx->inletQ;
+  a.exitWSE = ctx->exitWSE;
+  a.totalA = ctx->totalA;
+  a.g = ctx->c.g;
+  a.half_g = 0.5 * ctx->c.g;
+  a.kn2 = ctx->c.k_n * ctx->c.k_n;
+  a.hsmall = ctx->c.h_small;
+  a.dt = dt;
+  return a;
+}
+
+hg::InletArgs inlet_args(hg_ctx* ctx, int src) {
+  hg::InletArgs a{};
+  a.n_inlet = ctx->n_inlet;
+  a.off = ctx->in_off;
+  a.cell = ctx->in_cell;
+  a.L53 = ctx->in_L53;
+  a.xi = ctx->st[src][0];
+  a.hstill = ctx->hstill;
+  a.n = ctx->n;
+  a.hsmall = ctx->c.h_small;
+  a.totalA = ctx->totalA;
+  a.flag = ctx->flags;
+  return a;
+}
+
+int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
+
+// Enqueue one step (or one RHS evaluation) reading state `src`.
+int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
+                 double* o2 = nullptr) {
+  if (ctx->n_inlet > 0)
+    hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
+  StepArgs a = make_args(ctx, src, dst, dt, o0, o1, o2);
+  if (step)
+    hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_swe_step<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+int clear_flags(hg_ctx* ctx) {
+  const int32_t z[4] = {0, 0, 0, 0};
+  HG_TRY(hipMemcpyAsync(ctx->flags, z, sizeof(z), hipMemcpyHostToDevice, ctx->stream));
+  return HG_OK;
+}
+
+int check_flags(hg_ctx* ctx) {
+  int32_t f[4];
+  HG_TRY(hipMemcpyAsync(f, ctx->flags, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  if (f[0]) {
+    ctx->err = "Total cross-sectional conveyance for an inlet-q boundary is not positive (<= 1e-10)";
+    return HG_ERR_CONVEYANCE;
+  }
+  return HG_OK;
+}
+
+// Host 3N state (caller order) -> device state `slot` (library order).
+int state_in(hg_ctx* ctx, const double* Q, int slot) {
+  const int64_t N = ctx->N;
+  std::vector<double> tmp(3 * N);
+  for (int c = 0; c < 3; ++c)
+    for (int64_t i = 0; i < N; ++i) tmp[c * N + i] = Q[c * N + ctx->perm[i]];
+  for (int c = 0; c < 3; ++c)
+    HG_TRY(hipMemcpyAsync(ctx->st[slot][c], tmp.data() + c * N, N * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+int state_out(hg_ctx* ctx, double* const src[3], double* Q) {
+  const int64_t N = ctx->N;
+  std::vector<double> tmp(3 * N);
+  for (int c = 0; c < 3; ++c)
+    HG_TRY(hipMemcpyAsync(tmp.data() + c * N, src[c], N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < 3; ++c)
+    for (int64_t i = 0; i < N; ++i) Q[c * N + ctx->perm[i]] = tmp[c * N + i];
+  return HG_OK;
+}
+
+int cell_in(hg_ctx* ctx, double* dst, const double* src) {  // per-cell host array -> device (lib order)
+  std::vector<double> tmp(ctx->N);
+  for (int64_t i = 0; i < ctx->N; ++i) tmp[i] = src[ctx->perm[i]];
+  return upload(ctx, dst, tmp.data(), ctx->N);
+}
+
+// Working copies of zb / n / S0 / zb_ghost / inletQ <- the values given to
+// hg_set_fields (p_extra is immutable in the reference; params override it
+// per call only).
+int restore_base(hg_ctx* ctx) {
+  const int64_t N = ctx->N;
+  HG_TRY(hipMemcpyAsync(ctx->zb, ctx->zb0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->S0x, ctx->S0x0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->S0y, ctx->S0y0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->G)
+    HG_TRY(hipMemcpyAsync(ctx->g_zb, ctx->g_zb0, ctx->G * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->n_inlet)
+    HG_TRY(hipMemcpyAsync(ctx->inletQ, ctx->inletQ0, ctx->n_inlet * sizeof(double), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+// Apply params_vector per the active parameter (semi_discretize_swe_2D.jl:114-199).
+int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
+  const int64_t N = ctx->N;
+  int rc0 = restore_base(ctx);
+  if (rc0) return rc0;
+  switch (ctx->active) {
+    case HG_PARAM_NONE:
+      break;
+    case HG_PARAM_ZB: {
+      if (np != N || !params) { ctx->err = "HG_PARAM_ZB expects n_params == n_cells"; return HG_ERR_ARG; }
+      int rc = cell_in(ctx, ctx->zb, params);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_bed_data, dim3(grid_of(std::max<int64_t>(N, ctx->G))), dim3(kBlock), 0, ctx->stream,
+                         int(N), int(ctx->G), ctx->zb, ctx->deg, ctx->nbr, ctx->fnx, ctx->fny, ctx->flen, ctx->area,
+                         ctx->g_cell, ctx->S0x, ctx->S0y, ctx->g_zb);
+      HG_TRY(hipGetLastError());
+      break;
+    }
+    case HG_PARAM_N_ZONE: {
+      if (np != ctx->n_zones || !params) { ctx->err = "HG_PARAM_N_ZONE expects n_params == n_zones"; return HG_ERR_ARG; }
+      int rc = upload(ctx, ctx->scratch, params, size_t(np));
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_zone_gather, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, int(N), ctx->zone,
+                         ctx->scratch, ctx->n);
+      HG_TRY(hipGetLastError());
+      break;
+    }
+    case HG_PARAM_N_CELL: {
+      if (np != N || !params) { ctx->err = "HG_PARAM_N_CELL expects n_params == n_cells"; return HG_ERR_ARG; }
+      int rc = cell_in(ctx, ctx->n, params);
+      if (rc) return rc;
+      break;
+    }
+    case HG_PARAM_Q: {
+      if (np != ctx->n_inlet || !params) { ctx->err = "HG_PARAM_Q expects n_params == n_inlet"; return HG_ERR_ARG; }
+      int rc = upload(ctx, ctx->inletQ, params, size_t(np));
+      if (rc) return rc;
+      break;
+    }
+    default:
+      ctx->err = "unknown active parameter";
+      return HG_ERR_ARG;
+  }
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+int ensure_graph(hg_ctx* ctx, double dt) {
+  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
+  if (ctx->gexec) {
+    HG_TRY(hipGraphExecDestroy(ctx->gexec));
+    ctx->gexec = nullptr;
+  }
+  hipGraph_t graph;
+  HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(ctx, 0, 1, dt, true);
+  if (!rc) rc = enqueue_step(ctx, 1, 0, dt, true);
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+    return HG_ERR_HIP;
+  }
+  HG_TRY(hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0));
+  HG_TRY(hipGraphDestroy(graph));
+  ctx->gdt = dt;
+  return HG_OK;
+}
+
+// Advance the device state nsteps (no host sync at the end).
+int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
+  if (nsteps <= 0) return HG_OK;
+  if (ctx->cur == 1) {
+    int rc = enqueue_step(ctx, 1, 0, dt, true);
+    if (rc) return rc;
+    ctx->cur = 0;
+    --nsteps;
+  }
+  if (nsteps >= 2) {
+    int rc = ensure_graph(ctx, dt);
+    if (rc) return rc;
+    for (int64_t k = 0; k + 1 < nsteps; k += 2) HG_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
+  }
+  if (nsteps & 1) {
+    int rc = enqueue_step(ctx, 0, 1, dt, true);
+    if (rc) return rc;
+    ctx->cur = 1;
+  }
+  return HG_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ====
+extern "C" {
+
+const char* hg_info(void) {
+  static std::string s;
+  if (s.empty()) {
+    int dev = 0;
+    hipDeviceProp_t p{};
+    char buf[512];
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+      snprintf(buf, sizeof(buf), "libhgswe: %s (%s), %d CUs, %.1f GB HBM; FP64, no FMA contraction", p.name,
+               p.gcnArchName, p.multiProcessorCount, double(p.totalGlobalMem) / 1e9);
+    else
+      snprintf(buf, sizeof(buf), "libhgswe: no HIP device");
+    s = buf;
+  }
+  return s.c_str();
+}
+
+const char* hg_last_error(const hg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags) {
+  if (!out || !m || !b || !c) {
+    g_create_err = "hg_create: NULL argument";
+    return HG_ERR_ARG;
+  }
+  *out = nullptr;
+  const int64_t N = m->n_cells, G = b->n_ghosts;
+  if (N <= 0 || N > (int64_t(1) << 30) || G < 0) {
+    g_create_err = "hg_create: bad cell / ghost count";
+    return HG_ERR_ARG;
+  }
+  if (!m->face_off || !m->face_nbr || !m->face_nx || !m->face_ny || !m->face_len || !m->area ||
+      (G > 0 && (!b->ghost_slot || !b->ghost_kind || !b->ghost_bc)) ||
+      (b->n_inlet > 0 && (!b->inlet_off || !b->inlet_ghost)) || b->n_inlet < 0 || b->n_exit < 0) {
+    g_create_err = "hg_create: NULL array";
+    return HG_ERR_ARG;
+  }
+  // ---- validate topology
+  if (m->face_off[0] != 0) { g_create_err = "face_off[0] != 0"; return HG_ERR_MESH; }
+  int32_t maxdeg = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    const int32_t d = m->face_off[i + 1] - m->face_off[i];
+    if (d < 3 || d > 16) { g_create_err = "cell " + std::to_string(i) + " has " + std::to_string(d) + " faces"; return HG_ERR_MESH; }
+    maxdeg = std::max(maxdeg, d);
+    if (!(m->area[i] > 0.0)) { g_create_err = "cell " + std::to_string(i) + " has non-positive area"; return HG_ERR_MESH; }
+  }
+  const int64_t M = m->face_off[N];
+  if (int64_t(maxdeg) * N >= (int64_t(1) << 31)) {  // ELL slot k * N + i must fit an int
+    g_create_err = "mesh too large for 32-bit face slots";
+    return HG_ERR_ARG;
+  }
+  std::vector<int32_t> ghost_seen(G, -1);
+  for (int64_t i = 0; i < N; ++i)
+    for (int32_t s = m->face_off[i]; s < m->face_off[i + 1]; ++s) {
+      const int32_t j = m->face_nbr[s];
+      if (j >= 0) {
+        bool back = false;
+        if (j >= N || j == i) { g_create_err = "face_nbr out of range at slot " + std::to_string(s); return HG_ERR_MESH; }
+        for (int32_t t = m->face_off[j]; t < m->face_off[j + 1]; ++t) back |= (m->face_nbr[t] == i);
+        if (!back) { g_create_err = "neighbour relation not symmetric at slot " + std::to_string(s); return HG_ERR_MESH; }
+      } else {
+        const int64_t gi = ~int64_t(j);
+        if (gi >= G || ghost_seen[gi] >= 0) { g_create_err = "bad ghost reference at slot " + std::to_string(s); return HG_ERR_MESH; }
+        ghost_seen[gi] = s;
+      }
+    }
+  for (int64_t gi = 0; gi < G; ++gi) {
+    if (ghost_seen[gi] < 0 || b->ghost_slot[gi] != ghost_seen[gi]) { g_create_err = "ghost_slot mismatch for ghost " + std::to_string(gi); return HG_ERR_MESH; }
+    const int32_t k = b->ghost_kind[gi];
+    if (k < 0 || k > 3) { g_create_err = "bad ghost_kind"; return HG_ERR_MESH; }
+    if ((k == HG_BC_INLETQ && (b->ghost_bc[gi] < 0 || b->ghost_bc[gi] >= b->n_inlet)) ||
+        (k == HG_BC_EXITH && (b->ghost_bc[gi] < 0 || b->ghost_bc[gi] >= b->n_exit))) {
+      g_create_err = "bad ghost_bc"; return HG_ERR_MESH;
+    }
+  }
+  if (b->n_inlet > 0) {
+    if (b->inlet_off[0] != 0) { g_create_err = "inlet_off[0] != 0"; return HG_ERR_MESH; }
+    for (int bi = 0; bi < b->n_inlet; ++bi)
+      for (int32_t f = b->inlet_off[bi]; f < b->inlet_off[bi + 1]; ++f) {
+        const int32_t gi = b->inlet_ghost[f];
+        if (gi < 0 || gi >= G || b->ghost_kind[gi] != HG_BC_INLETQ || b->ghost_bc[gi] != bi) {
+          g_create_err = "inlet_ghost inconsistent with ghost_kind / ghost_bc"; return HG_ERR_MESH;
+        }
+      }
+  }
+
+  hg_ctx* ctx = new hg_ctx();
+  ctx->N = N;
+  ctx->G = G;
+  ctx->maxdeg = maxdeg;
+  ctx->c = *c;
+  ctx->n_inlet = b->n_inlet;
+  ctx->n_exit = b->n_exit;
+  ctx->n_inlet_faces = b->n_inlet ? b->inlet_off[b->n_inlet] : 0;
+  auto fail = [&](int rc) {
+    g_create_err = ctx->err;
+    hg_destroy(ctx);
+    return rc;
+  };
+  if (hipGetDevice(&ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    ctx->err = "no HIP device / stream";
+    return fail(HG_ERR_HIP);
+  }
+  // ---- renumbering
+  if ((flags & HG_REORDER) && m->centroid_xy) {
+    ctx->perm = morton_perm(N, m->centroid_xy);
+  } else {
+    ctx->perm.resize(N);
+    std::iota(ctx->perm.begin(), ctx->perm.end(), 0);
+  }
+  ctx->iperm.resize(N);
+  for (int64_t i = 0; i < N; ++i) ctx->iperm[ctx->perm[i]] = int32_t(i);
+
+  // ---- ELL face arrays in library order
+  const int64_t E = int64_t(maxdeg) * N;
+  std::vector<int32_t> h_deg(N), h_nbr(E, 0);
+  std::vector<double> h_nx(E, 0.0), h_ny(E, 0.0), h_len(E, 0.0), h_area(N);
+  std::vector<int32_t> g_cell(G), g_kind(G), g_bc(G);
+  std::vector<double> g_L23(G, 0.0);
+  for (int64_t li = 0; li < N; ++li) {
+    const int32_t oi = ctx->perm[li];
+    const int32_t s0 = m->face_off[oi], d = m->face_off[oi + 1] - s0;
+    h_deg[li] = d;
+    h_area[li] = m->area[oi];
+    for (int32_t k = 0; k < d; ++k) {
+      const int32_t j = m->face_nbr[s0 + k];
+      h_nbr[k * N + li] = (j >= 0) ? ctx->iperm[j] : j;
+      h_nx[k * N + li] = m->face_nx[s0 + k];
+      h_ny[k * N + li] = m->face_ny[s0 + k];
+      h_len[k * N + li] = m->face_len[s0 + k];
+    }
+  }
+  // slot -> owning cell (caller order)
+  std::vector<int32_t> slot_cell(M);
+  for (int64_t i = 0; i < N; ++i)
+    for (int32_t s = m->face_off[i]; s < m->face_off[i + 1]; ++s) slot_cell[s] = int32_t(i);
+  for (int64_t gi = 0; gi < G; ++gi) {
+    g_cell[gi] = ctx->iperm[slot_cell[b->ghost_slot[gi]]];
+    g_kind[gi] = b->ghost_kind[gi];
+    g_bc[gi] = b->ghost_bc[gi];
+    if (g_kind[gi] == HG_BC_INLETQ) g_L23[gi] = std::pow(m->face_len[b->ghost_slot[gi]], 2.0 / 3.0);
+  }
+  std::vector<int32_t> in_off(b->n_inlet + 1, 0), in_cell(ctx->n_inlet_faces);
+  std::vector<double> in_L53(ctx->n_inlet_faces);
+  for (int bi = 0; bi <= b->n_inlet && b->n_inlet > 0; ++bi) in_off[bi] = b->inlet_off[bi];
+  for (int32_t f = 0; f < ctx->n_inlet_faces; ++f) {
+    const int32_t gi = b->inlet_ghost[f];
+    in_cell[f] = g_cell[gi];
+    in_L53[f] = std::pow(m->face_len[b->ghost_slot[gi]], 5.0 / 3.0);
+  }
+
+  // ---- device allocation
+  int rc = 0;
+  for (int s = 0; s < 2 && !rc; ++s)
+    for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(ctx, &ctx->st[s][k], N);
+  double** cellbufs[] = {&ctx->hstill, &ctx->zb, &ctx->n, &ctx->S0x, &ctx->S0y, &ctx->area,
+                         &ctx->zb0, &ctx->n0, &ctx->S0x0, &ctx->S0y0};
+  for (double** p : cellbufs)
+    if (!rc) rc = dalloc(ctx, p, N);
+  if (!rc) rc = dalloc(ctx, &ctx->deg, N);
+  if (!rc) rc = dalloc(ctx, &ctx->nbr, E);
+  if (!rc) rc = dalloc(ctx, &ctx->fnx, E);
+  if (!rc) rc = dalloc(ctx, &ctx->fny, E);
+  if (!rc) rc = dalloc(ctx, &ctx->flen, E);
+  if (!rc) rc = dalloc(ctx, &ctx->g_kind, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_bc, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_cell, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_hstill, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_zb, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_zb0, G);
+  if (!rc) rc = dalloc(ctx, &ctx->g_L23, G);
+  if (!rc) rc = dalloc(ctx, &ctx->in_off, b->n_inlet + 1);
+  if (!rc) rc = dalloc(ctx, &ctx->in_cell, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->in_L53, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->inletQ, b->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->inletQ0, b->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->exitWSE, b->n_exit);
+  if (!rc) rc = dalloc(ctx, &ctx->totalA, b->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->flags, 4);
+  if (!rc) rc = dalloc(ctx, &ctx->zone, N);
+  ctx->scratch_n = size_t(std::max<int64_t>(3 * N, 4096));
+  if (!rc) rc = dalloc(ctx, &ctx->scratch, ctx->scratch_n);
+  if (!rc) rc = dalloc(ctx, &ctx->red, size_t(grid_of(N) + 64));
+  if (rc) return fail(rc);
+
+  if (!rc) rc = upload(ctx, ctx->deg, h_deg.data(), N);
+  if (!rc) rc = upload(ctx, ctx->nbr, h_nbr.data(), E);
+  if (!rc) rc = upload(ctx, ctx->fnx, h_nx.data(), E);
+  if (!rc) rc = upload(ctx, ctx->fny, h_ny.data(), E);
+  if (!rc) rc = upload(ctx, ctx->flen, h_len.data(), E);
+  if (!rc) rc = upload(ctx, ctx->area, h_area.data(), N);
+  if (!rc) rc = upload(ctx, ctx->g_kind, g_kind.data(), G);
+  if (!rc) rc = upload(ctx, ctx->g_bc, g_bc.data(), G);
+  if (!rc) rc = upload(ctx, ctx->g_cell, g_cell.data(), G);
+  if (!rc) rc = upload(ctx, ctx->g_L23, g_L23.data(), G);
+  if (!rc) rc = upload(ctx, ctx->in_off, in_off.data(), b->n_inlet + 1);
+  if (!rc) rc = upload(ctx, ctx->in_cell, in_cell.data(), ctx->n_inlet_faces);
+  if (!rc) rc = upload(ctx, ctx->in_L53, in_L53.data(), ctx->n_inlet_faces);
+  if (!rc) rc = clear_flags(ctx);
+  if (rc) return fail(rc);
+  *out = ctx;
+  return HG_OK;
+}
+
+int hg_destroy(hg_ctx* ctx) {
+  if (!ctx) return HG_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  for (void* p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HG_OK;
+}
+
+int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost, const double* zb,
+                  const double* zb_ghost, const double* S0x, const double* S0y, const double* n,
+                  const double* inletQ, const double* exitWSE) {
+  if (!ctx) return HG_ERR_ARG;
+  if (!hstill || !zb || !S0x || !S0y || !n || (ctx->G && (!hstill_ghost || !zb_ghost)) ||
+      (ctx->n_inlet && !inletQ) || (ctx->n_exit && !exitWSE)) {
+    ctx->err = "hg_set_fields: NULL array";
+    return HG_ERR_ARG;
+  }
+  int rc = 0;
+  if (!rc) rc = cell_in(ctx, ctx->hstill, hstill);
+  if (!rc) rc = cell_in(ctx, ctx->zb0, zb);
+  if (!rc) rc = cell_in(ctx, ctx->S0x0, S0x);
+  if (!rc) rc = cell_in(ctx, ctx->S0y0, S0y);
+  if (!rc) rc = cell_in(ctx, ctx->n0, n);
+  if (!rc) rc = upload(ctx, ctx->g_hstill, hstill_ghost, ctx->G);
+  if (!rc) rc = upload(ctx, ctx->g_zb0, zb_ghost, ctx->G);
+  if (!rc) rc = upload(ctx, ctx->inletQ0, inletQ, ctx->n_inlet);
+  if (!rc) rc = upload(ctx, ctx->exitWSE, exitWSE, ctx->n_exit);
+  if (rc) return rc;
+  ctx->fields_set = true;
+  return restore_base(ctx);
+}
+
+int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, int32_t n_zones) {
+  if (!ctx) return HG_ERR_ARG;
+  if (kind < HG_PARAM_NONE || kind > HG_PARAM_Q) {
+    ctx->err = "unknown active parameter kind";
+    return HG_ERR_ARG;
+  }
+  if (kind == HG_PARAM_N_ZONE) {
+    if (!zone_of_cell || n_zones <= 0 || size_t(n_zones) > ctx->scratch_n) {
+      ctx->err = "HG_PARAM_N_ZONE needs zone_of_cell and n_zones > 0";
+      return HG_ERR_ARG;
+    }
+    std::vector<int32_t> z(ctx->N);
+    for (int64_t i = 0; i < ctx->N; ++i) {
+      const int32_t v = zone_of_cell[ctx->perm[i]];
+      if (v < 0 || v >= n_zones) {
+        ctx->err = "zone_of_cell out of range";
+        return HG_ERR_ARG;
+      }
+      z[i] = v;
+    }
+    int rc = upload(ctx, ctx->zone, z.data(), ctx->N);
+    if (rc) return rc;
+    ctx->h_zone = z;
+    ctx->n_zones = n_zones;
+  }
+  ctx->active = kind;
+  return HG_OK;
+}
+
+int hg_set_params(hg_ctx* ctx, const double* params, int64_t n_params) {
+  if (!ctx) return HG_ERR_ARG;
+  if (!ctx->fields_set) {
+    ctx->err = "hg_set_fields has not been called";
+    return HG_ERR_STATE;
+  }
+  return apply_params(ctx, params, n_params);
+}
+
+int hg_rhs(hg_ctx* ctx, const double* Q, const double* params, int64_t n_params, double* dQdt) {
+  if (!ctx || !Q || !dQdt) return ctx ? (ctx->err = "hg_rhs: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (!rc) rc = state_in(ctx, Q, 0);
+  if (rc) return rc;
+  ctx->cur = 0;
+  if (!rc) rc = clear_flags(ctx);
+  double* o[3] = {ctx->scratch, ctx->scratch + ctx->N, ctx->scratch + 2 * ctx->N};
+  if (!rc) rc = enqueue_step(ctx, 0, 1, 0.0, false, o[0], o[1], o[2]);
+  if (!rc) rc = state_out(ctx, o, dQdt);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+int hg_state_upload(hg_ctx* ctx, const double* Q) {
+  if (!ctx || !Q) return HG_ERR_ARG;
+  int rc = state_in(ctx, Q, 0);
+  if (!rc) ctx->cur = 0;
+  return rc;
+}
+
+int hg_state_download(hg_ctx* ctx, double* Q) {
+  if (!ctx || !Q) return HG_ERR_ARG;
+  return state_out(ctx, ctx->st[ctx->cur], Q);
+}
+
+int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps) {
+  if (!ctx) return HG_ERR_ARG;
+  if (!ctx->fields_set) {
+    ctx->err = "hg_set_fields has not been called";
+    return HG_ERR_STATE;
+  }
+  int rc = clear_flags(ctx);
+  if (!rc) rc = advance_async(ctx, dt, nsteps);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+int hg_step(hg_ctx* ctx, double* Q, const double* params, int64_t n_params, double dt, int64_t nsteps) {
+  if (!ctx || !Q) return HG_ERR_ARG;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (!rc) rc = hg_state_upload(ctx, Q);
+  if (!rc) rc = hg_advance(ctx, dt, nsteps);
+  if (!rc) rc = hg_state_download(ctx, Q);
+  return rc;
+}
+
+int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
+  if (!ctx || !Q || !dt) return HG_ERR_ARG;
+  if (!ctx->fields_set) {
+    ctx->err = "hg_set_fields has not been called";
+    return HG_ERR_STATE;
+  }
+  int rc = state_in(ctx, Q, 0);
+  if (rc) return rc;
+  ctx->cur = 0;
+  const int nb = grid_of(ctx->N);
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->st[0][0], ctx->st[0][1],
+                     ctx->st[0][2], ctx->hstill, ctx->deg, ctx->flen, ctx->area, ctx->c.g, ctx->c.h_small, ctx->red);
+  HG_TRY(hipGetLastError());
+  std::vector<double> part(nb);
+  HG_TRY(hipMemcpyAsync(part.data(), ctx->red, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  double mn = INFINITY;
+  for (double v : part) mn = std::min(mn, v);
+  *dt = cfl * mn;
+  return HG_OK;
+}
+
+int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
+                    double* ms_step_kernel) {
+  if (!ctx || nsteps <= 0 || reps <= 0) return HG_ERR_ARG;
+  hipEvent_t e0, e1;
+  HG_TRY(hipEventCreate(&e0));
+  HG_TRY(hipEventCreate(&e1));
+  int rc = clear_flags(ctx);
+  if (!rc) rc = advance_async(ctx, dt, 2);  // warm-up (graph instantiation)
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  HG_TRY(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps && !rc; ++r) rc = advance_async(ctx, dt, nsteps);
+  HG_TRY(hipEventRecord(e1, ctx->stream));
+  HG_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HG_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_per_step) *ms_per_step = double(ms) / double(reps * nsteps);
+  if (ms_step_kernel && !rc) {  // the step kernel alone, back-to-back on the same stream
+    const int K = 50;
+    HG_TRY(hipEventRecord(e0, ctx->stream));
+    for (int k = 0; k < K; ++k) {
+      StepArgs a = make_args(ctx, k & 1, (k + 1) & 1, dt);
+      hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+    }
+    HG_TRY(hipEventRecord(e1, ctx->stream));
+    HG_TRY(hipEventSynchronize(e1));
+    HG_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *ms_step_kernel = double(ms) / K;
+    ctx->cur = 0;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+}  // extern "C"

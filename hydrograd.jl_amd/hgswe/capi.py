@@ -1,0 +1,189 @@
+"""ctypes binding of libhgswe.so (include/hgswe.h).
+
+The product path: every call goes to the HIP library; when the library is
+missing or cannot load, the import of this module's ``lib()`` raises --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhgswe.so")
+_LIB = None
+
+HG_OK, HG_ERR_ARG, HG_ERR_HIP, HG_ERR_MESH, HG_ERR_CONVEYANCE, HG_ERR_NONFINITE, HG_ERR_STATE = range(7)
+HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = range(5)
+HG_REORDER = 1
+
+# every function declared in include/hgswe.h
+EXPORTS = [
+    "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
+    "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
+    "hg_info",
+]
+
+
+class HGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libhgswe error {code}: {msg}")
+        self.code = code
+
+
+class hg_mesh(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("face_off", C.c_void_p), ("face_nbr", C.c_void_p),
+                ("face_nx", C.c_void_p), ("face_ny", C.c_void_p), ("face_len", C.c_void_p),
+                ("area", C.c_void_p), ("centroid_xy", C.c_void_p)]
+
+
+class hg_bcs(C.Structure):
+    _fields_ = [("n_ghosts", C.c_int64), ("ghost_slot", C.c_void_p), ("ghost_kind", C.c_void_p),
+                ("ghost_bc", C.c_void_p), ("n_inlet", C.c_int32), ("n_exit", C.c_int32),
+                ("inlet_off", C.c_void_p), ("inlet_ghost", C.c_void_p)]
+
+
+class hg_consts(C.Structure):
+    _fields_ = [("g", C.c_double), ("k_n", C.c_double), ("h_small", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        L.hg_last_error.restype = C.c_char_p
+        L.hg_last_error.argtypes = [C.c_void_p]
+        L.hg_info.restype = C.c_char_p
+        L.hg_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(hg_mesh), C.POINTER(hg_bcs), C.POINTER(hg_consts),
+                                C.c_int32]
+        for name in ("hg_destroy",):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.hg_set_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 9
+        L.hg_set_active_param.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+        L.hg_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.hg_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_int64]
+        L.hg_set_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.hg_state_upload.argtypes = [C.c_void_p, C.c_void_p]
+        L.hg_state_download.argtypes = [C.c_void_p, C.c_void_p]
+        L.hg_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64]
+        L.hg_cfl_dt.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
+        L.hg_time_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else C.c_void_p(0)
+
+
+class Context:
+    """Owns one hg_ctx (one HIP stream) built from a ``cases.Case``."""
+
+    def __init__(self, case, reorder=True):
+        L = lib()
+        m = case.mesh
+        self.case = case
+        self.N = m.n_cells
+        self._keep = []
+
+        def k(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            if a.size == 0:
+                a = np.zeros(1, dtype=dt)
+            self._keep.append(a)
+            return _ptr(a)
+
+        mesh = hg_mesh(m.n_cells, k(m.face_off, np.int32), k(m.face_nbr, np.int32), k(m.face_nx, np.float64),
+                       k(m.face_ny, np.float64), k(m.face_len, np.float64), k(m.area, np.float64),
+                       k(m.centroid.ravel(), np.float64))
+        bcs = hg_bcs(m.n_ghosts, k(m.ghost_slot, np.int32), k(m.ghost_kind, np.int32), k(m.ghost_bc, np.int32),
+                     m.n_inlet, m.n_exit, k(m.inlet_off, np.int32), k(m.inlet_ghost, np.int32))
+        consts = hg_consts(case.g, case.k_n, case.h_small)
+        h = C.c_void_p()
+        rc = L.hg_create(C.byref(h), C.byref(mesh), C.byref(bcs), C.byref(consts), HG_REORDER if reorder else 0)
+        if rc:
+            raise HGError(rc, L.hg_last_error(None).decode())
+        self.h = h
+        self._keep = []
+        self.set_fields(case)
+
+    def _chk(self, rc):
+        if rc:
+            raise HGError(rc, lib().hg_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, case, **over):
+        vals = dict(hstill=case.hstill, hstill_ghost=case.hstill_ghost, zb=case.zb, zb_ghost=case.zb_ghost,
+                    S0x=case.S0x, S0y=case.S0y, n=case.n, inletQ=case.inletQ, exitWSE=case.exitWSE)
+        vals.update(over)
+        arrs = []
+        for name in ("hstill", "hstill_ghost", "zb", "zb_ghost", "S0x", "S0y", "n", "inletQ", "exitWSE"):
+            a = np.ascontiguousarray(vals[name], dtype=np.float64)
+            arrs.append(a if a.size else np.zeros(1))
+        self._chk(lib().hg_set_fields(self.h, *[_ptr(a) for a in arrs]))
+
+    def set_active_param(self, kind, zone=None, n_zones=0):
+        z = None if zone is None else np.ascontiguousarray(zone, dtype=np.int32)
+        self._chk(lib().hg_set_active_param(self.h, int(kind), _ptr(z), int(n_zones)))
+
+    def rhs(self, Q, params=None):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        out = np.empty(3 * self.N)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(lib().hg_rhs(self.h, _ptr(Q), _ptr(p), 0 if p is None else p.size, _ptr(out)))
+        return out
+
+    def step(self, Q, dt, nsteps, params=None):
+        Q = np.array(Q, dtype=np.float64, copy=True)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(lib().hg_step(self.h, _ptr(Q), _ptr(p), 0 if p is None else p.size, C.c_double(dt),
+                                C.c_int64(nsteps)))
+        return Q
+
+    def set_params(self, params=None):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self._chk(lib().hg_set_params(self.h, _ptr(p), 0 if p is None else p.size))
+
+    def upload(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        self._chk(lib().hg_state_upload(self.h, _ptr(Q)))
+
+    def download(self):
+        out = np.empty(3 * self.N)
+        self._chk(lib().hg_state_download(self.h, _ptr(out)))
+        return out
+
+    def advance(self, dt, nsteps):
+        self._chk(lib().hg_advance(self.h, C.c_double(dt), C.c_int64(nsteps)))
+
+    def cfl_dt(self, Q, cfl):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        dt = C.c_double()
+        self._chk(lib().hg_cfl_dt(self.h, _ptr(Q), C.c_double(cfl), C.byref(dt)))
+        return dt.value
+
+    def time_advance(self, dt, nsteps, reps):
+        a = C.c_double()
+        b = C.c_double()
+        self._chk(lib().hg_time_advance(self.h, C.c_double(dt), C.c_int64(nsteps), C.c_int32(reps), C.byref(a),
+                                        C.byref(b)))
+        return a.value, b.value
+
+
+def info():
+    return lib().hg_info().decode()

@@ -1,0 +1,202 @@
+# HydrogradHIP.jl -- drop-in MI355X path for Hydrograd.jl's SWE right-hand
+# side and Euler step, through libhgswe.so (include/hgswe.h).
+#
+# Usage (inside a Hydrograd session):
+#     using Hydrograd, HydrogradHIP
+#     ode_f = HydrogradHIP.make_ode_f(swe2d_extra_params)   # replaces the closure of solve_swe_2D.jl:281-294
+#     Q = HydrogradHIP.custom_ODE_update_cells(ode_f, Q, params_vector, t, dt, swe2d_extra_params)
+#
+# The functions keep the reference's names and signatures:
+#   swe_2d_rhs(dQdt, Q, params_vector, t, p_extra)           semi_discretize_swe_2D.jl:18-19
+#   custom_ODE_update_cells(ode_f, Q, params_vector, t, dt, p)  custom_ODE_solvers.jl:5
+#   custom_ODE_solve(ode_f, Q0, params_vector, p)             custom_ODE_solvers.jl:36
+# and read the reference's own structs (mesh_2D, BoundaryConditions2D,
+# SWE2D_Extra_Parameters) unchanged.  ForwardDiff.Dual values cannot cross
+# the C ABI: they are rejected with an error.
+module HydrogradHIP
+
+export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f
+
+const libhgswe = joinpath(@__DIR__, "..", "lib", "libhgswe.so")
+
+const HG_BC = Dict("inlet-q" => Int32(0), "exit-h" => Int32(1), "wall" => Int32(2), "symm" => Int32(3))
+const HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = Int32.(0:4)
+
+struct HgMesh
+    n_cells::Int64
+    face_off::Ptr{Int32}
+    face_nbr::Ptr{Int32}
+    face_nx::Ptr{Float64}
+    face_ny::Ptr{Float64}
+    face_len::Ptr{Float64}
+    area::Ptr{Float64}
+    centroid_xy::Ptr{Float64}
+end
+
+struct HgBcs
+    n_ghosts::Int64
+    ghost_slot::Ptr{Int32}
+    ghost_kind::Ptr{Int32}
+    ghost_bc::Ptr{Int32}
+    n_inlet::Int32
+    n_exit::Int32
+    inlet_off::Ptr{Int32}
+    inlet_ghost::Ptr{Int32}
+end
+
+struct HgConsts
+    g::Float64
+    k_n::Float64
+    h_small::Float64
+end
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    N::Int
+    active::Int32
+end
+
+check(ctx, rc) = rc == 0 || error("libhgswe: " * unsafe_string(ccall((:hg_last_error, libhgswe), Cstring, (Ptr{Cvoid},), ctx)))
+
+# Flatten mesh_2D (mesh_2D.jl:2-71) + BoundaryConditions2D (bc_2D.jl:3-47) to
+# the CSR arrays of hg_mesh / hg_bcs (0-based).
+function flatten(m, bcs)
+    N = m.numOfCells
+    off = zeros(Int32, N + 1)
+    for i in 1:N
+        off[i+1] = off[i] + m.cellNodesCount[i]
+    end
+    M = off[end]
+    nbr = zeros(Int32, M); nx = zeros(M); ny = zeros(M); len = zeros(M)
+    slot_of_face = Dict{Int,Int}()
+    for i in 1:N, k in 1:m.cellNodesCount[i]
+        s = off[i] + k                              # 1-based slot
+        f = m.cellFacesList[i, k]
+        j = m.cellNeighbors_Dict[i][k]
+        nbr[s] = m.bFace_is_boundary[f] ? ~Int32(j - 1) : Int32(j - 1)
+        nx[s], ny[s] = m.cell_normals[i][k]
+        len[s] = m.face_lengths[f]
+        m.bFace_is_boundary[f] && (slot_of_face[f] = s)
+    end
+    G = m.numOfAllBounaryFaces
+    gslot = zeros(Int32, G); gkind = zeros(Int32, G); gbc = fill(Int32(-1), G)
+    function mark!(faceIDs, ghostIDs, kind, idx)
+        for (f, g) in zip(faceIDs, ghostIDs)
+            gslot[g] = slot_of_face[f] - 1; gkind[g] = kind; gbc[g] = idx
+        end
+    end
+    for b in 1:bcs.nInletQ_BCs; mark!(bcs.inletQ_faceIDs[b], bcs.inletQ_ghostCellIDs[b], 0, b - 1); end
+    for b in 1:bcs.nExitH_BCs;  mark!(bcs.exitH_faceIDs[b], bcs.exitH_ghostCellIDs[b], 1, b - 1); end
+    for b in 1:bcs.nWall_BCs;   mark!(bcs.wall_faceIDs[b], bcs.wall_ghostCellIDs[b], 2, -1); end
+    for b in 1:bcs.nSymm_BCs;   mark!(bcs.symm_faceIDs[b], bcs.symm_ghostCellIDs[b], 3, -1); end
+    inlet_off = zeros(Int32, bcs.nInletQ_BCs + 1)
+    inlet_ghost = Int32[]
+    for b in 1:bcs.nInletQ_BCs                      # NodeString order
+        append!(inlet_ghost, Int32.(bcs.inletQ_ghostCellIDs[b] .- 1))
+        inlet_off[b+1] = length(inlet_ghost)
+    end
+    cen = vec(permutedims(m.cell_centroids[:, 1:2]))
+    return (; N, off, nbr, nx, ny, len, area = Vector{Float64}(m.cell_areas), cen, G, gslot, gkind, gbc,
+              nI = bcs.nInletQ_BCs, nE = bcs.nExitH_BCs, inlet_off, inlet_ghost = isempty(inlet_ghost) ? Int32[0] : inlet_ghost)
+end
+
+const CONTEXTS = IdDict{Any,Context}()
+
+# One context per mesh object (objectid of p_extra.my_mesh_2D).
+function context(p)
+    get!(CONTEXTS, p.my_mesh_2D) do
+        d = flatten(p.my_mesh_2D, p.boundary_conditions)
+        c = p.swe_2D_constants
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve d begin
+            mesh = Ref(HgMesh(d.N, pointer(d.off), pointer(d.nbr), pointer(d.nx), pointer(d.ny), pointer(d.len),
+                              pointer(d.area), pointer(d.cen)))
+            bc = Ref(HgBcs(d.G, pointer(d.gslot), pointer(d.gkind), pointer(d.gbc), d.nI, d.nE,
+                           pointer(d.inlet_off), pointer(d.inlet_ghost)))
+            k = Ref(HgConsts(c.g, c.k_n, c.h_small))
+            rc = ccall((:hg_create, libhgswe), Cint, (Ref{Ptr{Cvoid}}, Ref{HgMesh}, Ref{HgBcs}, Ref{HgConsts}, Int32),
+                       out, mesh, bc, k, Int32(1))
+            check(C_NULL, rc)
+        end
+        ctx = Context(out[], d.N, HG_PARAM_NONE)
+        finalizer(x -> ccall((:hg_destroy, libhgswe), Cint, (Ptr{Cvoid},), x.ptr), ctx)
+        S0 = p.S0_cells
+        rc = ccall((:hg_set_fields, libhgswe), Cint,
+                   (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                   ctx.ptr, Vector{Float64}(p.hstill), Vector{Float64}(p.hstill_ghostCells), Vector{Float64}(p.zb_cells),
+                   Vector{Float64}(p.zb_ghostCells), Vector{Float64}(S0[:, 1]), Vector{Float64}(S0[:, 2]),
+                   Vector{Float64}(p.ManningN_cells), Vector{Float64}(p.inletQ_TotalQ), Vector{Float64}(p.exitH_WSE))
+        check(ctx.ptr, rc)
+        set_active!(ctx, p)
+        ctx
+    end
+end
+
+# active_param_name -> HG_PARAM_* (semi_discretize_swe_2D.jl:114-199)
+function set_active!(ctx, p)
+    s = p.settings
+    inv = s.bPerform_Inversion || s.bPerform_Sensitivity_Analysis
+    kind, zone, nz = HG_PARAM_NONE, Int32[0], Int32(0)
+    if inv && p.active_param_name == "zb"
+        kind = HG_PARAM_ZB
+    elseif inv && p.active_param_name == "ManningN"
+        kind = HG_PARAM_N_ZONE
+        zone = Int32.(p.srh_all_Dict["matID_cells"])
+        nz = Int32(maximum(zone) + 1)
+    elseif inv && p.active_param_name == "Q"
+        kind = HG_PARAM_Q
+    end
+    check(ctx.ptr, ccall((:hg_set_active_param, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Int32}, Int32),
+                         ctx.ptr, kind, zone, nz))
+    ctx.active = kind
+end
+
+nodual(x) = eltype(x) <: Float64 || error("HydrogradHIP: only Float64 arrays cross the C ABI (got $(eltype(x)))")
+
+function swe_2d_rhs(dQdt::AbstractVector, Q::AbstractVector, params_vector::AbstractVector, t::Float64, p_extra)
+    nodual(Q); nodual(params_vector)
+    ctx = context(p_extra)
+    q = Vector{Float64}(Q)
+    prm = Vector{Float64}(params_vector)
+    out = p_extra.bInPlaceODE ? dQdt : similar(q)
+    check(ctx.ptr, ccall((:hg_rhs, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}),
+                         ctx.ptr, q, prm, length(prm), out))
+    return out
+end
+
+# The ODEFunction closure of solve_swe_2D.jl:281-294, backed by the device.
+struct DeviceRHS{P}
+    p::P
+end
+(f::DeviceRHS)(u, p, t) = swe_2d_rhs(similar(u), u, p, t, f.p)
+make_ode_f(p_extra) = DeviceRHS(p_extra)
+
+function custom_ODE_update_cells(ode_f::DeviceRHS, Q, params_vector, t, dt, swe2d_extra_params)
+    nodual(Q); nodual(params_vector)
+    ctx = context(swe2d_extra_params)
+    q = Vector{Float64}(Q)
+    prm = Vector{Float64}(params_vector)
+    check(ctx.ptr, ccall((:hg_step, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64),
+                         ctx.ptr, q, prm, length(prm), dt, 1))
+    return q
+end
+
+# Same saves as the reference (one column per step, custom_ODE_solvers.jl:61-86),
+# the steps themselves run on the device.
+function custom_ODE_solve(ode_f::DeviceRHS, Q0, params_vector, swe2d_extra_params)
+    c = swe2d_extra_params.swe_2D_constants
+    steps = c.tspan[1]:c.dt:c.tspan[2]
+    ctx = context(swe2d_extra_params)
+    prm = Vector{Float64}(params_vector)
+    check(ctx.ptr, ccall((:hg_set_params, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx.ptr, prm, length(prm)))
+    check(ctx.ptr, ccall((:hg_state_upload, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, Vector{Float64}(Q0)))
+    sol = Matrix{Float64}(undef, length(Q0), length(steps))
+    for k in eachindex(steps)
+        check(ctx.ptr, ccall((:hg_advance, libhgswe), Cint, (Ptr{Cvoid}, Float64, Int64), ctx.ptr, c.dt, 1))
+        check(ctx.ptr, ccall((:hg_state_download, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, view(sol, :, k)))
+    end
+    return sol
+end
+
+end # module

@@ -1,0 +1,159 @@
+/* hgswe.h -- C ABI of libhgswe.so, the MI355X (gfx950) implementation of
+ * Hydrograd.jl's 2-D shallow-water right-hand side, explicit Euler step and
+ * their reverse-mode adjoint.
+ *
+ * Conventions
+ *  - All indices are 0-based; cells are numbered in the caller's order (the
+ *    library may renumber internally for locality and always returns results
+ *    in the caller's order).
+ *  - State vectors are block SoA of length 3N: [xi(0..N-1), q_x(..), q_y(..)],
+ *    exactly like Hydrograd's Q (semi_discretize_swe_2D.jl:93-95).
+ *  - Every entry point returns HG_OK (0) or an error code; hg_last_error()
+ *    gives the message.  The shim turns a nonzero code into a Julia
+ *    error(), mirroring the reference's exceptions.
+ *  - A context owns one HIP stream and is not thread-safe: use one context
+ *    per caller thread.
+ *  - Host-pointer entry points (hg_rhs, hg_step, hg_grad, ...) are
+ *    synchronous: they return after results are in host memory.
+ */
+#ifndef HGSWE_H
+#define HGSWE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hg_ctx hg_ctx;
+
+enum {
+  HG_OK = 0,
+  HG_ERR_ARG = 1,         /* invalid argument / inconsistent array lengths */
+  HG_ERR_HIP = 2,         /* HIP runtime error */
+  HG_ERR_MESH = 3,        /* malformed mesh (bad CSR, ghost map, clockwise cell) */
+  HG_ERR_CONVEYANCE = 4,  /* inlet conveyance <= 1e-10 (bc_2D.jl:678-680) */
+  HG_ERR_NONFINITE = 5,   /* NaN / Inf in the state */
+  HG_ERR_STATE = 6        /* call order (e.g. no fields set) */
+};
+
+/* Boundary kinds (bc_2D.jl:176-239). */
+enum { HG_BC_INLETQ = 0, HG_BC_EXITH = 1, HG_BC_WALL = 2, HG_BC_SYMM = 3 };
+
+/* Active parameter of params_vector (semi_discretize_swe_2D.jl:114-199). */
+enum {
+  HG_PARAM_NONE = 0,   /* params ignored (forward simulation) */
+  HG_PARAM_ZB = 1,     /* params = zb per cell; bed data rebuilt (process_bed_2D.jl:46-66) */
+  HG_PARAM_N_ZONE = 2, /* params = Manning n per zone; n_cell = params[zone[i]] (process_ManningN_2D.jl:71-98) */
+  HG_PARAM_N_CELL = 3, /* params = Manning n per cell */
+  HG_PARAM_Q = 4       /* params = total discharge per inlet-Q boundary */
+};
+
+/* hg_create flags */
+enum { HG_REORDER = 1 }; /* renumber cells along a Morton curve (needs centroid_xy) */
+
+/* Mesh topology + geometry, flattened from mesh_2D (mesh_2D.jl:2-71).
+ * Cell i owns CSR slots face_off[i] .. face_off[i+1]-1, one per local face
+ * in the reference's order (edge node_k -> node_k+1); face_nbr is the
+ * neighbour cell or ~g for ghost g; face_nx/ny = cell_normals[i][k];
+ * face_len = face_lengths[edge]; area = cell_areas. */
+typedef struct {
+  int64_t n_cells;
+  const int32_t* face_off;    /* [n_cells+1] */
+  const int32_t* face_nbr;    /* [face_off[n_cells]] */
+  const double* face_nx;      /* [face_off[n_cells]] */
+  const double* face_ny;
+  const double* face_len;
+  const double* area;         /* [n_cells] */
+  const double* centroid_xy;  /* [2*n_cells] or NULL; only used by HG_REORDER */
+} hg_mesh;
+
+/* Boundary data, flattened from BoundaryConditions2D (bc_2D.jl:3-47).
+ * Ghost g sits across CSR slot ghost_slot[g] of its internal cell; its
+ * outward normal and length are that slot's. */
+typedef struct {
+  int64_t n_ghosts;
+  const int32_t* ghost_slot;  /* [n_ghosts] */
+  const int32_t* ghost_kind;  /* [n_ghosts] HG_BC_* */
+  const int32_t* ghost_bc;    /* [n_ghosts] inlet / exit boundary index, -1 for wall / symm */
+  int32_t n_inlet;
+  int32_t n_exit;
+  const int32_t* inlet_off;   /* [n_inlet+1] */
+  const int32_t* inlet_ghost; /* ghosts of each inlet-Q boundary in NodeString order */
+} hg_bcs;
+
+/* swe_2D_consts (swe_2D_constants.jl:4-18): the scalars the RHS reads. */
+typedef struct {
+  double g;
+  double k_n;
+  double h_small;
+} hg_consts;
+
+/* ------------------------------------------------------------------ set-up */
+
+/* Build a context: validates and copies the mesh and boundary data to the
+ * device (the caller may free its arrays afterwards).  Replaces the set-up
+ * of SWE2D_Extra_Parameters (solve_swe_2D.jl:240-277,
+ * application_commons.jl:7-44). */
+int hg_create(hg_ctx** out, const hg_mesh* mesh, const hg_bcs* bcs, const hg_consts* consts, int32_t flags);
+
+int hg_destroy(hg_ctx* ctx);
+
+/* Message of the last error on this context (or of the last failed
+ * hg_create when ctx is NULL). */
+const char* hg_last_error(const hg_ctx* ctx);
+
+/* Per-cell / per-ghost fields of SWE2D_Extra_Parameters
+ * (application_commons.jl:7-44): hstill, hstill_ghostCells, zb_cells,
+ * zb_ghostCells, S0_cells (x and y columns), ManningN_cells,
+ * inletQ_TotalQ, exitH_WSE. */
+int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost, const double* zb,
+                  const double* zb_ghost, const double* S0x, const double* S0y, const double* n,
+                  const double* inletQ, const double* exitWSE);
+
+/* Selects how params_vector is read (semi_discretize_swe_2D.jl:114-199).
+ * zone_of_cell is required for HG_PARAM_N_ZONE (matID_cells, 0 = default
+ * zone; process_SRH_2D_input.jl:136-153). */
+int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, int32_t n_zones);
+
+/* ------------------------------------------------------------ forward path */
+
+/* dQdt = swe_2d_rhs(Q, params) (semi_discretize_swe_2D.jl:18-277).
+ * Host arrays of length 3N; params has n_params entries per the active
+ * parameter (ignored for HG_PARAM_NONE).  Synchronous. */
+int hg_rhs(hg_ctx* ctx, const double* Q, const double* params, int64_t n_params, double* dQdt);
+
+/* nsteps calls of custom_ODE_update_cells (custom_ODE_solvers.jl:5-33):
+ * Q <- Q + dt * RHS(Q), then xi < h_small => xi = h_small, q = 0.
+ * Q is a host array (3N) updated in place; the steps run device-resident
+ * with no per-step host traffic (custom_ODE_solve, :36-95, minus its
+ * per-step prints and saves). */
+int hg_step(hg_ctx* ctx, double* Q, const double* params, int64_t n_params, double dt, int64_t nsteps);
+
+/* Device-resident state: upload / download the 3N state and advance it
+ * nsteps with fixed dt (hipGraph-replayed).  Uses the parameters of the
+ * last hg_rhs / hg_step / hg_set_params call. */
+int hg_set_params(hg_ctx* ctx, const double* params, int64_t n_params);
+int hg_state_upload(hg_ctx* ctx, const double* Q);
+int hg_state_download(hg_ctx* ctx, double* Q);
+int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps);
+
+/* CFL time step: dt = cfl * min over wet cells of (2A/P)/(|u| + sqrt(g h)).
+ * No reference counterpart (the reference's CFL field is unused,
+ * custom_ODE_solvers.jl:39). */
+int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
+
+/* Timing helper: runs hg_advance(dt, nsteps) `reps` times on the device
+ * state and reports the mean time per step measured with HIP events on the
+ * context's stream, plus the mean duration of the step kernel alone. */
+int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
+                    double* ms_step_kernel);
+
+/* Library / device information string (arch, CU count, build flags). */
+const char* hg_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HGSWE_H */
