@@ -59,6 +59,56 @@ struct InletArgs {
   int32_t* flag;  // set to 1 when a conveyance is <= 1e-10
 };
 
+// Arguments of the tiled (face-once) step kernel.  Cells are split into
+// tiles of `T` consecutive (Morton-ordered) cells, one work-group per tile.
+// Work items of tile t are item_off[t] .. item_off[t+1]-1:
+//   code bits  0..9  a  (local cell of the face's first side)
+//              10..13 ka (local face index of that side)
+//              14..23 b  (local cell of the second side, intra-tile items)
+//              24..27 kb
+//              28..29 type: 0 intra-tile (both sides), 1 cross-tile
+//                     (first side only; ref = global neighbour), 2 ghost
+//                     (ref = ghost id)
+struct TileArgs {
+  int32_t N;
+  int32_t T;
+  int32_t maxdeg;
+  int32_t n_inlet;  // inlets whose conveyance the kernel forms (0: done by a separate kernel)
+  const double* xi;
+  const double* qx;
+  const double* qy;
+  double* xo;
+  double* qxo;
+  double* qyo;
+  const double* hstill;
+  const double* zb;
+  const double* n;
+  const double* S0x;
+  const double* S0y;
+  const double* area;
+  const int32_t* deg;
+  const int32_t* item_off;
+  const int32_t* item_code;
+  const int32_t* item_ref;
+  const double* item_nx;
+  const double* item_ny;
+  const double* item_len;
+  const uint32_t* tile_inlets;  // bit b set: tile has ghosts of inlet b
+  const int32_t* g_kind;
+  const int32_t* g_bc;
+  const double* g_hstill;
+  const double* g_zb;
+  const double* g_L23;
+  const double* inletQ;
+  const double* exitWSE;
+  const int32_t* in_off;
+  const int32_t* in_cell;
+  const double* in_L53;
+  const double* totalA;  // used when n_inlet == 0 (conveyance from the separate kernel)
+  int32_t* flag;
+  double g, half_g, kn2, hsmall, dt;
+};
+
 template <class T>
 struct DBuf {  // owning device buffer
   T* p = nullptr;
@@ -98,6 +148,13 @@ struct hg_ctx {
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
   int32_t* zone = nullptr;
+  // tiled kernel data
+  bool tiled = false;
+  int32_t tileT = 256, ntiles = 0;
+  bool fuse_inlet = false;
+  int32_t *item_off = nullptr, *item_code = nullptr, *item_ref = nullptr;
+  double *item_nx = nullptr, *item_ny = nullptr, *item_len = nullptr;
+  uint32_t* tile_inlets = nullptr;
   double* scratch = nullptr;  // 3N doubles for host transfers / params
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials

@@ -26,6 +26,7 @@
 
 #include "hg_ctx.hpp"
 #include "hg_device.hpp"
+#include "hg_roe_pair.hpp"
 #include "hgswe.h"
 
 using hg::Side;
@@ -203,6 +204,187 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
   }
 }
 
+// Tiled face-once step (production path).  One work-group per tile of T
+// consecutive cells:
+//  1. every cell's state and derived quantities (u, v, sqrt(h+eps),
+//     pressure) go to LDS; tiles that own inlet-Q ghosts also form the
+//     inlet conveyance (sequential left fold over the inlet's faces);
+//  2. work items: an intra-tile face is evaluated once and yields both
+//     sides' fluxes (roe_pair<true>, bit-identical to the per-side
+//     evaluation); cross-tile and boundary faces are evaluated from the
+//     tile's side only; each result F * L lands in the side's LDS slot;
+//  3. each cell sums its slots in the reference's local face order, adds
+//     the source terms and applies the Euler update and dry clamp.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_swe_tile(hg::TileArgs a) {
+  extern __shared__ double lds[];
+  const int T = a.T;
+  const int md = a.maxdeg;
+  const int t = blockIdx.x;
+  const int li = threadIdx.x;
+  const int i = t * T + li;
+  const bool valid = (li < T) && (i < a.N);
+  double* C = lds;               // 10 x T cell quantities
+  double* FL = lds + 10 * T;     // 3 x maxdeg x T face contributions
+  double* tA = FL + 3 * md * T;  // conveyance per inlet (32)
+  const double hsmall = a.hsmall;
+
+  double xi = 0.0, qx0 = 0.0, qy0 = 0.0;
+  hg::CellQ me{};
+  if (valid) {
+    xi = a.xi[i];
+    qx0 = a.qx[i];
+    qy0 = a.qy[i];
+    me = hg::make_cellq(xi, a.hstill[i], qx0, qy0, a.zb[i], a.half_g, hsmall);
+    C[0 * T + li] = me.xi; C[1 * T + li] = me.hs; C[2 * T + li] = me.h; C[3 * T + li] = me.hu;
+    C[4 * T + li] = me.hv; C[5 * T + li] = me.zb; C[6 * T + li] = me.u; C[7 * T + li] = me.v;
+    C[8 * T + li] = me.s; C[9 * T + li] = me.p;
+  }
+  if (a.n_inlet > 0 && li < 64) {  // wave 0: inlet conveyance (bc_2D.jl:640-680)
+    uint32_t mask = a.tile_inlets[t];
+    while (mask) {
+      const int b = __ffs(mask) - 1;
+      mask &= mask - 1;
+      const int f0 = a.in_off[b], f1 = a.in_off[b + 1];
+      double acc = 0.0;
+      for (int base = f0; base < f1; base += 64) {
+        const int f = base + li;
+        double term = 0.0;
+        if (f < f1) {
+          const int c = a.in_cell[f];
+          double h = a.xi[c] + a.hstill[c];
+          if (h <= hsmall) h = hsmall;
+          const double w = (h > hsmall) ? 1.0 : 0.0;
+          term = a.in_L53[f] * h / a.n[c] * w;
+        }
+        const int cnt = min(64, f1 - base);
+        for (int k = 0; k < cnt; ++k) {
+          const double tk = __shfl(term, k, 64);
+          acc = (base + k == f0) ? tk : acc + tk;
+        }
+      }
+      if (li == 0) {
+        tA[b] = acc;
+        if (!(acc > 1e-10)) a.flag[0] = 1;
+      }
+    }
+  }
+  __syncthreads();
+
+  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
+  for (int it = it0 + li; it < it1; it += blockDim.x) {
+    const int code = a.item_code[it];
+    const int ia = code & 1023;
+    const int ka = (code >> 10) & 15;
+    const int type = (code >> 28) & 3;
+    const double nx = a.item_nx[it], ny = a.item_ny[it], L = a.item_len[it];
+    hg::CellQ A;
+    A.xi = C[0 * T + ia]; A.hs = C[1 * T + ia]; A.h = C[2 * T + ia]; A.hu = C[3 * T + ia];
+    A.hv = C[4 * T + ia]; A.zb = C[5 * T + ia]; A.u = C[6 * T + ia]; A.v = C[7 * T + ia];
+    A.s = C[8 * T + ia]; A.p = C[9 * T + ia];
+    double fa[3], fb[3];
+    if (type == 0) {
+      const int ib = (code >> 14) & 1023;
+      const int kb = (code >> 24) & 15;
+      hg::CellQ B;
+      B.xi = C[0 * T + ib]; B.hs = C[1 * T + ib]; B.h = C[2 * T + ib]; B.hu = C[3 * T + ib];
+      B.hv = C[4 * T + ib]; B.zb = C[5 * T + ib]; B.u = C[6 * T + ib]; B.v = C[7 * T + ib];
+      B.s = C[8 * T + ib]; B.p = C[9 * T + ib];
+      hg::roe_pair<true>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
+      for (int c = 0; c < 3; ++c) FL[(c * md + kb) * T + ib] = fb[c] * L;
+    } else if (type == 1) {
+      const int j = a.item_ref[it];
+      const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
+      hg::roe_pair<false>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
+    } else {  // ghost (bc_2D.jl:640-834); its internal cell is A
+      const int gi = a.item_ref[it];
+      double hg_, qxg, qyg;
+      switch (a.g_kind[gi]) {
+        case HG_BC_INLETQ: {
+          const int b = a.g_bc[gi];
+          const double w = (A.h > hsmall) ? 1.0 : 0.0;
+          const double tot = (a.n_inlet > 0) ? tA[b] : a.totalA[b];
+          const double v = a.inletQ[b] / tot * a.g_L23[gi] / a.n[t * T + ia];
+          hg_ = A.h;
+          qxg = -A.h * v * nx * w;
+          qyg = -A.h * v * ny * w;
+          break;
+        }
+        case HG_BC_EXITH: {
+          const double hx = a.exitWSE[a.g_bc[gi]] - A.zb;
+          hg_ = (hx > hsmall) ? hx : hsmall;
+          qxg = A.hu;
+          qyg = A.hv;
+          break;
+        }
+        case HG_BC_WALL:
+          hg_ = A.h;
+          qxg = -A.hu;
+          qyg = -A.hv;
+          break;
+        default: {
+          const double vdn = A.hu * nx + A.hv * ny;
+          hg_ = A.h;
+          qxg = A.hu - 2.0 * vdn * nx;
+          qyg = A.hv - 2.0 * vdn * ny;
+          break;
+        }
+      }
+      hg::CellQ B;
+      B.hs = a.g_hstill[gi];
+      B.xi = hg_ - B.hs;
+      B.h = hg_;
+      B.hu = qxg;
+      B.hv = qyg;
+      B.zb = a.g_zb[gi];
+      B.u = qxg / hg_;
+      B.v = qyg / hg_;
+      B.s = sqrt(hg_ + hg::EPS);
+      B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
+      hg::roe_pair<false>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
+    }
+    for (int c = 0; c < 3; ++c) FL[(c * md + ka) * T + ia] = fa[c] * L;
+  }
+  __syncthreads();
+
+  if (!valid) return;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  const int d = a.deg[i];
+  for (int k = 0; k < d; ++k) {  // local face order (semi_discretize_swe_2D.jl:293-387)
+    acc0 = acc0 + FL[(0 * md + k) * T + li];
+    acc1 = acc1 + FL[(1 * md + k) * T + li];
+    acc2 = acc2 + FL[(2 * md + k) * T + li];
+  }
+  const double Ar = a.area[i];
+  const double r0 = -acc0 / Ar, r1 = -acc1 / Ar, r2 = -acc2 / Ar;
+  double s1 = 0.0, s2 = 0.0;
+  const double h = me.h;
+  if (h > hsmall) {
+    double fx, fy;
+    hg::friction(a.g, a.kn2, hsmall, h, me.hu, me.hv, a.n[i], fx, fy);
+    s1 = a.g * xi * a.S0x[i] - fx;
+    s2 = a.g * xi * a.S0y[i] - fy;
+  }
+  const double d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;
+  if (STEP) {
+    double xn = xi + a.dt * d0;
+    double qxn = qx0 + a.dt * d1;
+    double qyn = qy0 + a.dt * d2;
+    if (xn < hsmall) {
+      xn = hsmall;
+      qxn = 0.0;
+      qyn = 0.0;
+    }
+    a.xo[i] = xn;
+    a.qxo[i] = qxn;
+    a.qyo[i] = qyn;
+  } else {
+    a.xo[i] = d0;
+    a.qxo[i] = d1;
+    a.qyo[i] = d2;
+  }
+}
+
 // update_bed_data (process_bed_2D.jl:46-66): zb_ghost and S0 from zb.
 __global__ __launch_bounds__(kBlock) void k_bed_data(int N, int G, const double* zb, const int32_t* deg,
                                                     const int32_t* nbr, const double* fnx, const double* fny,
@@ -292,6 +474,120 @@ std::vector<int32_t> morton_perm(int64_t N, const double* xy) {
   return perm;
 }
 
+// Work items of the tiled kernel (see TileArgs): per tile, intra-tile
+// faces first (owned by the side with the lower local index), then
+// cross-tile faces, then boundary faces, each group ordered by cell.
+struct TileBuild {
+  std::vector<int32_t> off, code, ref;
+  std::vector<double> nx, ny, len;
+  std::vector<uint32_t> inlets;
+};
+
+TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const std::vector<int32_t>& nbr,
+                      const std::vector<double>& fnx, const std::vector<double>& fny, const std::vector<double>& flen,
+                      const std::vector<int32_t>& g_kind, const std::vector<int32_t>& g_bc) {
+  TileBuild tb;
+  const int64_t ntiles = (N + T - 1) / T;
+  tb.off.assign(ntiles + 1, 0);
+  tb.inlets.assign(ntiles, 0u);
+  std::vector<int64_t> grp[3];
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int64_t c0 = t * T, c1 = std::min<int64_t>(N, c0 + T);
+    for (auto& v : grp) v.clear();
+    for (int64_t i = c0; i < c1; ++i) {
+      const int32_t li = int32_t(i - c0);
+      for (int32_t k = 0; k < deg[i]; ++k) {
+        const int64_t s = int64_t(k) * N + i;
+        const int32_t j = nbr[s];
+        int32_t code, ref = 0, type;
+        if (j < 0) {
+          type = 2;
+          ref = ~j;
+          code = li | (k << 10) | (type << 28);
+          if (g_kind[ref] == HG_BC_INLETQ && g_bc[ref] < 32) tb.inlets[t] |= (1u << g_bc[ref]);
+        } else if (j >= c0 && j < c1) {
+          const int32_t lj = int32_t(j - c0);
+          if (lj < li) continue;  // owned by the other side
+          int32_t kb = -1;
+          for (int32_t k2 = 0; k2 < deg[j]; ++k2)
+            if (nbr[int64_t(k2) * N + j] == int32_t(i)) kb = k2;
+          type = 0;
+          code = li | (k << 10) | (lj << 14) | (kb << 24) | (type << 28);
+        } else {
+          type = 1;
+          ref = j;
+          code = li | (k << 10) | (type << 28);
+        }
+        grp[type].push_back(int64_t(code) | (int64_t(uint32_t(ref)) << 32));
+        (void)s;
+      }
+    }
+    for (int type = 0; type < 3; ++type)
+      for (int64_t e : grp[type]) {
+        const int32_t code = int32_t(e & 0xffffffff);
+        const int32_t ref = int32_t(uint32_t(uint64_t(e) >> 32));
+        const int32_t li = code & 1023, k = (code >> 10) & 15;
+        const int64_t s = int64_t(k) * N + c0 + li;
+        tb.code.push_back(code);
+        tb.ref.push_back(ref);
+        tb.nx.push_back(fnx[s]);
+        tb.ny.push_back(fny[s]);
+        tb.len.push_back(flen[s]);
+      }
+    tb.off[t + 1] = int32_t(tb.code.size());
+  }
+  return tb;
+}
+
+size_t tile_lds_bytes(int T, int maxdeg) { return size_t(10 * T + 3 * maxdeg * T + 32) * sizeof(double); }
+
+hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0 = nullptr, double* o1 = nullptr,
+                            double* o2 = nullptr) {
+  hg::TileArgs a{};
+  a.N = int32_t(ctx->N);
+  a.T = ctx->tileT;
+  a.maxdeg = ctx->maxdeg;
+  a.n_inlet = ctx->fuse_inlet ? ctx->n_inlet : 0;
+  a.xi = ctx->st[src][0];
+  a.qx = ctx->st[src][1];
+  a.qy = ctx->st[src][2];
+  a.xo = o0 ? o0 : ctx->st[dst][0];
+  a.qxo = o1 ? o1 : ctx->st[dst][1];
+  a.qyo = o2 ? o2 : ctx->st[dst][2];
+  a.hstill = ctx->hstill;
+  a.zb = ctx->zb;
+  a.n = ctx->n;
+  a.S0x = ctx->S0x;
+  a.S0y = ctx->S0y;
+  a.area = ctx->area;
+  a.deg = ctx->deg;
+  a.item_off = ctx->item_off;
+  a.item_code = ctx->item_code;
+  a.item_ref = ctx->item_ref;
+  a.item_nx = ctx->item_nx;
+  a.item_ny = ctx->item_ny;
+  a.item_len = ctx->item_len;
+  a.tile_inlets = ctx->tile_inlets;
+  a.g_kind = ctx->g_kind;
+  a.g_bc = ctx->g_bc;
+  a.g_hstill = ctx->g_hstill;
+  a.g_zb = ctx->g_zb;
+  a.g_L23 = ctx->g_L23;
+  a.inletQ = ctx->inletQ;
+  a.exitWSE = ctx->exitWSE;
+  a.in_off = ctx->in_off;
+  a.in_cell = ctx->in_cell;
+  a.in_L53 = ctx->in_L53;
+  a.totalA = ctx->totalA;
+  a.flag = ctx->flags;
+  a.g = ctx->c.g;
+  a.half_g = 0.5 * ctx->c.g;
+  a.kn2 = ctx->c.k_n * ctx->c.k_n;
+  a.hsmall = ctx->c.h_small;
+  a.dt = dt;
+  return a;
+}
+
 StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = nullptr, double* out1 = nullptr,
                    double* out2 = nullptr) {
   StepArgs a{};
@@ -350,6 +646,18 @@ int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
 // Enqueue one step (or one RHS evaluation) reading state `src`.
 int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
                  double* o2 = nullptr) {
+  if (ctx->tiled) {
+    if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
+      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
+    hg::TileArgs a = make_tile_args(ctx, src, dst, dt, o0, o1, o2);
+    const size_t lds = tile_lds_bytes(ctx->tileT, ctx->maxdeg);
+    if (step)
+      hipLaunchKernelGGL(k_swe_tile<true>, dim3(ctx->ntiles), dim3(ctx->tileT), lds, ctx->stream, a);
+    else
+      hipLaunchKernelGGL(k_swe_tile<false>, dim3(ctx->ntiles), dim3(ctx->tileT), lds, ctx->stream, a);
+    HG_TRY(hipGetLastError());
+    return HG_OK;
+  }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
   StepArgs a = make_args(ctx, src, dst, dt, o0, o1, o2);
@@ -578,8 +886,10 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
       if (j >= 0) {
         bool back = false;
         if (j >= N || j == i) { g_create_err = "face_nbr out of range at slot " + std::to_string(s); return HG_ERR_MESH; }
-        for (int32_t t = m->face_off[j]; t < m->face_off[j + 1]; ++t) back |= (m->face_nbr[t] == i);
-        if (!back) { g_create_err = "neighbour relation not symmetric at slot " + std::to_string(s); return HG_ERR_MESH; }
+        int nback = 0;
+        for (int32_t t = m->face_off[j]; t < m->face_off[j + 1]; ++t) nback += (m->face_nbr[t] == i);
+        back = nback == 1;
+        if (!back) { g_create_err = "neighbour relation not symmetric (or cells share several faces) at slot " + std::to_string(s); return HG_ERR_MESH; }
       } else {
         const int64_t gi = ~int64_t(j);
         if (gi >= G || ghost_seen[gi] >= 0) { g_create_err = "bad ghost reference at slot " + std::to_string(s); return HG_ERR_MESH; }
@@ -720,6 +1030,58 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
   if (!rc) rc = upload(ctx, ctx->in_L53, in_L53.data(), ctx->n_inlet_faces);
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
+
+  // ---- tiled face-once kernel data
+  ctx->tiled = !(flags & HG_CELL_KERNEL);
+  if (ctx->tiled) {
+    int T = 256;
+    while (T > 64 && tile_lds_bytes(T, maxdeg) > 64 * 1024) T /= 2;
+    ctx->tileT = T;
+    ctx->ntiles = int32_t((N + T - 1) / T);
+    ctx->fuse_inlet = b->n_inlet > 0 && b->n_inlet <= 32;
+    TileBuild tb = build_tiles(N, T, h_deg, h_nbr, h_nx, h_ny, h_len, g_kind, g_bc);
+    const size_t ni = tb.code.size();
+    {  // every (cell, local face) slot must receive exactly one flux
+      std::vector<uint8_t> hits(size_t(maxdeg) * N, 0);
+      bool ok = true;
+      for (int64_t t = 0; t < ctx->ntiles && ok; ++t)
+        for (int32_t it = tb.off[t]; it < tb.off[t + 1]; ++it) {
+          const int32_t code = tb.code[it];
+          const int32_t ia = code & 1023, ka = (code >> 10) & 15, ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
+          const int32_t type = (code >> 28) & 3;
+          const int64_t ca = t * T + ia;
+          if (ca >= N || ka >= h_deg[ca]) { ok = false; break; }
+          hits[size_t(ka) * N + ca]++;
+          if (type == 0) {
+            const int64_t cb = t * T + ib;
+            if (cb >= N || kb >= h_deg[cb]) { ok = false; break; }
+            hits[size_t(kb) * N + cb]++;
+          }
+        }
+      for (int64_t i = 0; i < N && ok; ++i)
+        for (int32_t k = 0; k < maxdeg; ++k)
+          if (hits[size_t(k) * N + i] != (k < h_deg[i] ? 1 : 0)) { ok = false; break; }
+      if (!ok) {
+        ctx->err = "internal: tiled work items do not cover every face slot exactly once";
+        return fail(HG_ERR_MESH);
+      }
+    }
+    if (!rc) rc = dalloc(ctx, &ctx->item_off, tb.off.size());
+    if (!rc) rc = dalloc(ctx, &ctx->item_code, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->item_ref, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->item_nx, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->item_ny, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->item_len, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->tile_inlets, tb.inlets.size());
+    if (!rc) rc = upload(ctx, ctx->item_off, tb.off.data(), tb.off.size());
+    if (!rc) rc = upload(ctx, ctx->item_code, tb.code.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->item_ref, tb.ref.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->item_nx, tb.nx.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->item_ny, tb.ny.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->item_len, tb.len.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->tile_inlets, tb.inlets.data(), tb.inlets.size());
+    if (rc) return fail(rc);
+  }
   *out = ctx;
   return HG_OK;
 }
@@ -885,8 +1247,14 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
     const int K = 50;
     HG_TRY(hipEventRecord(e0, ctx->stream));
     for (int k = 0; k < K; ++k) {
-      StepArgs a = make_args(ctx, k & 1, (k + 1) & 1, dt);
-      hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+      if (ctx->tiled) {
+        hg::TileArgs a = make_tile_args(ctx, k & 1, (k + 1) & 1, dt);
+        hipLaunchKernelGGL(k_swe_tile<true>, dim3(ctx->ntiles), dim3(ctx->tileT),
+                           tile_lds_bytes(ctx->tileT, ctx->maxdeg), ctx->stream, a);
+      } else {
+        StepArgs a = make_args(ctx, k & 1, (k + 1) & 1, dt);
+        hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+      }
     }
     HG_TRY(hipEventRecord(e1, ctx->stream));
     HG_TRY(hipEventSynchronize(e1));

@@ -18,6 +18,7 @@ _LIB = None
 HG_OK, HG_ERR_ARG, HG_ERR_HIP, HG_ERR_MESH, HG_ERR_CONVEYANCE, HG_ERR_NONFINITE, HG_ERR_STATE = range(7)
 HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = range(5)
 HG_REORDER = 1
+HG_CELL_KERNEL = 2
 
 # every function declared in include/hgswe.h
 EXPORTS = [
@@ -84,7 +85,7 @@ def _ptr(a):
 class Context:
     """Owns one hg_ctx (one HIP stream) built from a ``cases.Case``."""
 
-    def __init__(self, case, reorder=True):
+    def __init__(self, case, reorder=True, cell_kernel=False):
         L = lib()
         m = case.mesh
         self.case = case
@@ -105,7 +106,8 @@ class Context:
                      m.n_inlet, m.n_exit, k(m.inlet_off, np.int32), k(m.inlet_ghost, np.int32))
         consts = hg_consts(case.g, case.k_n, case.h_small)
         h = C.c_void_p()
-        rc = L.hg_create(C.byref(h), C.byref(mesh), C.byref(bcs), C.byref(consts), HG_REORDER if reorder else 0)
+        flags = (HG_REORDER if reorder else 0) | (HG_CELL_KERNEL if cell_kernel else 0)
+        rc = L.hg_create(C.byref(h), C.byref(mesh), C.byref(bcs), C.byref(consts), flags)
         if rc:
             raise HGError(rc, L.hg_last_error(None).decode())
         self.h = h

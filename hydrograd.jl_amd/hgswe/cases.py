@@ -185,6 +185,11 @@ def channel(n_target=1_000_000, seed=0, Lx=10_000.0, Ly=1_000.0, Q_in=500.0, wse
     h0 = np.maximum(h_small, wse_out - zb)
     wstill = zb.copy() if wstill_level is None else np.full(N, float(wstill_level))
     wse = zb + h0
+    for _ in range(4):  # (zb + h_small) - zb may round below h_small; the IC check rejects that
+        low = (wse - zb) < h_small
+        if not low.any():
+            break
+        wse = np.where(low, np.nextafter(wse, np.inf), wse)
     n_in = sum(1 for t in bcs.values() if t.upper() == "INLET-Q")
     n_ex = sum(1 for t in bcs.values() if t.upper() == "EXIT-H")
     case = _finish(name or f"channel_{N}", mesh, zb, n, wse, wstill, np.zeros(N), np.zeros(N), [Q_in] * n_in,

@@ -50,7 +50,12 @@ enum {
 };
 
 /* hg_create flags */
-enum { HG_REORDER = 1 }; /* renumber cells along a Morton curve (needs centroid_xy) */
+enum {
+  HG_REORDER = 1,    /* renumber cells along a Morton curve (needs centroid_xy) */
+  HG_CELL_KERNEL = 2 /* one work-item per cell evaluating every face from its side, instead of the
+                        default tiled kernel that evaluates intra-tile faces once for both sides
+                        (both are bit-identical to the reference's per-side evaluation) */
+};
 
 /* Mesh topology + geometry, flattened from mesh_2D (mesh_2D.jl:2-71).
  * Cell i owns CSR slots face_off[i] .. face_off[i+1]-1, one per local face

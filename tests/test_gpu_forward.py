@@ -101,6 +101,23 @@ def test_reordering_is_result_neutral(channel20k):
     assert np.array_equal(a, b)
 
 
+def test_tiled_kernel_matches_cell_kernel(channel20k, tmp_path):
+    """The tiled face-once kernel and the one-work-item-per-cell kernel are
+    bit-identical (both reproduce the per-side evaluation exactly)."""
+    from conftest import load_golden_case
+    from test_oracle_golden import savannah_ic_path
+
+    sav = load_golden_case("savannah", ic=savannah_ic_path(tmp_path))
+    for c, dt in ((channel20k, channel20k.dt), (sav, 0.2 * cases.cfl_dt(sav, 1.0))):
+        a = capi.Context(c).step(c.Q0, dt, 40)
+        b = capi.Context(c, cell_kernel=True).step(c.Q0, dt, 40)
+        say(f"  tiled vs cell kernel {c.name}: identical={np.array_equal(a, b)}")
+        assert np.array_equal(a, b)
+        ra = capi.Context(c).rhs(a)
+        rb = capi.Context(c, cell_kernel=True).rhs(a)
+        assert np.array_equal(ra, rb)
+
+
 def test_deterministic_repeat(channel20k):
     c = channel20k
     ctx = capi.Context(c)
