@@ -31,7 +31,7 @@ def traffic_from_profiles():
     """HBM bytes per launch of the step kernel from the committed rocprof
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
     calibration), or None when not measured."""
-    p = os.path.join(ROOT, "profiles", "k_swe_step_channel1M_traffic.txt")
+    p = os.path.join(ROOT, "profiles", "k_swe_tile_channel1M_traffic.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed region, run one device-to-device copy of this many MiB "
+                         "(calibrates rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU domain decomposition "
@@ -109,8 +112,11 @@ def main():
                    "ghosts": case.G, "cfl": 0.5, "dt": case.dt, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_swe_step", "kernel_ms": ms_kernel, "bytes_per_cell_step": bpc},
+                     "kernel": "k_swe_tile", "kernel_ms": ms_kernel, "bytes_per_cell_step": bpc},
     }
+    if args.copy_mb > 0:  # one launch of a copy kernel of known traffic (calibrates rocprof counters)
+        nbytes = args.copy_mb * (1 << 20)
+        out["copy_probe"] = {"bytes_each_way": nbytes, "GBps": capi.probe_copy(ctx, nbytes, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -59,6 +59,13 @@ struct InletArgs {
   int32_t* flag;  // set to 1 when a conveyance is <= 1e-10
 };
 
+// One work item of the tiled kernel (32 bytes, one address per item).
+struct ItemRec {
+  int32_t code;
+  int32_t ref;
+  double nx, ny, len;
+};
+
 // Arguments of the tiled (face-once) step kernel.  Cells are split into
 // tiles of `T` consecutive (Morton-ordered) cells, one work-group per tile.
 // Work items of tile t are item_off[t] .. item_off[t+1]-1:
@@ -88,11 +95,7 @@ struct TileArgs {
   const double* area;
   const int32_t* deg;
   const int32_t* item_off;
-  const int32_t* item_code;
-  const int32_t* item_ref;
-  const double* item_nx;
-  const double* item_ny;
-  const double* item_len;
+  const struct ItemRec* items;
   const uint32_t* tile_inlets;  // bit b set: tile has ghosts of inlet b
   const int32_t* g_kind;
   const int32_t* g_bc;
@@ -152,14 +155,15 @@ struct hg_ctx {
   bool tiled = false;
   int32_t tileT = 256, ntiles = 0;
   bool fuse_inlet = false;
-  int32_t *item_off = nullptr, *item_code = nullptr, *item_ref = nullptr;
-  double *item_nx = nullptr, *item_ny = nullptr, *item_len = nullptr;
+  int32_t* item_off = nullptr;
+  hg::ItemRec* items = nullptr;
+  int32_t tileMD = 3;
   uint32_t* tile_inlets = nullptr;
   double* scratch = nullptr;  // 3N doubles for host transfers / params
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials
 
-  // cached graph of two steps (A->B->A) per dt
+  // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;
 

@@ -46,6 +46,8 @@ static std::string g_create_err;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kGraphSteps = 16;
+constexpr int kXCD = 8;  // MI355X: 8 accelerator complex dies, one L2 each  // steps per captured graph (even: the graph returns to buffer 0)
 
 template <class T>
 int dalloc(hg_ctx* ctx, T** p, size_t n) {
@@ -215,27 +217,45 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
 //     tile's side only; each result F * L lands in the side's LDS slot;
 //  3. each cell sums its slots in the reference's local face order, adds
 //     the source terms and applies the Euler update and dry clamp.
-template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_swe_tile(hg::TileArgs a) {
+// T (cells per tile) and MD (slots per cell) are compile-time constants so
+// every LDS address is a base register plus an immediate offset.
+template <bool STEP, int T, int MD>
+__global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
   extern __shared__ double lds[];
-  const int T = a.T;
-  const int md = a.maxdeg;
-  const int t = blockIdx.x;
+  // Work-groups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8);
+  // give each XCD a contiguous run of the Morton-ordered tiles so that
+  // cross-tile neighbours mostly sit in the same XCD's L2.  Bijection:
+  // nt = q * 8 + r, XCD x owns tiles [x q + min(x, r), ...) of length q + (x < r).
+  const int nt = gridDim.x;
+  const int q = nt / kXCD, r = nt % kXCD;
+  const int x = blockIdx.x % kXCD;
+  const int t = x * q + min(x, r) + blockIdx.x / kXCD;
   const int li = threadIdx.x;
   const int i = t * T + li;
-  const bool valid = (li < T) && (i < a.N);
+  const bool valid = i < a.N;
   double* C = lds;               // 10 x T cell quantities
-  double* FL = lds + 10 * T;     // 3 x maxdeg x T face contributions
-  double* tA = FL + 3 * md * T;  // conveyance per inlet (32)
+  double* FL = lds + 10 * T;     // 3 x MD x T face contributions
+  double* tA = FL + 3 * MD * T;  // conveyance per inlet (32)
   const double hsmall = a.hsmall;
 
-  double xi = 0.0, qx0 = 0.0, qy0 = 0.0;
-  hg::CellQ me{};
+  double xi = 0.0, qx0 = 0.0, qy0 = 0.0, hs_i = 0.0, zb_i = 0.0;
+  double n_i = 0.0, S0x_i = 0.0, S0y_i = 0.0, A_i = 1.0;
+  int d_i = 0;
   if (valid) {
     xi = a.xi[i];
     qx0 = a.qx[i];
     qy0 = a.qy[i];
-    me = hg::make_cellq(xi, a.hstill[i], qx0, qy0, a.zb[i], a.half_g, hsmall);
+    hs_i = a.hstill[i];
+    zb_i = a.zb[i];
+    n_i = a.n[i];
+    S0x_i = a.S0x[i];
+    S0y_i = a.S0y[i];
+    A_i = a.area[i];
+    d_i = a.deg[i];
+  }
+  hg::CellQ me{};
+  if (valid) {
+    me = hg::make_cellq(xi, hs_i, qx0, qy0, zb_i, a.half_g, hsmall);
     C[0 * T + li] = me.xi; C[1 * T + li] = me.hs; C[2 * T + li] = me.h; C[3 * T + li] = me.hu;
     C[4 * T + li] = me.hv; C[5 * T + li] = me.zb; C[6 * T + li] = me.u; C[7 * T + li] = me.v;
     C[8 * T + li] = me.s; C[9 * T + li] = me.p;
@@ -272,32 +292,31 @@ __global__ __launch_bounds__(kBlock) void k_swe_tile(hg::TileArgs a) {
   __syncthreads();
 
   const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
-  for (int it = it0 + li; it < it1; it += blockDim.x) {
-    const int code = a.item_code[it];
+  for (int it = it0 + li; it < it1; it += T) {
+    const hg::ItemRec rec = a.items[it];
+    const int code = rec.code;
     const int ia = code & 1023;
     const int ka = (code >> 10) & 15;
     const int type = (code >> 28) & 3;
-    const double nx = a.item_nx[it], ny = a.item_ny[it], L = a.item_len[it];
+    const double nx = rec.nx, ny = rec.ny, L = rec.len;
     hg::CellQ A;
     A.xi = C[0 * T + ia]; A.hs = C[1 * T + ia]; A.h = C[2 * T + ia]; A.hu = C[3 * T + ia];
     A.hv = C[4 * T + ia]; A.zb = C[5 * T + ia]; A.u = C[6 * T + ia]; A.v = C[7 * T + ia];
     A.s = C[8 * T + ia]; A.p = C[9 * T + ia];
-    double fa[3], fb[3];
+    // Only the right-hand state depends on the item type; the flux itself
+    // is one code path for all items (no divergent duplicates of it).
+    hg::CellQ B;
+    const int ib = (code >> 14) & 1023;
+    const int kb = (code >> 24) & 15;
     if (type == 0) {
-      const int ib = (code >> 14) & 1023;
-      const int kb = (code >> 24) & 15;
-      hg::CellQ B;
       B.xi = C[0 * T + ib]; B.hs = C[1 * T + ib]; B.h = C[2 * T + ib]; B.hu = C[3 * T + ib];
       B.hv = C[4 * T + ib]; B.zb = C[5 * T + ib]; B.u = C[6 * T + ib]; B.v = C[7 * T + ib];
       B.s = C[8 * T + ib]; B.p = C[9 * T + ib];
-      hg::roe_pair<true>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
-      for (int c = 0; c < 3; ++c) FL[(c * md + kb) * T + ib] = fb[c] * L;
     } else if (type == 1) {
-      const int j = a.item_ref[it];
-      const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
-      hg::roe_pair<false>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
+      const int j = rec.ref;
+      B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
     } else {  // ghost (bc_2D.jl:640-834); its internal cell is A
-      const int gi = a.item_ref[it];
+      const int gi = rec.ref;
       double hg_, qxg, qyg;
       switch (a.g_kind[gi]) {
         case HG_BC_INLETQ: {
@@ -330,7 +349,6 @@ __global__ __launch_bounds__(kBlock) void k_swe_tile(hg::TileArgs a) {
           break;
         }
       }
-      hg::CellQ B;
       B.hs = a.g_hstill[gi];
       B.xi = hg_ - B.hs;
       B.h = hg_;
@@ -341,29 +359,32 @@ __global__ __launch_bounds__(kBlock) void k_swe_tile(hg::TileArgs a) {
       B.v = qyg / hg_;
       B.s = sqrt(hg_ + hg::EPS);
       B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
-      hg::roe_pair<false>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
     }
-    for (int c = 0; c < 3; ++c) FL[(c * md + ka) * T + ia] = fa[c] * L;
+    double fa[3], fb[3];
+    hg::roe_pair<true>(A, B, nx, ny, a.g, a.half_g, hsmall, fa, fb);
+    if (type == 0)
+      for (int c = 0; c < 3; ++c) FL[(c * MD + kb) * T + ib] = fb[c] * L;
+    for (int c = 0; c < 3; ++c) FL[(c * MD + ka) * T + ia] = fa[c] * L;
   }
   __syncthreads();
 
   if (!valid) return;
   double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-  const int d = a.deg[i];
-  for (int k = 0; k < d; ++k) {  // local face order (semi_discretize_swe_2D.jl:293-387)
-    acc0 = acc0 + FL[(0 * md + k) * T + li];
-    acc1 = acc1 + FL[(1 * md + k) * T + li];
-    acc2 = acc2 + FL[(2 * md + k) * T + li];
+  for (int k = 0; k < MD; ++k) {  // local face order (semi_discretize_swe_2D.jl:293-387)
+    if (k < d_i) {
+      acc0 = acc0 + FL[(0 * MD + k) * T + li];
+      acc1 = acc1 + FL[(1 * MD + k) * T + li];
+      acc2 = acc2 + FL[(2 * MD + k) * T + li];
+    }
   }
-  const double Ar = a.area[i];
-  const double r0 = -acc0 / Ar, r1 = -acc1 / Ar, r2 = -acc2 / Ar;
+  const double r0 = -acc0 / A_i, r1 = -acc1 / A_i, r2 = -acc2 / A_i;
   double s1 = 0.0, s2 = 0.0;
   const double h = me.h;
   if (h > hsmall) {
     double fx, fy;
-    hg::friction(a.g, a.kn2, hsmall, h, me.hu, me.hv, a.n[i], fx, fy);
-    s1 = a.g * xi * a.S0x[i] - fx;
-    s2 = a.g * xi * a.S0y[i] - fy;
+    hg::friction(a.g, a.kn2, hsmall, h, me.hu, me.hv, n_i, fx, fy);
+    s1 = a.g * xi * S0x_i - fx;
+    s2 = a.g * xi * S0y_i - fy;
   }
   const double d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;
   if (STEP) {
@@ -438,6 +459,15 @@ __global__ __launch_bounds__(kBlock) void k_cfl(int N, const double* xi, const d
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Plain streaming copy (16 B per lane per iteration): the reference kernel
+// of known traffic used to calibrate rocprof's FETCH_SIZE / WRITE_SIZE and to
+// measure the achievable HBM bandwidth.
+__global__ __launch_bounds__(kBlock) void k_copy_probe(int64_t n2, const double2* __restrict__ src,
+                                                      double2* __restrict__ dst) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n2; k += int64_t(gridDim.x) * blockDim.x)
+    dst[k] = src[k];
+}
+
 // ------------------------------------------------------------ host helpers --
 
 uint64_t spread21(uint64_t v) {
@@ -474,9 +504,10 @@ std::vector<int32_t> morton_perm(int64_t N, const double* xy) {
   return perm;
 }
 
-// Work items of the tiled kernel (see TileArgs): per tile, intra-tile
-// faces first (owned by the side with the lower local index), then
-// cross-tile faces, then boundary faces, each group ordered by cell.
+// Work items of the tiled kernel (see TileArgs): per tile, cross-tile faces
+// first (the kernel prefetches their neighbour states before its first
+// barrier), then boundary faces, then intra-tile faces (owned by the side
+// with the lower local index), each group ordered by cell.
 struct TileBuild {
   std::vector<int32_t> off, code, ref;
   std::vector<double> nx, ny, len;
@@ -522,7 +553,7 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
         (void)s;
       }
     }
-    for (int type = 0; type < 3; ++type)
+    for (int type : {1, 2, 0})  // cross-tile first: their neighbour gathers are prefetched
       for (int64_t e : grp[type]) {
         const int32_t code = int32_t(e & 0xffffffff);
         const int32_t ref = int32_t(uint32_t(uint64_t(e) >> 32));
@@ -546,7 +577,7 @@ hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0
   hg::TileArgs a{};
   a.N = int32_t(ctx->N);
   a.T = ctx->tileT;
-  a.maxdeg = ctx->maxdeg;
+  a.maxdeg = ctx->tileMD;
   a.n_inlet = ctx->fuse_inlet ? ctx->n_inlet : 0;
   a.xi = ctx->st[src][0];
   a.qx = ctx->st[src][1];
@@ -562,11 +593,7 @@ hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0
   a.area = ctx->area;
   a.deg = ctx->deg;
   a.item_off = ctx->item_off;
-  a.item_code = ctx->item_code;
-  a.item_ref = ctx->item_ref;
-  a.item_nx = ctx->item_nx;
-  a.item_ny = ctx->item_ny;
-  a.item_len = ctx->item_len;
+  a.items = ctx->items;
   a.tile_inlets = ctx->tile_inlets;
   a.g_kind = ctx->g_kind;
   a.g_bc = ctx->g_bc;
@@ -643,6 +670,27 @@ hg::InletArgs inlet_args(hg_ctx* ctx, int src) {
 
 int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
 
+// Tile kernel instances: T = 256 cells, MD = 3 (triangles), 4 (quads or
+// mixed), 8 (any polygon up to octagons).
+template <int MD>
+void launch_tile_md(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+  constexpr int T = 256;
+  const size_t lds = tile_lds_bytes(T, MD);
+  if (step)
+    hipLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((k_swe_tile<false, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
+}
+
+void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+  if (ctx->tileMD == 3)
+    launch_tile_md<3>(ctx, a, step);
+  else if (ctx->tileMD == 4)
+    launch_tile_md<4>(ctx, a, step);
+  else
+    launch_tile_md<8>(ctx, a, step);
+}
+
 // Enqueue one step (or one RHS evaluation) reading state `src`.
 int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
                  double* o2 = nullptr) {
@@ -650,11 +698,7 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
     if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
       hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
     hg::TileArgs a = make_tile_args(ctx, src, dst, dt, o0, o1, o2);
-    const size_t lds = tile_lds_bytes(ctx->tileT, ctx->maxdeg);
-    if (step)
-      hipLaunchKernelGGL(k_swe_tile<true>, dim3(ctx->ntiles), dim3(ctx->tileT), lds, ctx->stream, a);
-    else
-      hipLaunchKernelGGL(k_swe_tile<false>, dim3(ctx->ntiles), dim3(ctx->tileT), lds, ctx->stream, a);
+    launch_tile(ctx, a, step);
     HG_TRY(hipGetLastError());
     return HG_OK;
   }
@@ -789,8 +833,8 @@ int ensure_graph(hg_ctx* ctx, double dt) {
   }
   hipGraph_t graph;
   HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(ctx, 0, 1, dt, true);
-  if (!rc) rc = enqueue_step(ctx, 1, 0, dt, true);
+  int rc = 0;
+  for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(ctx, k & 1, (k + 1) & 1, dt, true);
   hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
   if (rc) return rc;
   if (e != hipSuccess) {
@@ -812,15 +856,15 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
     ctx->cur = 0;
     --nsteps;
   }
-  if (nsteps >= 2) {
+  if (nsteps >= kGraphSteps) {
     int rc = ensure_graph(ctx, dt);
     if (rc) return rc;
-    for (int64_t k = 0; k + 1 < nsteps; k += 2) HG_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
+    for (; nsteps >= kGraphSteps; nsteps -= kGraphSteps) HG_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
   }
-  if (nsteps & 1) {
-    int rc = enqueue_step(ctx, 0, 1, dt, true);
+  for (; nsteps > 0; --nsteps) {  // remainder, single launches
+    int rc = enqueue_step(ctx, ctx->cur, 1 - ctx->cur, dt, true);
     if (rc) return rc;
-    ctx->cur = 1;
+    ctx->cur = 1 - ctx->cur;
   }
   return HG_OK;
 }
@@ -1034,9 +1078,13 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
   // ---- tiled face-once kernel data
   ctx->tiled = !(flags & HG_CELL_KERNEL);
   if (ctx->tiled) {
-    int T = 256;
-    while (T > 64 && tile_lds_bytes(T, maxdeg) > 64 * 1024) T /= 2;
+    const int T = 256;
     ctx->tileT = T;
+    ctx->tileMD = maxdeg <= 3 ? 3 : (maxdeg <= 4 ? 4 : 8);
+    if (maxdeg > 8) {
+      ctx->err = "the tiled kernel supports cells with up to 8 faces (use HG_CELL_KERNEL)";
+      return fail(HG_ERR_MESH);
+    }
     ctx->ntiles = int32_t((N + T - 1) / T);
     ctx->fuse_inlet = b->n_inlet > 0 && b->n_inlet <= 32;
     TileBuild tb = build_tiles(N, T, h_deg, h_nbr, h_nx, h_ny, h_len, g_kind, g_bc);
@@ -1066,19 +1114,13 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
         return fail(HG_ERR_MESH);
       }
     }
+    std::vector<hg::ItemRec> recs(ni);
+    for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
     if (!rc) rc = dalloc(ctx, &ctx->item_off, tb.off.size());
-    if (!rc) rc = dalloc(ctx, &ctx->item_code, ni);
-    if (!rc) rc = dalloc(ctx, &ctx->item_ref, ni);
-    if (!rc) rc = dalloc(ctx, &ctx->item_nx, ni);
-    if (!rc) rc = dalloc(ctx, &ctx->item_ny, ni);
-    if (!rc) rc = dalloc(ctx, &ctx->item_len, ni);
+    if (!rc) rc = dalloc(ctx, &ctx->items, ni);
     if (!rc) rc = dalloc(ctx, &ctx->tile_inlets, tb.inlets.size());
     if (!rc) rc = upload(ctx, ctx->item_off, tb.off.data(), tb.off.size());
-    if (!rc) rc = upload(ctx, ctx->item_code, tb.code.data(), ni);
-    if (!rc) rc = upload(ctx, ctx->item_ref, tb.ref.data(), ni);
-    if (!rc) rc = upload(ctx, ctx->item_nx, tb.nx.data(), ni);
-    if (!rc) rc = upload(ctx, ctx->item_ny, tb.ny.data(), ni);
-    if (!rc) rc = upload(ctx, ctx->item_len, tb.len.data(), ni);
+    if (!rc) rc = upload(ctx, ctx->items, recs.data(), ni);
     if (!rc) rc = upload(ctx, ctx->tile_inlets, tb.inlets.data(), tb.inlets.size());
     if (rc) return fail(rc);
   }
@@ -1227,6 +1269,39 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
   return HG_OK;
 }
 
+int hg_probe_copy(hg_ctx* ctx, int64_t nbytes, int32_t reps, double* gbps) {
+  if (!ctx || nbytes < 16 || reps <= 0) return HG_ERR_ARG;
+  const int64_t n2 = nbytes / 16;
+  double2 *src = nullptr, *dst = nullptr;
+  HG_TRY(hipMalloc(reinterpret_cast<void**>(&src), n2 * 16));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&dst), n2 * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return HG_ERR_HIP;
+  }
+  int rc = HG_OK;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int grid = 256 * 8 * 4;  // 4 work-groups per SIMD
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kBlock), 0, ctx->stream, n2, src, dst);  // warm-up
+  (void)hipEventRecord(e0, ctx->stream);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(kBlock), 0, ctx->stream, n2, src, dst);
+  (void)hipEventRecord(e1, ctx->stream);
+  if (hipEventSynchronize(e1) != hipSuccess) rc = HG_ERR_HIP;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (gbps) *gbps = 2.0 * double(n2) * 16.0 * reps / (double(ms) * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (rc) ctx->err = "hg_probe_copy: kernel failed";
+  return rc;
+}
+
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel) {
   if (!ctx || nsteps <= 0 || reps <= 0) return HG_ERR_ARG;
@@ -1234,7 +1309,7 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   HG_TRY(hipEventCreate(&e0));
   HG_TRY(hipEventCreate(&e1));
   int rc = clear_flags(ctx);
-  if (!rc) rc = advance_async(ctx, dt, 2);  // warm-up (graph instantiation)
+  if (!rc) rc = advance_async(ctx, dt, kGraphSteps);  // warm-up (graph instantiation)
   HG_TRY(hipStreamSynchronize(ctx->stream));
   HG_TRY(hipEventRecord(e0, ctx->stream));
   for (int r = 0; r < reps && !rc; ++r) rc = advance_async(ctx, dt, nsteps);
@@ -1248,9 +1323,7 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
     HG_TRY(hipEventRecord(e0, ctx->stream));
     for (int k = 0; k < K; ++k) {
       if (ctx->tiled) {
-        hg::TileArgs a = make_tile_args(ctx, k & 1, (k + 1) & 1, dt);
-        hipLaunchKernelGGL(k_swe_tile<true>, dim3(ctx->ntiles), dim3(ctx->tileT),
-                           tile_lds_bytes(ctx->tileT, ctx->maxdeg), ctx->stream, a);
+        launch_tile(ctx, make_tile_args(ctx, k & 1, (k + 1) & 1, dt), true);
       } else {
         StepArgs a = make_args(ctx, k & 1, (k + 1) & 1, dt);
         hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);

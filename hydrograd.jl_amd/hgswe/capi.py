@@ -24,7 +24,7 @@ HG_CELL_KERNEL = 2
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
-    "hg_info",
+    "hg_info", "hg_probe_copy",
 ]
 
 
@@ -74,6 +74,7 @@ def lib():
         L.hg_cfl_dt.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
         L.hg_time_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
+        L.hg_probe_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -185,6 +186,12 @@ class Context:
         self._chk(lib().hg_time_advance(self.h, C.c_double(dt), C.c_int64(nsteps), C.c_int32(reps), C.byref(a),
                                         C.byref(b)))
         return a.value, b.value
+
+
+def probe_copy(ctx, nbytes, reps=1):
+    g = C.c_double()
+    ctx._chk(lib().hg_probe_copy(ctx.h, C.c_int64(nbytes), C.c_int32(reps), C.byref(g)))
+    return g.value
 
 
 def info():

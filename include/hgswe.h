@@ -154,6 +154,12 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel);
 
+/* Measurement helper with no counterpart in the reference: copies nbytes
+ * between two scratch device buffers `reps` times with a plain streaming
+ * kernel and reports the achieved bandwidth (read + write bytes / time).  Its
+ * known traffic calibrates rocprof's FETCH_SIZE / WRITE_SIZE counters. */
+int hg_probe_copy(hg_ctx* ctx, int64_t nbytes, int32_t reps, double* gbps);
+
 /* Library / device information string (arch, CU count, build flags). */
 const char* hg_info(void);
 
