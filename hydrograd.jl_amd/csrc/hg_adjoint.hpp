@@ -1,0 +1,193 @@
+// hg_adjoint.hpp -- hand-written reverse-mode derivatives (vector-Jacobian
+// products) of the flux and source terms of the shallow-water RHS.
+//
+// Semantics follow forward-mode / reverse-mode AD of the reference
+// (ForwardDiff / Zygote through swe_2d_rhs): branch tests carry no
+// derivative, the derivative follows the taken branch, clamped depths and
+// masked discharges are constants, and a branch that copies a mirrored state
+// (swe_2D_solvers.jl:23-51) routes the copy's cotangent back to its source.
+#pragma once
+#include "hg_roe_pair.hpp"
+
+namespace hg {
+
+// Cotangents of one side's raw flux inputs.
+struct SideBar {
+  double xi, h, hu, hv;
+};
+
+// Cotangents of one side including its derived quantities (u, v, s, p).
+struct ExtBar {
+  double xi, h, hu, hv, u, v, s, p;
+};
+
+__host__ __device__ __forceinline__ void ext_zero(ExtBar& e) { e.xi = e.h = e.hu = e.hv = e.u = e.v = e.s = e.p = 0.0; }
+
+// u = hu/h, v = hv/h, s = sqrt(h + eps), p = 0.5 g ((xi+eps)^2 + 2 xi hs):
+// fold the derived cotangents into (xi, h, hu, hv).
+__host__ __device__ __forceinline__ SideBar ext_to_side(const ExtBar& e, const CellQ& c, double half_g) {
+  SideBar s;
+  s.hu = e.hu + e.u / c.h;
+  s.hv = e.hv + e.v / c.h;
+  s.h = e.h - e.u * c.u / c.h - e.v * c.v / c.h + e.s / (2.0 * c.s);
+  s.xi = e.xi + e.p * half_g * (2.0 * (c.xi + EPS) + 2.0 * c.hs);
+  return s;
+}
+
+// Physical flux of one side (hu, hv, u, v, p), reverse.
+__host__ __device__ __forceinline__ void phys_vjp(double hu, double hv, double u, double v, double nx, double ny,
+                                                  const double fb[3], ExtBar& e) {
+  const double un = nx * u + ny * v;
+  e.hu += fb[0] * nx + fb[1] * un;
+  e.hv += fb[0] * ny + fb[2] * un;
+  e.u += fb[1] * nx * hu + fb[2] * nx * hv;
+  e.v += fb[1] * ny * hu + fb[2] * ny * hv;
+  e.p += fb[1] * nx + fb[2] * ny;
+}
+
+// VJP of the flux F(A | B, n) seen from A (Riemann_2D_Roe,
+// swe_2D_solvers.jl:4-164) with cotangent Fb: cotangents of both sides'
+// (xi, h, hu, hv).  A and B are the flux inputs (clamped depth, masked
+// discharge, derived u, v, s, p) as make_cellq forms them.
+__host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double nx, double ny, double g, double half_g,
+                                        double hmin, const double Fb[3], SideBar& Ab, SideBar& Bb) {
+  ExtBar ea, eb;
+  ext_zero(ea);
+  ext_zero(eb);
+  if (A.h <= hmin && B.h <= hmin) {
+    Ab = SideBar{0.0, 0.0, 0.0, 0.0};
+    Bb = SideBar{0.0, 0.0, 0.0, 0.0};
+    return;
+  }
+  int mode = 0;  // 0 general, 1 B mirrored from A, 2 A mirrored from B
+  double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
+  double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
+  if ((A.h + A.zb) < (B.zb + hmin) && (B.h <= hmin)) {
+    mode = 1;
+    hB = A.h; huB = -A.hu; hvB = -A.hv; uB = -A.u; vB = -A.v; sB = A.s;
+  } else if ((B.h + B.zb) < (A.zb + hmin) && (A.h <= hmin)) {
+    mode = 2;
+    hA = B.h; huA = -B.hu; hvA = -B.hv; uA = -B.u; vA = -B.v; sA = B.s;
+  } else if (A.h <= hmin || B.h <= hmin) {  // physical flux of the wet side, h^2 pressure
+    const bool left_dry = A.h <= hmin;
+    const CellQ& W = left_dry ? B : A;
+    ExtBar& ew = left_dry ? eb : ea;
+    const double un = nx * W.u + ny * W.v;
+    ew.hu += Fb[0] * nx + Fb[1] * un;
+    ew.hv += Fb[0] * ny + Fb[2] * un;
+    ew.u += Fb[1] * nx * W.hu + Fb[2] * nx * W.hv;
+    ew.v += Fb[1] * ny * W.hu + Fb[2] * ny * W.hv;
+    const double prb = Fb[1] * nx + Fb[2] * ny;
+    ew.h += prb * half_g * 2.0 * (W.h + EPS);
+    Ab = ext_to_side(ea, A, half_g);
+    Bb = ext_to_side(eb, B, half_g);
+    return;
+  }
+  // ---- forward recompute (general branch)
+  const double hRoe = (hA + hB) * 0.5;
+  const double sAB = sA + sB;
+  const double uRoe = (sA * uA + sB * uB) / sAB;
+  const double vRoe = (sA * vA + sB * vB) / sAB;
+  const double un = uRoe * nx + vRoe * ny;
+  const double c = sqrt(g * hRoe + EPS);
+  const double o = 0.5 / c;
+  const double d0 = B.xi - A.xi, d1 = huB - huA, d2 = hvB - hvA;
+  const double k0 = -(uRoe * ny - vRoe * nx);
+  const double w0 = k0 * d0 + ny * d1 + (-nx) * d2;
+  const double w1 = (un * o + 0.5) * d0 + (-nx * o) * d1 + (-ny * o) * d2;
+  const double w2 = (-un * o + 0.5) * d0 + (nx * o) * d1 + (ny * o) * d2;
+  const double um = un - c, up = un + c;
+  const double l0 = sqrt(un * un + EPS), l1 = sqrt(um * um + EPS), l2 = sqrt(up * up + EPS);
+  const double a1 = l1 * w1, a2 = l2 * w2;
+  // ---- reverse: F = ((fA + fB) - r) / 2
+  const double hf[3] = {0.5 * Fb[0], 0.5 * Fb[1], 0.5 * Fb[2]};
+  ExtBar ma, mb;  // cotangents of the (possibly mirrored) flux states
+  ext_zero(ma);
+  ext_zero(mb);
+  phys_vjp(huA, hvA, uA, vA, nx, ny, hf, ma);
+  phys_vjp(huB, hvB, uB, vB, nx, ny, hf, mb);
+  const double rb0 = -hf[0], rb1 = -hf[1], rb2 = -hf[2];
+  // r0 = a1 + a2; r1 = ny a0 + (u - c nx) a1 + (u + c nx) a2; r2 = -nx a0 + (v - c ny) a1 + (v + c ny) a2
+  const double ab0 = rb1 * ny - rb2 * nx;
+  const double ab1 = rb0 + rb1 * (uRoe - c * nx) + rb2 * (vRoe - c * ny);
+  const double ab2 = rb0 + rb1 * (uRoe + c * nx) + rb2 * (vRoe + c * ny);
+  double uRb = rb1 * (a1 + a2);
+  double vRb = rb2 * (a1 + a2);
+  double cb = (rb1 * nx + rb2 * ny) * (a2 - a1);
+  // a_k = l_k w_k
+  const double l0b = ab0 * w0, l1b = ab1 * w1, l2b = ab2 * w2;
+  const double w0b = ab0 * l0, w1b = ab1 * l1, w2b = ab2 * l2;
+  double unb = l0b * un / l0 + l1b * um / l1 + l2b * up / l2;
+  cb += -l1b * um / l1 + l2b * up / l2;
+  // w's
+  const double k0b = w0b * d0;
+  const double d0b = w0b * k0 + w1b * (un * o + 0.5) + w2b * (-un * o + 0.5);
+  const double d1b = w0b * ny - w1b * nx * o + w2b * nx * o;
+  const double d2b = -w0b * nx - w1b * ny * o + w2b * ny * o;
+  unb += (w1b - w2b) * o * d0;
+  const double ob = w1b * (un * d0 - nx * d1 - ny * d2) + w2b * (-un * d0 + nx * d1 + ny * d2);
+  uRb += -k0b * ny;
+  vRb += k0b * nx;
+  cb += -ob * o / c;
+  const double hRb = cb * g / (2.0 * c);
+  uRb += unb * nx;
+  vRb += unb * ny;
+  ma.s += (uRb * (uA - uRoe) + vRb * (vA - vRoe)) / sAB;
+  mb.s += (uRb * (uB - uRoe) + vRb * (vB - vRoe)) / sAB;
+  ma.u += uRb * sA / sAB;
+  ma.v += vRb * sA / sAB;
+  mb.u += uRb * sB / sAB;
+  mb.v += vRb * sB / sAB;
+  ma.h += 0.5 * hRb;
+  mb.h += 0.5 * hRb;
+  // d = B - A (xi of both sides is never mirrored)
+  ma.xi -= d0b;
+  mb.xi += d0b;
+  ma.hu -= d1b;
+  mb.hu += d1b;
+  ma.hv -= d2b;
+  mb.hv += d2b;
+  // ---- route the mirrored copies back to their source side
+  if (mode == 1) {  // hB = hA, huB = -huA, hvB = -hvA, uB = -uA, vB = -vA, sB = sA
+    ea = ma;
+    ea.h += mb.h; ea.hu -= mb.hu; ea.hv -= mb.hv; ea.u -= mb.u; ea.v -= mb.v; ea.s += mb.s;
+    eb.xi = mb.xi;
+    eb.p = mb.p;
+  } else if (mode == 2) {
+    eb = mb;
+    eb.h += ma.h; eb.hu -= ma.hu; eb.hv -= ma.hv; eb.u -= ma.u; eb.v -= ma.v; eb.s += ma.s;
+    ea.xi = ma.xi;
+    ea.p = ma.p;
+  } else {
+    ea = ma;
+    eb = mb;
+  }
+  Ab = ext_to_side(ea, A, half_g);
+  Bb = ext_to_side(eb, B, half_g);
+}
+
+// VJP of the momentum source of one wet cell (semi_discretize_swe_2D.jl:
+// 474-475, 544-547): src = g xi S0 - friction.  y1, y2 are the cotangents of
+// the x / y momentum RHS; outputs add to the cotangents of xi, h, hu, hv,
+// n, S0x, S0y.
+__host__ __device__ __forceinline__ void source_vjp(double g, double kn2, double hsmall, double xi, double h,
+                                                    double qx, double qy, double n, double S0x, double S0y, double y1,
+                                                    double y2, double& xib, double& hb, double& qxb, double& qyb,
+                                                    double& nb, double& S0xb, double& S0yb) {
+  xib += g * S0x * y1 + g * S0y * y2;
+  S0xb = g * xi * y1;
+  S0yb = g * xi * y2;
+  const double fxb = -y1, fyb = -y2;
+  const double x = h + hsmall;
+  const double p73 = pow_7_3(x);
+  const double cc = g * (n * n) / kn2 / p73;
+  const double s = sqrt(qx * qx + qy * qy + EPS);
+  const double cb = fxb * s * qx + fyb * s * qy;
+  const double sb = fxb * cc * qx + fyb * cc * qy;
+  qxb += fxb * cc * s + sb * qx / s;
+  qyb += fyb * cc * s + sb * qy / s;
+  nb += cb * 2.0 * g * n / kn2 / p73;
+  hb += cb * (-7.0 / 3.0) * cc / x;
+}
+
+}  // namespace hg

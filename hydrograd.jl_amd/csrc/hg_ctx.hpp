@@ -43,6 +43,7 @@ struct StepArgs {
   const double* inletQ;
   const double* exitWSE;
   const double* totalA;  // per inlet, this step
+  uint8_t* clamp;        // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
 };
 
@@ -109,6 +110,47 @@ struct TileArgs {
   const double* in_L53;
   const double* totalA;  // used when n_inlet == 0 (conveyance from the separate kernel)
   int32_t* flag;
+  uint8_t* clamp;  // optional: 1 where the step's dry clamp fired (adjoint recompute)
+  double g, half_g, kn2, hsmall, dt;
+};
+
+// Arguments of the adjoint (VJP) kernel of one RHS evaluation / one step.
+struct AdjArgs {
+  int32_t N;
+  int32_t maxdeg;
+  const double* xi;  // state the RHS was evaluated at
+  const double* qx;
+  const double* qy;
+  const double* lx;  // cotangent of the step output (STEP) or of the RHS
+  const double* lqx;
+  const double* lqy;
+  const uint8_t* mask;  // STEP: dry-clamp flags of the step output (nullable)
+  double* ox;           // cotangent of the state
+  double* oqx;
+  double* oqy;
+  double* nbar;   // += d/dn per cell
+  double* zbbar;  // += d/dzb per cell (exit-H depth)
+  double* S0bar;  // [2N] d/dS0 of this evaluation
+  double* vbar;   // [G] d/d(inlet ghost velocity) of this evaluation
+  const double* hstill;
+  const double* zb;
+  const double* n;
+  const double* S0x;
+  const double* S0y;
+  const double* area;
+  const int32_t* deg;
+  const int32_t* nbr;
+  const double* fnx;
+  const double* fny;
+  const double* flen;
+  const int32_t* g_kind;
+  const int32_t* g_bc;
+  const double* g_hstill;
+  const double* g_zb;
+  const double* g_L23;
+  const double* inletQ;
+  const double* exitWSE;
+  const double* totalA;
   double g, half_g, kn2, hsmall, dt;
 };
 
@@ -146,7 +188,7 @@ struct hg_ctx {
   double *fnx = nullptr, *fny = nullptr, *flen = nullptr;
   int32_t *g_kind = nullptr, *g_bc = nullptr, *g_cell = nullptr;
   double *g_hstill = nullptr, *g_zb = nullptr, *g_zb0 = nullptr, *g_L23 = nullptr;
-  int32_t *in_off = nullptr, *in_cell = nullptr;
+  int32_t *in_off = nullptr, *in_cell = nullptr, *in_ghost = nullptr;
   double* in_L53 = nullptr;
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
@@ -162,6 +204,14 @@ struct hg_ctx {
   double* scratch = nullptr;  // 3N doubles for host transfers / params
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials
+
+  // adjoint work buffers (allocated on first use)
+  double* adj_lam[2] = {nullptr, nullptr};  // 3N each
+  double *adj_nbar = nullptr, *adj_zbbar = nullptr, *adj_S0bar = nullptr, *adj_vbar = nullptr, *adj_Qbar = nullptr;
+  double* adj_seg = nullptr;    // (kck + 1) x 3N states of one checkpoint segment
+  uint8_t* adj_mask = nullptr;  // kck x N clamp flags
+  double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
+  size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
 
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;

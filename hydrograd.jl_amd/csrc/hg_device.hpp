@@ -20,7 +20,7 @@ struct Side {  // one side of a face: the inputs Riemann_2D_Roe reads
 // Riemann_2D_Roe (swe_2D_solvers.jl:4-164).  The 3x3 products of
 // :103-118 are written out; entries that multiply by the literal 0.0 / 1.0
 // of R and |Lambda| are folded (identical up to the sign of an exact zero).
-__device__ __forceinline__ void roe_flux(Side L, Side R, double nx, double ny, double g, double half_g,
+__host__ __device__ __forceinline__ void roe_flux(Side L, Side R, double nx, double ny, double g, double half_g,
                                          double hmin, double& f0, double& f1, double& f2) {
   if (L.h <= hmin && R.h <= hmin) {  // :16-22
     f0 = 0.0; f1 = 0.0; f2 = 0.0;
@@ -86,11 +86,11 @@ __device__ __forceinline__ void roe_flux(Side L, Side R, double nx, double ny, d
 // the correctly rounded power (the reference's pow(x, 7.0/3.0) carries an
 // exponent 1.5e-16 off 7/3, a relative 1.5e-16 * |ln x| on its own), at a
 // sixth of the instruction count of a general double pow.
-__device__ __forceinline__ double pow_7_3(double x) { return (x * x) * cbrt(x); }
+__host__ __device__ __forceinline__ double pow_7_3(double x) { return (x * x) * cbrt(x); }
 
 // Manning friction (semi_discretize_swe_2D.jl:544-547); h is the clamped
 // depth, q the masked discharge.  Dividing by k_n^2 = 1 is exact and skipped.
-__device__ __forceinline__ void friction(double g, double kn2, double hsmall, double h, double qx, double qy,
+__host__ __device__ __forceinline__ void friction(double g, double kn2, double hsmall, double h, double qx, double qy,
                                          double n, double& fx, double& fy) {
   double c = g * (n * n);
   if (kn2 != 1.0) c = c / kn2;

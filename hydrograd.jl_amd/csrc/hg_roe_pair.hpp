@@ -27,7 +27,7 @@ struct CellQ {
   double xi, hs, h, hu, hv, zb, u, v, s, p;
 };
 
-__device__ __forceinline__ CellQ make_cellq(double xi, double hs, double qx, double qy, double zb, double half_g,
+__host__ __device__ __forceinline__ CellQ make_cellq(double xi, double hs, double qx, double qy, double zb, double half_g,
                                             double hsmall) {
   CellQ c;
   c.xi = xi;
@@ -52,7 +52,7 @@ __device__ __forceinline__ CellQ make_cellq(double xi, double hs, double qx, dou
 // Flux through the face (A | B), normal n pointing from A to B.
 // fa = flux seen from A; when BOTH, fb = flux seen from B (normal -n).
 template <bool BOTH>
-__device__ __forceinline__ void roe_pair(const CellQ& A, const CellQ& B, double nx, double ny, double g,
+__host__ __device__ __forceinline__ void roe_pair(const CellQ& A, const CellQ& B, double nx, double ny, double g,
                                          double half_g, double hmin, double fa[3], double fb[3]) {
   if (A.h <= hmin && B.h <= hmin) {  // :16-22
     fa[0] = fa[1] = fa[2] = 0.0;

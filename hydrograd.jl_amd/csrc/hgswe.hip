@@ -27,6 +27,7 @@
 #include "hg_ctx.hpp"
 #include "hg_device.hpp"
 #include "hg_roe_pair.hpp"
+#include "hg_adjoint.hpp"
 #include "hgswe.h"
 
 using hg::Side;
@@ -191,11 +192,13 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
     double xn = xi + a.dt * d0;
     double qxn = qx0 + a.dt * d1;
     double qyn = qy0 + a.dt * d2;
-    if (xn < hsmall) {
+    const bool dry = xn < hsmall;
+    if (dry) {
       xn = hsmall;
       qxn = 0.0;
       qyn = 0.0;
     }
+    if (a.clamp) a.clamp[i] = dry ? 1 : 0;
     a.xo[i] = xn;
     a.qxo[i] = qxn;
     a.qyo[i] = qyn;
@@ -391,11 +394,13 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
     double xn = xi + a.dt * d0;
     double qxn = qx0 + a.dt * d1;
     double qyn = qy0 + a.dt * d2;
-    if (xn < hsmall) {
+    const bool dry = xn < hsmall;
+    if (dry) {
       xn = hsmall;
       qxn = 0.0;
       qyn = 0.0;
     }
+    if (a.clamp) a.clamp[i] = dry ? 1 : 0;
     a.xo[i] = xn;
     a.qxo[i] = qxn;
     a.qyo[i] = qyn;
@@ -466,6 +471,211 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(int64_t n2, const double2
                                                       double2* __restrict__ dst) {
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n2; k += int64_t(gridDim.x) * blockDim.x)
     dst[k] = src[k];
+}
+
+// ------------------------------------------------------------- adjoint ----
+// Reverse-mode adjoint of one RHS evaluation (STEP = false: the cotangent of
+// dQ/dt is given) or of one Euler step with its dry clamp (STEP = true: the
+// cotangent of the step output is given; cells the clamp reset pass
+// nothing back).  One work-item per cell gathers, deterministically, the
+// cotangent that flows into its own state:
+//   * an internal face (i | j) enters RHS_i as -F L / A_i and, through the
+//     exact mirror F(j | i, -n) = -F(i | j, n), RHS_j as +F L / A_j; the cell
+//     takes the left-state VJP of F(i | j, n) with the combined cotangent
+//     c = -(L / A_i) ybar_i + (L / A_j) ybar_j;
+//   * a boundary face also routes the ghost-state VJP back through the
+//     ghost's boundary rule (bc_2D.jl:640-834);
+//   * the source term (bed slope, Manning friction) is the cell's own.
+// Derivatives with respect to n, the exit-H bed term, S0 and the inlet
+// ghost velocity are accumulated / written for the follow-up kernels.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_swe_adj(hg::AdjArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const double hsmall = a.hsmall;
+  const double xi = a.xi[i], hs = a.hstill[i];
+  const double zb_i = a.zb[i], n_i = a.n[i], A_i = a.area[i];
+  const hg::CellQ me = hg::make_cellq(xi, hs, a.qx[i], a.qy[i], zb_i, a.half_g, hsmall);
+  const bool wet = me.h > hsmall;
+  double lt0 = a.lx[i], lt1 = a.lqx[i], lt2 = a.lqy[i];
+  if (STEP && a.mask && a.mask[i]) lt0 = lt1 = lt2 = 0.0;
+  const double y0 = STEP ? a.dt * lt0 : lt0, y1 = STEP ? a.dt * lt1 : lt1, y2 = STEP ? a.dt * lt2 : lt2;
+  hg::SideBar U{0.0, 0.0, 0.0, 0.0};
+  double nb = 0.0, zbb = 0.0;
+  const int d = a.deg[i];
+  for (int k = 0; k < d; ++k) {
+    const int s = k * a.N + i;
+    const int j = a.nbr[s];
+    const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
+    const double fi = -(L / A_i);
+    if (j >= 0) {
+      const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
+      double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
+      if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
+      const double yj0 = STEP ? a.dt * m0 : m0, yj1 = STEP ? a.dt * m1 : m1, yj2 = STEP ? a.dt * m2 : m2;
+      const double fj = L / a.area[j];
+      const double c[3] = {fi * y0 + fj * yj0, fi * y1 + fj * yj1, fi * y2 + fj * yj2};
+      hg::SideBar Ab, Bb;
+      hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
+      U.xi += Ab.xi; U.h += Ab.h; U.hu += Ab.hu; U.hv += Ab.hv;
+    } else {
+      const int gi = ~j;
+      const int kind = a.g_kind[gi];
+      double hgh, qxg, qyg, v = 0.0, w = 0.0;
+      switch (kind) {
+        case HG_BC_INLETQ: {
+          const int b = a.g_bc[gi];
+          w = wet ? 1.0 : 0.0;
+          v = a.inletQ[b] / a.totalA[b] * a.g_L23[gi] / n_i;
+          hgh = me.h;
+          qxg = -me.h * v * nx * w;
+          qyg = -me.h * v * ny * w;
+          break;
+        }
+        case HG_BC_EXITH: {
+          const double hx = a.exitWSE[a.g_bc[gi]] - zb_i;
+          hgh = (hx > hsmall) ? hx : hsmall;
+          qxg = me.hu;
+          qyg = me.hv;
+          break;
+        }
+        case HG_BC_WALL:
+          hgh = me.h;
+          qxg = -me.hu;
+          qyg = -me.hv;
+          break;
+        default: {
+          const double vdn = me.hu * nx + me.hv * ny;
+          hgh = me.h;
+          qxg = me.hu - 2.0 * vdn * nx;
+          qyg = me.hv - 2.0 * vdn * ny;
+          break;
+        }
+      }
+      hg::CellQ B;
+      B.hs = a.g_hstill[gi];
+      B.xi = hgh - B.hs;
+      B.h = hgh;
+      B.hu = qxg;
+      B.hv = qyg;
+      B.zb = a.g_zb[gi];
+      B.u = qxg / hgh;
+      B.v = qyg / hgh;
+      B.s = sqrt(hgh + hg::EPS);
+      B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
+      const double c[3] = {fi * y0, fi * y1, fi * y2};
+      hg::SideBar Ab, Bb;
+      hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
+      U.xi += Ab.xi; U.h += Ab.h; U.hu += Ab.hu; U.hv += Ab.hv;
+      const double hgb = Bb.h + Bb.xi;  // xi_g = h_g - hstill_g
+      switch (kind) {
+        case HG_BC_INLETQ: {  // h_g = h, q_g = -h v n w, v = (Q / A) L^(2/3) / n
+          U.h += hgb + Bb.hu * (-v * nx * w) + Bb.hv * (-v * ny * w);
+          const double vb = Bb.hu * (-me.h * nx * w) + Bb.hv * (-me.h * ny * w);
+          nb += vb * (-v / n_i);
+          a.vbar[gi] = vb;
+          break;
+        }
+        case HG_BC_EXITH: {  // h_g = max(h_small, WSE - zb_cell), q_g = q
+          if (a.exitWSE[a.g_bc[gi]] - zb_i > hsmall) zbb -= hgb;
+          U.hu += Bb.hu;
+          U.hv += Bb.hv;
+          break;
+        }
+        case HG_BC_WALL:
+          U.h += hgb;
+          U.hu -= Bb.hu;
+          U.hv -= Bb.hv;
+          break;
+        default: {
+          const double vdnb = -2.0 * (Bb.hu * nx + Bb.hv * ny);
+          U.h += hgb;
+          U.hu += Bb.hu + vdnb * nx;
+          U.hv += Bb.hv + vdnb * ny;
+          break;
+        }
+      }
+    }
+  }
+  double xib = U.xi, hb = U.h, qxb = U.hu, qyb = U.hv, S0xb = 0.0, S0yb = 0.0;
+  if (wet)
+    hg::source_vjp(a.g, a.kn2, hsmall, xi, me.h, me.hu, me.hv, n_i, a.S0x[i], a.S0y[i], y1, y2, xib, hb, qxb, qyb,
+                   nb, S0xb, S0yb);
+  // clamped depth / masked discharge are constants (semi_discretize_swe_2D.jl:101-106)
+  double ox = xib, oqx = 0.0, oqy = 0.0;
+  if (wet) {
+    ox += hb;
+    oqx = qxb;
+    oqy = qyb;
+  }
+  if (STEP) {
+    ox += lt0;
+    oqx += lt1;
+    oqy += lt2;
+  }
+  a.ox[i] = ox;
+  a.oqx[i] = oqx;
+  a.oqy[i] = oqy;
+  if (a.nbar) a.nbar[i] += nb;
+  if (a.zbbar) a.zbbar[i] += zbb;
+  if (a.S0bar) {
+    a.S0bar[i] = S0xb;
+    a.S0bar[a.N + i] = S0yb;
+  }
+}
+
+// Inlet conveyance adjoint (bc_2D.jl:640-694): v_f = (Q_b / A_b) L^(2/3) / n
+// and A_b = sum_f (L^(5/3) h / n) w.  One work-item walks every inlet's
+// faces in NodeString order (deterministic; a cell may own several inlet
+// faces), adds d/dh of A_b to the state cotangent and accumulates d/dn and
+// d/dQ_b.
+__global__ void k_inlet_adj(int n_inlet, const int32_t* off, const int32_t* ghost, const int32_t* cell,
+                            const double* L53, const double* g_L23, const double* xi, const double* hstill,
+                            const double* n, const double* inletQ, const double* totalA, const double* vbar,
+                            double hsmall, double* ox, double* nbar, double* Qbar) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int b = 0; b < n_inlet; ++b) {
+    const double A = totalA[b];
+    double Ab = 0.0, Qb = 0.0;
+    for (int f = off[b]; f < off[b + 1]; ++f) {
+      const int gi = ghost[f], c = cell[f];
+      const double vb = vbar[gi];
+      const double v = inletQ[b] / A * g_L23[gi] / n[c];
+      Ab += vb * (-v / A);
+      Qb += vb * (g_L23[gi] / (A * n[c]));
+    }
+    Qbar[b] += Qb;
+    for (int f = off[b]; f < off[b + 1]; ++f) {
+      const int c = cell[f];
+      double h = xi[c] + hstill[c];
+      const bool wet = h > hsmall;
+      if (!wet) continue;  // w = 0 and h clamped: no derivative
+      ox[c] += Ab * L53[f] / n[c];
+      nbar[c] += Ab * (-L53[f] * h / (n[c] * n[c]));
+    }
+  }
+}
+
+// Transpose of the Green-Gauss bed slope (process_bed_2D.jl:46-66,
+// fvm_schemes_2D.jl:89-167): zbbar_m += sum over its faces of
+//   -(w L / A_m) (n . S0bar_m) + [internal] (L / (2 A_j)) (n . S0bar_j),
+// w = 1/2 on internal faces, 1 on boundary faces.
+__global__ __launch_bounds__(kBlock) void k_bed_adj(int N, const int32_t* deg, const int32_t* nbr, const double* fnx,
+                                                   const double* fny, const double* flen, const double* area,
+                                                   const double* S0bar, double* zbbar) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  const double sx = S0bar[m], sy = S0bar[N + m], Am = area[m];
+  double acc = 0.0;
+  for (int k = 0; k < deg[m]; ++k) {
+    const int s = k * N + m;
+    const int j = nbr[s];
+    const double nx = fnx[s], ny = fny[s], L = flen[s];
+    const double w = (j >= 0) ? 0.5 : 1.0;
+    acc += -(w * L / Am) * (nx * sx + ny * sy);
+    if (j >= 0) acc += (0.5 * L / area[j]) * (nx * S0bar[j] + ny * S0bar[N + j]);
+  }
+  zbbar[m] += acc;
 }
 
 // ------------------------------------------------------------ host helpers --
@@ -572,19 +782,20 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
 
 size_t tile_lds_bytes(int T, int maxdeg) { return size_t(10 * T + 3 * maxdeg * T + 32) * sizeof(double); }
 
-hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0 = nullptr, double* o1 = nullptr,
-                            double* o2 = nullptr) {
+hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt,
+                              uint8_t* clamp = nullptr) {
   hg::TileArgs a{};
   a.N = int32_t(ctx->N);
   a.T = ctx->tileT;
   a.maxdeg = ctx->tileMD;
   a.n_inlet = ctx->fuse_inlet ? ctx->n_inlet : 0;
-  a.xi = ctx->st[src][0];
-  a.qx = ctx->st[src][1];
-  a.qy = ctx->st[src][2];
-  a.xo = o0 ? o0 : ctx->st[dst][0];
-  a.qxo = o1 ? o1 : ctx->st[dst][1];
-  a.qyo = o2 ? o2 : ctx->st[dst][2];
+  a.xi = in[0];
+  a.qx = in[1];
+  a.qy = in[2];
+  a.xo = out[0];
+  a.qxo = out[1];
+  a.qyo = out[2];
+  a.clamp = clamp;
   a.hstill = ctx->hstill;
   a.zb = ctx->zb;
   a.n = ctx->n;
@@ -615,17 +826,24 @@ hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0
   return a;
 }
 
-StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = nullptr, double* out1 = nullptr,
-                   double* out2 = nullptr) {
+hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0 = nullptr, double* o1 = nullptr,
+                            double* o2 = nullptr) {
+  double* out[3] = {o0 ? o0 : ctx->st[dst][0], o1 ? o1 : ctx->st[dst][1], o2 ? o2 : ctx->st[dst][2]};
+  return make_tile_args_p(ctx, ctx->st[src], out, dt);
+}
+
+StepArgs make_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt,
+                     uint8_t* clamp = nullptr) {
   StepArgs a{};
   a.N = int32_t(ctx->N);
   a.maxdeg = ctx->maxdeg;
-  a.xi = ctx->st[src][0];
-  a.qx = ctx->st[src][1];
-  a.qy = ctx->st[src][2];
-  a.xo = out0 ? out0 : ctx->st[dst][0];
-  a.qxo = out1 ? out1 : ctx->st[dst][1];
-  a.qyo = out2 ? out2 : ctx->st[dst][2];
+  a.xi = in[0];
+  a.qx = in[1];
+  a.qy = in[2];
+  a.xo = out[0];
+  a.qxo = out[1];
+  a.qyo = out[2];
+  a.clamp = clamp;
   a.hstill = ctx->hstill;
   a.zb = ctx->zb;
   a.n = ctx->n;
@@ -650,6 +868,27 @@ StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = null
   a.kn2 = ctx->c.k_n * ctx->c.k_n;
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
+  return a;
+}
+
+StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = nullptr, double* out1 = nullptr,
+                   double* out2 = nullptr) {
+  double* out[3] = {out0 ? out0 : ctx->st[dst][0], out1 ? out1 : ctx->st[dst][1], out2 ? out2 : ctx->st[dst][2]};
+  return make_args_p(ctx, ctx->st[src], out, dt);
+}
+
+hg::InletArgs inlet_args_p(hg_ctx* ctx, const double* xi) {
+  hg::InletArgs a{};
+  a.n_inlet = ctx->n_inlet;
+  a.off = ctx->in_off;
+  a.cell = ctx->in_cell;
+  a.L53 = ctx->in_L53;
+  a.xi = xi;
+  a.hstill = ctx->hstill;
+  a.n = ctx->n;
+  a.hsmall = ctx->c.h_small;
+  a.totalA = ctx->totalA;
+  a.flag = ctx->flags;
   return a;
 }
 
@@ -710,6 +949,183 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
   else
     hipLaunchKernelGGL(k_swe_step<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+// One step (or RHS) from explicit state buffers; optional clamp record.
+int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt, bool step,
+                   uint8_t* clamp = nullptr) {
+  if (ctx->tiled) {
+    if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
+      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream,
+                         inlet_args_p(ctx, in[0]));
+    launch_tile(ctx, make_tile_args_p(ctx, in, out, dt, clamp), step);
+  } else {
+    if (ctx->n_inlet > 0)
+      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream,
+                         inlet_args_p(ctx, in[0]));
+    StepArgs a = make_args_p(ctx, in, out, dt, clamp);
+    if (step)
+      hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL(k_swe_step<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  }
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+// ---- adjoint host side
+int advance_async(hg_ctx* ctx, double dt, int64_t nsteps);
+
+int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
+  const int64_t N = ctx->N;
+  int rc = 0;
+  if (!ctx->adj_lam[0]) {
+    rc = dalloc(ctx, &ctx->adj_lam[0], 3 * N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_lam[1], 3 * N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_nbar, N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_zbbar, N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_S0bar, 2 * N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_vbar, std::max<int64_t>(ctx->G, 1));
+    if (!rc) rc = dalloc(ctx, &ctx->adj_Qbar, std::max<int32_t>(ctx->n_inlet, 1));
+    if (rc) return rc;
+  }
+  if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
+    rc = dalloc(ctx, &ctx->adj_seg, size_t(kck + 1) * 3 * N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_mask, size_t(kck) * N);
+    if (rc) return rc;
+    ctx->adj_seg_cap = size_t(kck);
+  }
+  if (size_t(nseg) > ctx->adj_ckpt_cap) {
+    rc = dalloc(ctx, &ctx->adj_ckpt, size_t(nseg) * 3 * N);
+    if (rc) return rc;
+    ctx->adj_ckpt_cap = size_t(nseg);
+  }
+  return HG_OK;
+}
+
+hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
+                          double* const out[3], double dt) {
+  hg::AdjArgs a{};
+  a.N = int32_t(ctx->N);
+  a.maxdeg = ctx->maxdeg;
+  a.xi = q[0]; a.qx = q[1]; a.qy = q[2];
+  a.lx = lam[0]; a.lqx = lam[1]; a.lqy = lam[2];
+  a.mask = mask;
+  a.ox = out[0]; a.oqx = out[1]; a.oqy = out[2];
+  a.nbar = ctx->adj_nbar;
+  a.zbbar = ctx->adj_zbbar;
+  a.S0bar = ctx->adj_S0bar;
+  a.vbar = ctx->adj_vbar;
+  a.hstill = ctx->hstill;
+  a.zb = ctx->zb;
+  a.n = ctx->n;
+  a.S0x = ctx->S0x;
+  a.S0y = ctx->S0y;
+  a.area = ctx->area;
+  a.deg = ctx->deg;
+  a.nbr = ctx->nbr;
+  a.fnx = ctx->fnx;
+  a.fny = ctx->fny;
+  a.flen = ctx->flen;
+  a.g_kind = ctx->g_kind;
+  a.g_bc = ctx->g_bc;
+  a.g_hstill = ctx->g_hstill;
+  a.g_zb = ctx->g_zb;
+  a.g_L23 = ctx->g_L23;
+  a.inletQ = ctx->inletQ;
+  a.exitWSE = ctx->exitWSE;
+  a.totalA = ctx->totalA;
+  a.g = ctx->c.g;
+  a.half_g = 0.5 * ctx->c.g;
+  a.kn2 = ctx->c.k_n * ctx->c.k_n;
+  a.hsmall = ctx->c.h_small;
+  a.dt = dt;
+  return a;
+}
+
+// Cotangent of one RHS evaluation (step = false) or one step (step = true)
+// at state q, given the output cotangent lam; writes the state cotangent to
+// out and accumulates into adj_nbar / adj_zbbar / adj_Qbar.
+int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
+                    double* const out[3], double dt, bool step) {
+  if (ctx->n_inlet > 0)
+    hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
+  hg::AdjArgs a = make_adj_args(ctx, q, lam, mask, out, dt);
+  if (step)
+    hipLaunchKernelGGL(k_swe_adj<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_swe_adj<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  if (ctx->n_inlet > 0)
+    hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(64), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
+                       ctx->in_ghost, ctx->in_cell, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
+                       ctx->totalA, ctx->adj_vbar, ctx->c.h_small, out[0], ctx->adj_nbar, ctx->adj_Qbar);
+  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
+                     ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+int zero_adj_accumulators(hg_ctx* ctx) {
+  HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
+  HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
+  HG_TRY(hipMemsetAsync(ctx->adj_Qbar, 0, std::max<int32_t>(ctx->n_inlet, 1) * sizeof(double), ctx->stream));
+  HG_TRY(hipMemsetAsync(ctx->adj_vbar, 0, std::max<int64_t>(ctx->G, 1) * sizeof(double), ctx->stream));
+  return HG_OK;
+}
+
+// Per-cell device array (library order) -> host array (caller order).
+int cell_out(hg_ctx* ctx, const double* dsrc, double* hdst) {
+  std::vector<double> tmp(ctx->N);
+  HG_TRY(hipMemcpyAsync(tmp.data(), dsrc, ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < ctx->N; ++i) hdst[ctx->perm[i]] = tmp[i];
+  return HG_OK;
+}
+
+double* seg_state(hg_ctx* ctx, int64_t t, int c) { return ctx->adj_seg + (t * 3 + c) * ctx->N; }
+
+// Forward nsteps from the device state, storing a checkpoint every kck steps.
+int forward_checkpointed(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck) {
+  const int64_t N = ctx->N;
+  const int64_t nseg = (nsteps + kck - 1) / kck;
+  for (int64_t sgi = 0; sgi < nseg; ++sgi) {
+    for (int c = 0; c < 3; ++c)
+      HG_TRY(hipMemcpyAsync(ctx->adj_ckpt + (sgi * 3 + c) * N, ctx->st[ctx->cur][c], N * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    int rc = advance_async(ctx, dt, std::min<int64_t>(kck, nsteps - sgi * kck));
+    if (rc) return rc;
+  }
+  return HG_OK;
+}
+
+// Reverse sweep over the checkpointed trajectory; lam (device, 3 x N) holds
+// the cotangent of the final state on entry and of the initial state on exit.
+int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_cur) {
+  const int64_t N = ctx->N;
+  const int64_t nseg = (nsteps + kck - 1) / kck;
+  for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
+    const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
+    for (int c = 0; c < 3; ++c)
+      HG_TRY(hipMemcpyAsync(seg_state(ctx, 0, c), ctx->adj_ckpt + (sgi * 3 + c) * N, N * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    for (int64_t t = 0; t < ns; ++t) {  // recompute the segment, recording the clamp of every step
+      const double* in[3] = {seg_state(ctx, t, 0), seg_state(ctx, t, 1), seg_state(ctx, t, 2)};
+      double* out[3] = {seg_state(ctx, t + 1, 0), seg_state(ctx, t + 1, 1), seg_state(ctx, t + 1, 2)};
+      int rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask + t * N);
+      if (rc) return rc;
+    }
+    for (int64_t t = ns - 1; t >= 0; --t) {
+      const double* q[3] = {seg_state(ctx, t, 0), seg_state(ctx, t, 1), seg_state(ctx, t, 2)};
+      double* L = ctx->adj_lam[lam_cur];
+      double* O = ctx->adj_lam[1 - lam_cur];
+      const double* lam[3] = {L, L + N, L + 2 * N};
+      double* out[3] = {O, O + N, O + 2 * N};
+      int rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask + t * N, out, dt, true);
+      if (rc) return rc;
+      lam_cur = 1 - lam_cur;
+    }
+  }
   return HG_OK;
 }
 
@@ -1047,6 +1463,7 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
   if (!rc) rc = dalloc(ctx, &ctx->g_L23, G);
   if (!rc) rc = dalloc(ctx, &ctx->in_off, b->n_inlet + 1);
   if (!rc) rc = dalloc(ctx, &ctx->in_cell, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->in_ghost, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_L53, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ, b->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ0, b->n_inlet);
@@ -1071,6 +1488,7 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
   if (!rc) rc = upload(ctx, ctx->g_L23, g_L23.data(), G);
   if (!rc) rc = upload(ctx, ctx->in_off, in_off.data(), b->n_inlet + 1);
   if (!rc) rc = upload(ctx, ctx->in_cell, in_cell.data(), ctx->n_inlet_faces);
+  if (!rc && ctx->n_inlet_faces) rc = upload(ctx, ctx->in_ghost, b->inlet_ghost, ctx->n_inlet_faces);
   if (!rc) rc = upload(ctx, ctx->in_L53, in_L53.data(), ctx->n_inlet_faces);
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
@@ -1338,6 +1756,124 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+// ------------------------------------------------------------ adjoint ABI --
+static int adj_upload_lam(hg_ctx* ctx, const double* hbar, double* dst) {
+  const int64_t N = ctx->N;
+  std::vector<double> tmp(3 * N);
+  for (int c = 0; c < 3; ++c)
+    for (int64_t i = 0; i < N; ++i) tmp[c * N + i] = hbar[c * N + ctx->perm[i]];
+  HG_TRY(hipMemcpyAsync(dst, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+static int adj_download(hg_ctx* ctx, const double* lam_dev, double* Q_bar, double* n_bar, double* zb_bar,
+                        double* inletQ_bar) {
+  const int64_t N = ctx->N;
+  int rc = 0;
+  if (Q_bar) {
+    double* src[3] = {const_cast<double*>(lam_dev), const_cast<double*>(lam_dev) + N,
+                      const_cast<double*>(lam_dev) + 2 * N};
+    rc = state_out(ctx, src, Q_bar);
+  }
+  if (!rc && n_bar) rc = cell_out(ctx, ctx->adj_nbar, n_bar);
+  if (!rc && zb_bar) rc = cell_out(ctx, ctx->adj_zbbar, zb_bar);
+  if (!rc && inletQ_bar && ctx->n_inlet) {
+    HG_TRY(hipMemcpyAsync(inletQ_bar, ctx->adj_Qbar, ctx->n_inlet * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return rc;
+}
+
+int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_params, const double* dQdt_bar,
+               double* Q_bar, double* n_bar, double* zb_bar, double* inletQ_bar) {
+  if (!ctx || !Q || !dQdt_bar) return ctx ? (ctx->err = "hg_rhs_vjp: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  const int64_t N = ctx->N;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (!rc) rc = ensure_adj(ctx, 1, 1);
+  if (!rc) rc = state_in(ctx, Q, 0);
+  if (!rc) rc = adj_upload_lam(ctx, dQdt_bar, ctx->adj_lam[0]);
+  if (!rc) rc = zero_adj_accumulators(ctx);
+  if (!rc) rc = clear_flags(ctx);
+  if (rc) return rc;
+  ctx->cur = 0;
+  const double* lam[3] = {ctx->adj_lam[0], ctx->adj_lam[0] + N, ctx->adj_lam[0] + 2 * N};
+  double* out[3] = {ctx->adj_lam[1], ctx->adj_lam[1] + N, ctx->adj_lam[1] + 2 * N};
+  rc = enqueue_adjoint(ctx, ctx->st[0], lam, nullptr, out, 1.0, false);
+  if (!rc) rc = adj_download(ctx, ctx->adj_lam[1], Q_bar, n_bar, zb_bar, inletQ_bar);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+static int adj_run(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
+                   int64_t kck) {
+  if (nsteps <= 0 || kck <= 0) {
+    ctx->err = "nsteps and the checkpoint interval must be positive";
+    return HG_ERR_ARG;
+  }
+  const int64_t nseg = (nsteps + kck - 1) / kck;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (!rc) rc = ensure_adj(ctx, kck, nseg);
+  if (!rc) rc = hg_state_upload(ctx, Q0);
+  if (!rc) rc = clear_flags(ctx);
+  if (!rc) rc = forward_checkpointed(ctx, dt, nsteps, kck);
+  return rc;
+}
+
+int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
+               int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar,
+               double* inletQ_bar) {
+  if (!ctx || !Q0 || !QN_bar) return ctx ? (ctx->err = "hg_adjoint: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  int rc = adj_run(ctx, Q0, params, n_params, dt, nsteps, kck);
+  if (!rc) rc = adj_upload_lam(ctx, QN_bar, ctx->adj_lam[0]);
+  if (!rc) rc = zero_adj_accumulators(ctx);
+  int cur = 0;
+  if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
+  if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], Q0_bar, n_bar, zb_bar, inletQ_bar);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
+                   int64_t kck, int64_t n_probe, const int32_t* probe, const double* target, double* loss,
+                   double* n_bar, double* zb_bar, double* inletQ_bar) {
+  if (!ctx || !Q0 || (n_probe > 0 && (!probe || !target)) || !loss)
+    return ctx ? (ctx->err = "hg_grad_probes: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  const int64_t N = ctx->N;
+  for (int64_t k = 0; k < n_probe; ++k)
+    if (probe[k] < 0 || probe[k] >= N) {
+      ctx->err = "probe cell out of range";
+      return HG_ERR_ARG;
+    }
+  int rc = adj_run(ctx, Q0, params, n_params, dt, nsteps, kck);
+  if (rc) return rc;
+  // WSE = (xi + h_still) + zb at the probes (swe_2D_inversion.jl:387-399)
+  std::vector<double> QN(3 * N), hs(N), zb(N);
+  rc = hg_state_download(ctx, QN.data());
+  if (!rc) rc = cell_out(ctx, ctx->hstill, hs.data());
+  if (!rc) rc = cell_out(ctx, ctx->zb, zb.data());
+  if (rc) return rc;
+  std::vector<double> lamN(3 * N, 0.0), r(n_probe);
+  double L = 0.0;
+  for (int64_t k = 0; k < n_probe; ++k) {
+    const int32_t p = probe[k];
+    r[k] = QN[p] + hs[p] + zb[p] - target[k];
+    L = L + r[k] * r[k];
+    lamN[p] += 2.0 * r[k];
+  }
+  *loss = L;
+  rc = adj_upload_lam(ctx, lamN.data(), ctx->adj_lam[0]);
+  if (!rc) rc = zero_adj_accumulators(ctx);
+  int cur = 0;
+  if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
+  if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], nullptr, n_bar, zb_bar, inletQ_bar);
+  if (!rc) rc = check_flags(ctx);
+  if (!rc && zb_bar)  // the loss reads zb directly
+    for (int64_t k = 0; k < n_probe; ++k) zb_bar[probe[k]] += 2.0 * r[k];
   return rc;
 }
 

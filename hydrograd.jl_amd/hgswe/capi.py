@@ -24,7 +24,7 @@ HG_CELL_KERNEL = 2
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
-    "hg_info", "hg_probe_copy",
+    "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes",
 ]
 
 
@@ -75,6 +75,12 @@ def lib():
         L.hg_time_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.hg_probe_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+        L.hg_rhs_vjp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p] + [C.c_void_p] * 4
+        L.hg_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_int64, C.c_int64,
+                                 C.c_void_p] + [C.c_void_p] * 4
+        L.hg_grad_probes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_int64,
+                                     C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)] + \
+            [C.c_void_p] * 3
         _LIB = L
     return _LIB
 
@@ -179,6 +185,47 @@ class Context:
         dt = C.c_double()
         self._chk(lib().hg_cfl_dt(self.h, _ptr(Q), C.c_double(cfl), C.byref(dt)))
         return dt.value
+
+    def rhs_vjp(self, Q, ybar, params=None):
+        """Cotangents (Q_bar, n_bar, zb_bar, inletQ_bar) of one RHS evaluation."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        y = np.ascontiguousarray(ybar, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        qb = np.empty(3 * self.N)
+        nb = np.empty(self.N)
+        zb = np.empty(self.N)
+        ib = np.zeros(max(1, self.case.mesh.n_inlet))
+        self._chk(lib().hg_rhs_vjp(self.h, _ptr(Q), _ptr(p), 0 if p is None else p.size, _ptr(y), _ptr(qb),
+                                   _ptr(nb), _ptr(zb), _ptr(ib)))
+        return qb, nb, zb, ib[: self.case.mesh.n_inlet]
+
+    def adjoint(self, Q0, dt, nsteps, kck, QN_bar, params=None):
+        Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
+        lb = np.ascontiguousarray(QN_bar, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        qb = np.empty(3 * self.N)
+        nb = np.empty(self.N)
+        zb = np.empty(self.N)
+        ib = np.zeros(max(1, self.case.mesh.n_inlet))
+        self._chk(lib().hg_adjoint(self.h, _ptr(Q0), _ptr(p), 0 if p is None else p.size, C.c_double(dt),
+                                   C.c_int64(nsteps), C.c_int64(kck), _ptr(lb), _ptr(qb), _ptr(nb), _ptr(zb),
+                                   _ptr(ib)))
+        return qb, nb, zb, ib[: self.case.mesh.n_inlet]
+
+    def grad_probes(self, Q0, dt, nsteps, kck, probe, target, params=None):
+        """(loss, d/dn, d/dzb, d/dQ_inlet) of sum_p (WSE_p - target_p)^2 after nsteps."""
+        Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
+        pr = np.ascontiguousarray(probe, dtype=np.int32)
+        tg = np.ascontiguousarray(target, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        loss = C.c_double()
+        nb = np.empty(self.N)
+        zb = np.empty(self.N)
+        ib = np.zeros(max(1, self.case.mesh.n_inlet))
+        self._chk(lib().hg_grad_probes(self.h, _ptr(Q0), _ptr(p), 0 if p is None else p.size, C.c_double(dt),
+                                       C.c_int64(nsteps), C.c_int64(kck), C.c_int64(pr.size), _ptr(pr), _ptr(tg),
+                                       C.byref(loss), _ptr(nb), _ptr(zb), _ptr(ib)))
+        return loss.value, nb, zb, ib[: self.case.mesh.n_inlet]
 
     def time_advance(self, dt, nsteps, reps):
         a = C.c_double()

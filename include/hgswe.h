@@ -154,6 +154,35 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel);
 
+/* ---------------------------------------------------------- adjoint path */
+
+/* Vector-Jacobian product of one swe_2d_rhs evaluation (the pullback Zygote
+ * takes through semi_discretize_swe_2D.jl:18-277): given the cotangent of
+ * dQdt (3N), returns the cotangents of Q (3N), of the per-cell Manning n
+ * (N), of the bed zb (N; zb enters through S0 and zb_ghost rebuilt from zb as
+ * on the active-zb path, process_bed_2D.jl:46-66, and through the exit-H
+ * depth, bc_2D.jl:763-770) and of the inlet discharges (n_inlet).  Any
+ * output pointer may be NULL.  Branch tests and clamps carry no derivative
+ * (the semantics ForwardDiff / Zygote give the reference). */
+int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_params, const double* dQdt_bar,
+               double* Q_bar, double* n_bar, double* zb_bar, double* inletQ_bar);
+
+/* Reverse-mode adjoint of nsteps Euler steps (custom_ODE_update_cells,
+ * custom_ODE_solvers.jl:5-33) from Q0: forward sweep storing a checkpoint
+ * every kck steps, then a reverse sweep that recomputes each segment from
+ * its checkpoint.  QN_bar is the cotangent of the final state; outputs are
+ * the cotangents of Q0, n, zb and the inlet discharges (NULL to skip). */
+int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
+               int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar, double* inletQ_bar);
+
+/* Gradient of the probe loss L = sum_p (xi_p + h_still_p + zb_p - target_p)^2
+ * at the final state (the WSE misfit of swe_2D_inversion.jl:387-399 without
+ * its normalisation) after nsteps steps from Q0, with respect to n, zb and
+ * the inlet discharges; one checkpointed forward + reverse sweep. */
+int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
+                   int64_t kck, int64_t n_probe, const int32_t* probe, const double* target, double* loss,
+                   double* n_bar, double* zb_bar, double* inletQ_bar);
+
 /* Measurement helper with no counterpart in the reference: copies nbytes
  * between two scratch device buffers `reps` times with a plain streaming
  * kernel and reports the achieved bandwidth (read + write bytes / time).  Its

@@ -130,6 +130,35 @@ double orc_cfl_dt(const Mesh* m, const Consts* c, const double* hstill, const do
   return orc::cfl_dt(*m, *c, hstill, Q, cfl);
 }
 
+// Forward-mode derivative of one swe_2d_rhs evaluation along a tangent on
+// (Q, n, zb, inletQ); a non-NULL dzb rebuilds the bed data from the Dual zb
+// (the active-zb path, semi_discretize_swe_2D.jl:114-126).
+int orc_rhs_jvp(const Mesh* m, const Consts* c, const FieldPtrs* f, const double* Q, const double* dQ,
+                const double* dn, const double* dzb, const double* dQin, double* out, double* dout) {
+  return guarded([&] {
+    using D = Dual<1>;
+    Fields<D> p = make_fields<D>(*m, *f);
+    if (dn)
+      for (int64_t i = 0; i < m->N; ++i) p.n[i].d[0] = dn[i];
+    if (dQin)
+      for (int b = 0; b < m->n_inlet; ++b) p.inletQ[b].d[0] = dQin[b];
+    if (dzb) {
+      for (int64_t i = 0; i < m->N; ++i) p.zb[i].d[0] = dzb[i];
+      orc::update_bed_data(*m, p.zb, p.zb_ghost, p.S0x, p.S0y);
+    }
+    std::vector<D> q(3 * m->N), r;
+    for (int64_t k = 0; k < 3 * m->N; ++k) {
+      q[k] = D(Q[k]);
+      if (dQ) q[k].d[0] = dQ[k];
+    }
+    orc::swe_2d_rhs(*m, *c, p, q, r);
+    for (int64_t k = 0; k < 3 * m->N; ++k) {
+      out[k] = r[k].v;
+      dout[k] = r[k].d[0];
+    }
+  });
+}
+
 // Forward-mode (Dual, one tangent direction) run of nsteps Euler steps with
 // tangents on the per-cell Manning n, the bed zb and the inlet discharges.
 // Outputs the final state, its tangent and the probe loss

@@ -127,6 +127,17 @@ class Oracle:
         hs = np.ascontiguousarray(self.case.hstill, dtype=D)
         return lib().orc_cfl_dt(C.byref(self.mesh), C.byref(self.consts), _p(hs), _p(Q), C.c_double(cfl))
 
+    def rhs_jvp(self, Q, dQ=None, dn=None, dzb=None, dQin=None):
+        N = self.case.N
+        Q = np.ascontiguousarray(Q, dtype=D)
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=D) for a in (dQ, dn, dzb, dQin)]
+        out = np.empty(3 * N)
+        dout = np.empty(3 * N)
+        self._check(lib().orc_rhs_jvp(C.byref(self.mesh), C.byref(self.consts), C.byref(self.fields), _p(Q),
+                                      *[(_p(a) if a is not None else C.c_void_p(0)) for a in arrs], _p(out),
+                                      _p(dout)))
+        return out, dout
+
     def jvp_run(self, Q0, dt, nsteps, dn=None, dzb=None, dQin=None, probe=None, target=None):
         N = self.case.N
         Q0 = np.ascontiguousarray(Q0, dtype=D)

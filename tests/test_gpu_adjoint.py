@@ -1,0 +1,181 @@
+"""GPU parity of the hand-written adjoint (libhgswe's reverse mode).
+
+Pins:
+* one RHS evaluation: <ybar, J v> from the oracle's forward-mode (Dual)
+  derivative equals <J^T ybar, v> from hg_rhs_vjp, for tangents on Q, the
+  per-cell Manning n, the bed zb (bed data rebuilt from zb) and the inlet
+  discharge -- exact identity up to rounding (tolerance 1e-10 relative);
+* the checkpointed gradient of the probe WSE loss after N steps: g . v
+  against the oracle's Dual run of the same N steps (1e-8 relative) and
+  against central finite differences of the GPU forward solve;
+* checkpoint-interval invariance and run-to-run determinism (bitwise).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from hgswe import capi, cases
+
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_channel(seed=21):
+    return cases.channel(6_000, seed=seed, Lx=1500.0, Ly=150.0, Q_in=60.0, nsteps=60)
+
+
+def wet_dry_case():
+    c = cases.channel(6_000, seed=12, Lx=1500.0, Ly=150.0, Q_in=60.0, nsteps=60)
+    x = c.mesh.centroid[:, 0]
+    h = np.where(x < 500.0, np.maximum(c.h_small, 2.2 - c.zb), c.h_small)
+    Q0 = c.Q0.copy()
+    Q0[: c.N] = h - c.hstill
+    return c.copy_fields(Q0=Q0, dt=cases.cfl_dt(c, 0.4, Q0), name="wet_dry")
+
+
+def golden(name, tmp_path):
+    from conftest import load_golden_case
+    from test_oracle_golden import savannah_ic_path
+
+    if name == "bump":
+        c = load_golden_case("bump", ic=(0.33, 0.2, 0.0, 0.0))
+        return c.copy_fields(dt=0.01)
+    c = load_golden_case("savannah", ic=savannah_ic_path(tmp_path))
+    return c.copy_fields(dt=0.2 * cases.cfl_dt(c, 1.0))
+
+
+def vjp_check(c, Q, seed=0):
+    ctx, orc = capi.Context(c), pyoracle.Oracle(c)
+    rng = np.random.default_rng(seed)
+    N = c.N
+    ybar = rng.normal(size=3 * N)
+    qb, nb, zbb, ib = ctx.rhs_vjp(Q, ybar)
+    tangents = {
+        "Q": dict(dQ=rng.normal(size=3 * N) * np.concatenate([np.full(N, 1e-2), np.full(2 * N, 1e-2)])),
+        "n": dict(dn=rng.normal(size=N) * 1e-3),
+        "zb": dict(dzb=rng.normal(size=N) * 1e-2),
+    }
+    if c.mesh.n_inlet:
+        tangents["Qin"] = dict(dQin=rng.normal(size=c.mesh.n_inlet))
+    for name, tg in tangents.items():
+        _, jv = orc.rhs_jvp(Q, **tg)
+        lhs = float(np.dot(ybar, jv))
+        v = tg.get("dQ", tg.get("dn", tg.get("dzb", tg.get("dQin"))))
+        g = {"Q": qb, "n": nb, "zb": zbb, "Qin": ib}[name]
+        rhs = float(np.dot(g, v))
+        scale = float(np.sum(np.abs(ybar * jv))) + 1e-300
+        say(f"  vjp {c.name} [{name}]: <y,Jv>={lhs:.12e} <J^T y,v>={rhs:.12e} rel={abs(lhs - rhs) / scale:.2e}")
+        assert abs(lhs - rhs) <= 1e-10 * scale
+
+
+def test_rhs_vjp_channel():
+    c = small_channel()
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 30)
+    vjp_check(c, c.Q0, seed=1)
+    vjp_check(c, Q, seed=2)
+
+
+def test_rhs_vjp_wet_dry():
+    c = wet_dry_case()
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 40)
+    h = Q[: c.N] + c.hstill
+    say(f"  dry cells: {int(np.sum(h <= c.h_small))}")
+    assert np.sum(h <= c.h_small) > 50
+    vjp_check(c, Q, seed=3)
+
+
+@pytest.mark.parametrize("name", ["bump", "savannah"])
+def test_rhs_vjp_reference_cases(name, tmp_path):
+    c = golden(name, tmp_path)
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 50)
+    vjp_check(c, Q, seed=4)
+
+
+def _probe_problem(c, nsteps, seed=5, nprobe=30):
+    rng = np.random.default_rng(seed)
+    probe = rng.choice(c.N, size=nprobe, replace=False).astype(np.int32)
+    truth = c.copy_fields(n=c.n * 1.15)
+    QT = pyoracle.Oracle(truth).run(c.Q0, c.dt, nsteps)
+    target = QT[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.01, nprobe)  # observation noise
+    return probe, target
+
+
+@pytest.mark.parametrize("maker", [small_channel, wet_dry_case])
+def test_gradient_matches_oracle_dual_run(maker):
+    c = maker()
+    nsteps, kck = 60, 7
+    probe, target = _probe_problem(c, nsteps)
+    ctx, orc = capi.Context(c), pyoracle.Oracle(c)
+    loss, gn, gzb, gq = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    rng = np.random.default_rng(7)
+    dirs = {"n": dict(dn=rng.normal(size=c.N) * 1e-3), "zb": dict(dzb=rng.normal(size=c.N) * 1e-2),
+            "Qin": dict(dQin=rng.normal(size=c.mesh.n_inlet))}
+    for name, tg in dirs.items():
+        _, _, L, dL = orc.jvp_run(c.Q0, c.dt, nsteps, probe=probe, target=target, **tg)
+        v = list(tg.values())[0]
+        g = {"n": gn, "zb": gzb, "Qin": gq}[name]
+        gv = float(np.dot(g, v))
+        say(f"  grad {c.name} [{name}]: loss gpu {loss:.12e} oracle {L:.12e}; g.v {gv:.12e} oracle JVP {dL:.12e}")
+        assert abs(loss - L) <= 1e-10 * abs(L)
+        assert abs(gv - dL) <= 1e-8 * abs(dL)
+
+
+def test_checkpoint_interval_invariance_and_determinism():
+    c = small_channel()
+    nsteps = 40
+    probe, target = _probe_problem(c, nsteps)
+    ctx = capi.Context(c)
+    ref = ctx.grad_probes(c.Q0, c.dt, nsteps, nsteps, probe, target)
+    for kck in (1, 7, 13):
+        out = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+        assert out[0] == ref[0]
+        for a, b in zip(out[1:], ref[1:]):
+            assert np.array_equal(a, b)
+
+
+def test_gradient_matches_central_differences():
+    """Central differences of the GPU forward solve along random directions
+    (the C2-style check at reduced size)."""
+    c = cases.channel(20_000, seed=31, Lx=2000.0, Ly=200.0, Q_in=100.0, nsteps=200)
+    nsteps, kck = 200, 50
+    probe, target = _probe_problem(c, nsteps, nprobe=60)
+    ctx = capi.Context(c)
+    loss, gn, gzb, gq = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    rng = np.random.default_rng(11)
+
+    def L(n=None, zb=None, Qin=None):
+        cc = c.copy_fields(n=c.n if n is None else n, inletQ=c.inletQ if Qin is None else Qin)
+        if zb is not None:
+            from hgswe import mesh as hm
+
+            zg, sx, sy = hm.bed_data(c.mesh, zb)
+            cc = cc.copy_fields(zb=zb, zb_ghost=zg, S0x=sx, S0y=sy)
+        k = capi.Context(cc)
+        QN = k.step(c.Q0, c.dt, nsteps)
+        r = QN[probe] + cc.hstill[probe] + cc.zb[probe] - target
+        return float(np.sum(r * r))
+
+    for trial in range(4):
+        vn = rng.normal(size=c.N) * c.n
+        e = 1e-5
+        fd = (L(n=c.n + e * vn) - L(n=c.n - e * vn)) / (2 * e)
+        gv = float(np.dot(gn, vn))
+        say(f"  FD n dir {trial}: fd {fd:.10e} adjoint {gv:.10e}")
+        assert abs(fd - gv) <= 5e-5 * abs(gv) + 1e-12
+    vz = rng.normal(size=c.N) * 0.01
+    e = 1e-4
+    fd = (L(zb=c.zb + e * vz) - L(zb=c.zb - e * vz)) / (2 * e)
+    gv = float(np.dot(gzb, vz))
+    say(f"  FD zb: fd {fd:.10e} adjoint {gv:.10e}")
+    assert abs(fd - gv) <= 5e-5 * abs(gv) + 1e-12
+    e = 1e-3
+    fd = (L(Qin=c.inletQ + e) - L(Qin=c.inletQ - e)) / (2 * e)
+    say(f"  FD Q: fd {fd:.10e} adjoint {gq[0]:.10e}")
+    assert abs(fd - gq[0]) <= 5e-5 * abs(gq[0]) + 1e-12
