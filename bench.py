@@ -9,6 +9,8 @@ Prints one JSON line (see the keys below).  Single GPU only.
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -60,6 +62,67 @@ def cpu_baseline(case, seconds):
             "sample": f"{n} Euler steps of the same {case.N}-cell mesh, C++ oracle -O3, 1 thread"}
 
 
+def gradient_case(cells, steps, seed=2, n_probe=200):
+    """Adjoint workload: the channel generator at ~`cells` triangles, per-cell
+    n = 0.03 as the initial guess, probe targets from a forward run with a
+    'true' lognormal n (mean 0.035, sigma 0.3), 200 seeded probe cells."""
+    from hgswe import cases
+
+    c = cases.channel(cells, seed=seed, nsteps=steps)
+    rng = np.random.default_rng(seed + 100)
+    sig = 0.3
+    n_true = np.exp(rng.normal(np.log(0.035) - 0.5 * sig * sig, sig, c.N))
+    probe = rng.choice(c.N, size=n_probe, replace=False).astype(np.int32)
+    return c.copy_fields(n=np.full(c.N, 0.03)), n_true, probe
+
+
+def cpu_gradient_baseline(case, probe, target, seconds):
+    """CPU cost of a gradient: the oracle's forward Euler steps plus a
+    forward-mode (Dual) tangent sweep of the same steps, one thread.  A CPU
+    reverse-mode gradient needs at least one forward and one sweep of
+    tangent-linear cost, so this is a lower bound on its time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    orc = pyoracle.Oracle(case)
+    pyoracle.set_threads(1)
+    dn = np.ones(case.N) * 1e-3
+    t0 = time.perf_counter()
+    orc.run(case.Q0, case.dt, 1)
+    orc.jvp_run(case.Q0, case.dt, 1, dn=dn, probe=probe, target=target)
+    per = time.perf_counter() - t0
+    n = max(1, min(200, int(seconds / max(per, 1e-3))))
+    t0 = time.perf_counter()
+    orc.run(case.Q0, case.dt, n)
+    orc.jvp_run(case.Q0, case.dt, n, dn=dn, probe=probe, target=target)
+    el = time.perf_counter() - t0
+    return {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle forward + Dual tangent sweep "
+                      "(lower bound on a CPU reverse-mode gradient), 1 thread"}
+
+
+def measure_gradient(args, capi):
+    c, n_true, probe = gradient_case(args.grad_cells, args.grad_steps)
+    ctx = capi.Context(c)
+    ctx.set_active_param(capi.HG_PARAM_N_CELL)
+    QT = ctx.step(c.Q0, c.dt, args.grad_steps, params=n_true)
+    target = QT[probe] + c.hstill[probe] + c.zb[probe]
+    ctx.grad_probes(c.Q0, c.dt, 20, args.grad_kck, probe, target, params=c.n)  # warm-up
+    t0 = time.perf_counter()
+    loss, gn, gzb, gq = ctx.grad_probes(c.Q0, c.dt, args.grad_steps, args.grad_kck, probe, target, params=c.n)
+    el = time.perf_counter() - t0
+    out = {"value": c.N * args.grad_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+           "config": {"workload": "channel_100k_tri_probe_loss_gradient", "cells": c.N, "steps": args.grad_steps,
+                      "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "per-cell Manning n",
+                      "dt": c.dt},
+           "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
+    if not args.no_cpu:
+        cb = cpu_gradient_baseline(c, probe, target, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +131,10 @@ def main():
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grad-cells", type=int, default=100_000)
+    ap.add_argument("--grad-steps", type=int, default=2000)
+    ap.add_argument("--grad-kck", type=int, default=50)
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(calibrates rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -119,6 +186,8 @@ def main():
         out["copy_probe"] = {"bytes_each_way": nbytes, "GBps": capi.probe_copy(ctx, nbytes, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    if not args.no_grad:
+        out["gradient"] = measure_gradient(args, capi)
     print(json.dumps(out), flush=True)
 
 
