@@ -97,7 +97,8 @@ struct TileArgs {
   const int32_t* deg;
   const int32_t* item_off;
   const struct ItemRec* items;
-  const uint32_t* tile_inlets;  // bit b set: tile has ghosts of inlet b
+  const int32_t* tile_in_off;   // [ntiles+1] into tile_in_list
+  const int32_t* tile_in_list;  // inlets with ghosts in the tile; a ghost item's local slot indexes this list
   const int32_t* g_kind;
   const int32_t* g_bc;
   const double* g_hstill;
@@ -112,6 +113,7 @@ struct TileArgs {
   int32_t* flag;
   uint8_t* clamp;  // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
+  int32_t xcd_map;  // 1: XCD-contiguous block -> tile mapping, 0: identity
 };
 
 // Arguments of the adjoint (VJP) kernel of one RHS evaluation / one step.
@@ -200,7 +202,13 @@ struct hg_ctx {
   int32_t* item_off = nullptr;
   hg::ItemRec* items = nullptr;
   int32_t tileMD = 3;
-  uint32_t* tile_inlets = nullptr;
+  int32_t xcd_map = 1;
+  int32_t *tile_in_off = nullptr, *tile_in_list = nullptr;
+  std::vector<void*> tile_allocs;  // device buffers of the current tiling (freed on re-tiling)
+  size_t lds_max = 65536;          // dynamic LDS a work-group may use on this device
+  // host copies of the ELL face arrays and ghost tables (lib order), kept for re-tiling
+  std::vector<int32_t> h_deg, h_nbr, h_gkind, h_gbc;
+  std::vector<double> h_nx, h_ny, h_len;
   double* scratch = nullptr;  // 3N doubles for host transfers / params
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials

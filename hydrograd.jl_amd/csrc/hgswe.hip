@@ -47,8 +47,9 @@ static std::string g_create_err;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGraphSteps = 16;
-constexpr int kXCD = 8;  // MI355X: 8 accelerator complex dies, one L2 each  // steps per captured graph (even: the graph returns to buffer 0)
+constexpr int kGraphSteps = 16;  // steps per captured graph (even: the graph returns to buffer 0)
+constexpr int kXCD = 8;         // MI355X: 8 accelerator complex dies, one L2 each
+constexpr int kTileInlets = 64;  // inlet boundaries one tile may touch when the conveyance is fused
 
 template <class T>
 int dalloc(hg_ctx* ctx, T** p, size_t n) {
@@ -232,13 +233,13 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
   const int nt = gridDim.x;
   const int q = nt / kXCD, r = nt % kXCD;
   const int x = blockIdx.x % kXCD;
-  const int t = x * q + min(x, r) + blockIdx.x / kXCD;
+  const int t = a.xcd_map ? x * q + min(x, r) + blockIdx.x / kXCD : int(blockIdx.x);
   const int li = threadIdx.x;
   const int i = t * T + li;
   const bool valid = i < a.N;
   double* C = lds;               // 10 x T cell quantities
   double* FL = lds + 10 * T;     // 3 x MD x T face contributions
-  double* tA = FL + 3 * MD * T;  // conveyance per inlet (32)
+  double* tA = FL + 3 * MD * T;  // conveyance of the tile's inlets (kTileInlets)
   const double hsmall = a.hsmall;
 
   double xi = 0.0, qx0 = 0.0, qy0 = 0.0, hs_i = 0.0, zb_i = 0.0;
@@ -264,10 +265,9 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
     C[8 * T + li] = me.s; C[9 * T + li] = me.p;
   }
   if (a.n_inlet > 0 && li < 64) {  // wave 0: inlet conveyance (bc_2D.jl:640-680)
-    uint32_t mask = a.tile_inlets[t];
-    while (mask) {
-      const int b = __ffs(mask) - 1;
-      mask &= mask - 1;
+    const int s0 = a.tile_in_off[t], s1 = a.tile_in_off[t + 1];
+    for (int sl = s0; sl < s1; ++sl) {
+      const int b = a.tile_in_list[sl];
       const int f0 = a.in_off[b], f1 = a.in_off[b + 1];
       double acc = 0.0;
       for (int base = f0; base < f1; base += 64) {
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
         }
       }
       if (li == 0) {
-        tA[b] = acc;
+        tA[sl - s0] = acc;
         if (!(acc > 1e-10)) a.flag[0] = 1;
       }
     }
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
         case HG_BC_INLETQ: {
           const int b = a.g_bc[gi];
           const double w = (A.h > hsmall) ? 1.0 : 0.0;
-          const double tot = (a.n_inlet > 0) ? tA[b] : a.totalA[b];
+          const double tot = (a.n_inlet > 0) ? tA[(code >> 14) & (kTileInlets - 1)] : a.totalA[b];
           const double v = a.inletQ[b] / tot * a.g_L23[gi] / a.n[t * T + ia];
           hg_ = A.h;
           qxg = -A.h * v * nx * w;
@@ -690,27 +690,34 @@ uint64_t spread21(uint64_t v) {
   return v;
 }
 
-std::vector<int32_t> morton_perm(int64_t N, const double* xy) {
-  double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
-  for (int64_t i = 0; i < N; ++i)
-    for (int d = 0; d < 2; ++d) {
-      lo[d] = std::min(lo[d], xy[2 * i + d]);
-      hi[d] = std::max(hi[d], xy[2 * i + d]);
-    }
-  const double span = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
-  std::vector<std::pair<uint64_t, int32_t>> key(N);
-  const double scale = double((1u << 21) - 1) / span;
-  for (int64_t i = 0; i < N; ++i) {
-    uint64_t qx = uint64_t(std::min(std::max((xy[2 * i] - lo[0]) * scale, 0.0), double((1u << 21) - 1)));
-    uint64_t qy = uint64_t(std::min(std::max((xy[2 * i + 1] - lo[1]) * scale, 0.0), double((1u << 21) - 1)));
-    key[i] = {spread21(qx) | (spread21(qy) << 1), int32_t(i)};
-  }
-  std::stable_sort(key.begin(), key.end(),
-                   [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
-                     return a.first < b.first;
-                   });
+// Morton (Z-order) renumbering of the cell centroids, separately within
+// each segment [seg[s], seg[s+1]) so that batched instances stay contiguous.
+std::vector<int32_t> morton_perm(const std::vector<int64_t>& seg, const double* xy) {
+  const int64_t N = seg.back();
   std::vector<int32_t> perm(N);
-  for (int64_t i = 0; i < N; ++i) perm[i] = key[i].second;
+  std::vector<std::pair<uint64_t, int32_t>> key;
+  for (size_t sg = 0; sg + 1 < seg.size(); ++sg) {
+    const int64_t a = seg[sg], b = seg[sg + 1];
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int64_t i = a; i < b; ++i)
+      for (int d = 0; d < 2; ++d) {
+        lo[d] = std::min(lo[d], xy[2 * i + d]);
+        hi[d] = std::max(hi[d], xy[2 * i + d]);
+      }
+    const double span = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+    const double scale = double((1u << 21) - 1) / span;
+    key.resize(b - a);
+    for (int64_t i = a; i < b; ++i) {
+      uint64_t qx = uint64_t(std::min(std::max((xy[2 * i] - lo[0]) * scale, 0.0), double((1u << 21) - 1)));
+      uint64_t qy = uint64_t(std::min(std::max((xy[2 * i + 1] - lo[1]) * scale, 0.0), double((1u << 21) - 1)));
+      key[i - a] = {spread21(qx) | (spread21(qy) << 1), int32_t(i)};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int32_t>& x, const std::pair<uint64_t, int32_t>& y) {
+                       return x.first < y.first;
+                     });
+    for (int64_t i = a; i < b; ++i) perm[i] = key[i - a].second;
+  }
   return perm;
 }
 
@@ -718,10 +725,13 @@ std::vector<int32_t> morton_perm(int64_t N, const double* xy) {
 // first (the kernel prefetches their neighbour states before its first
 // barrier), then boundary faces, then intra-tile faces (owned by the side
 // with the lower local index), each group ordered by cell.
+// Inlet ghosts carry the slot of their inlet in the tile's inlet list in
+// code bits 14..19; max_inlets is the longest such list.
 struct TileBuild {
   std::vector<int32_t> off, code, ref;
   std::vector<double> nx, ny, len;
-  std::vector<uint32_t> inlets;
+  std::vector<int32_t> in_off, in_list;
+  int32_t max_inlets = 0;
 };
 
 TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const std::vector<int32_t>& nbr,
@@ -730,11 +740,13 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
   TileBuild tb;
   const int64_t ntiles = (N + T - 1) / T;
   tb.off.assign(ntiles + 1, 0);
-  tb.inlets.assign(ntiles, 0u);
+  tb.in_off.assign(ntiles + 1, 0);
   std::vector<int64_t> grp[3];
+  std::vector<int32_t> local;  // this tile's inlets in order of first appearance
   for (int64_t t = 0; t < ntiles; ++t) {
     const int64_t c0 = t * T, c1 = std::min<int64_t>(N, c0 + T);
     for (auto& v : grp) v.clear();
+    local.clear();
     for (int64_t i = c0; i < c1; ++i) {
       const int32_t li = int32_t(i - c0);
       for (int32_t k = 0; k < deg[i]; ++k) {
@@ -745,7 +757,12 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
           type = 2;
           ref = ~j;
           code = li | (k << 10) | (type << 28);
-          if (g_kind[ref] == HG_BC_INLETQ && g_bc[ref] < 32) tb.inlets[t] |= (1u << g_bc[ref]);
+          if (g_kind[ref] == HG_BC_INLETQ) {
+            int32_t sl = 0;
+            while (sl < int32_t(local.size()) && local[sl] != g_bc[ref]) ++sl;
+            if (sl == int32_t(local.size())) local.push_back(g_bc[ref]);
+            code |= (sl & (kTileInlets - 1)) << 14;
+          }
         } else if (j >= c0 && j < c1) {
           const int32_t lj = int32_t(j - c0);
           if (lj < li) continue;  // owned by the other side
@@ -776,11 +793,16 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
         tb.len.push_back(flen[s]);
       }
     tb.off[t + 1] = int32_t(tb.code.size());
+    tb.in_list.insert(tb.in_list.end(), local.begin(), local.end());
+    tb.in_off[t + 1] = int32_t(tb.in_list.size());
+    tb.max_inlets = std::max(tb.max_inlets, int32_t(local.size()));
   }
   return tb;
 }
 
-size_t tile_lds_bytes(int T, int maxdeg) { return size_t(10 * T + 3 * maxdeg * T + 32) * sizeof(double); }
+size_t tile_lds_bytes(int T, int maxdeg) {
+  return size_t(10 * T + 3 * maxdeg * T + kTileInlets) * sizeof(double);
+}
 
 hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt,
                               uint8_t* clamp = nullptr) {
@@ -805,7 +827,9 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.deg = ctx->deg;
   a.item_off = ctx->item_off;
   a.items = ctx->items;
-  a.tile_inlets = ctx->tile_inlets;
+  a.tile_in_off = ctx->tile_in_off;
+  a.tile_in_list = ctx->tile_in_list;
+  a.xcd_map = ctx->xcd_map;
   a.g_kind = ctx->g_kind;
   a.g_bc = ctx->g_bc;
   a.g_hstill = ctx->g_hstill;
@@ -909,11 +933,10 @@ hg::InletArgs inlet_args(hg_ctx* ctx, int src) {
 
 int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
 
-// Tile kernel instances: T = 256 cells, MD = 3 (triangles), 4 (quads or
-// mixed), 8 (any polygon up to octagons).
-template <int MD>
-void launch_tile_md(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
-  constexpr int T = 256;
+// Tile kernel instances: T in {128, 256, 512} cells per tile (one work-item
+// per cell), MD = 3 (triangles), 4 (quads or mixed), 8 (up to octagons).
+template <int T, int MD>
+void launch_tile_tm(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
   const size_t lds = tile_lds_bytes(T, MD);
   if (step)
     hipLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
@@ -921,13 +944,23 @@ void launch_tile_md(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
     hipLaunchKernelGGL((k_swe_tile<false, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
 }
 
-void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+template <int T>
+void launch_tile_t(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
   if (ctx->tileMD == 3)
-    launch_tile_md<3>(ctx, a, step);
+    launch_tile_tm<T, 3>(ctx, a, step);
   else if (ctx->tileMD == 4)
-    launch_tile_md<4>(ctx, a, step);
+    launch_tile_tm<T, 4>(ctx, a, step);
   else
-    launch_tile_md<8>(ctx, a, step);
+    launch_tile_tm<T, 8>(ctx, a, step);
+}
+
+void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+  if (ctx->tileT == 128)
+    launch_tile_t<128>(ctx, a, step);
+  else if (ctx->tileT == 256)
+    launch_tile_t<256>(ctx, a, step);
+  else
+    launch_tile_t<512>(ctx, a, step);
 }
 
 // Enqueue one step (or one RHS evaluation) reading state `src`.
@@ -1241,12 +1274,90 @@ int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
   return HG_OK;
 }
 
-int ensure_graph(hg_ctx* ctx, double dt) {
-  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
+int drop_graph(hg_ctx* ctx) {
   if (ctx->gexec) {
+    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of it may still be running
     HG_TRY(hipGraphExecDestroy(ctx->gexec));
     ctx->gexec = nullptr;
   }
+  return HG_OK;
+}
+
+// (Re)build the tiled kernel's work items for tiles of T cells from the
+// host copies of the face arrays.  Any T gives bit-identical results: every
+// face flux is formed exactly as the per-side evaluation forms it.
+int setup_tiles(hg_ctx* ctx, int T) {
+  if (T != 128 && T != 256 && T != 512) {
+    ctx->err = "tile size must be 128, 256 or 512 cells";
+    return HG_ERR_ARG;
+  }
+  if (tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) {
+    ctx->err = "a tile of " + std::to_string(T) + " cells needs " + std::to_string(tile_lds_bytes(T, ctx->tileMD)) +
+               " bytes of LDS; the device allows " + std::to_string(ctx->lds_max);
+    return HG_ERR_ARG;
+  }
+  const int64_t N = ctx->N;
+  const int32_t maxdeg = ctx->maxdeg;
+  const std::vector<int32_t>& h_deg = ctx->h_deg;
+  TileBuild tb = build_tiles(N, T, h_deg, ctx->h_nbr, ctx->h_nx, ctx->h_ny, ctx->h_len, ctx->h_gkind, ctx->h_gbc);
+  const int64_t ntiles = (N + T - 1) / T;
+  const size_t ni = tb.code.size();
+  {  // every (cell, local face) slot must receive exactly one flux
+    std::vector<uint8_t> hits(size_t(maxdeg) * N, 0);
+    bool ok = true;
+    for (int64_t t = 0; t < ntiles && ok; ++t)
+      for (int32_t it = tb.off[t]; it < tb.off[t + 1]; ++it) {
+        const int32_t code = tb.code[it];
+        const int32_t ia = code & 1023, ka = (code >> 10) & 15, ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
+        const int32_t type = (code >> 28) & 3;
+        const int64_t ca = t * T + ia;
+        if (ia >= T || ca >= N || ka >= h_deg[ca]) { ok = false; break; }
+        hits[size_t(ka) * N + ca]++;
+        if (type == 0) {
+          const int64_t cb = t * T + ib;
+          if (ib >= T || cb >= N || kb >= h_deg[cb]) { ok = false; break; }
+          hits[size_t(kb) * N + cb]++;
+        }
+      }
+    for (int64_t i = 0; i < N && ok; ++i)
+      for (int32_t k = 0; k < maxdeg; ++k)
+        if (hits[size_t(k) * N + i] != (k < h_deg[i] ? 1 : 0)) { ok = false; break; }
+    if (!ok) {
+      ctx->err = "internal: tiled work items do not cover every face slot exactly once";
+      return HG_ERR_MESH;
+    }
+  }
+  std::vector<hg::ItemRec> recs(ni);
+  for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
+  // release the previous tiling (and the graph that captured it)
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  int rc = drop_graph(ctx);
+  if (rc) return rc;
+  for (void* p : ctx->tile_allocs) {
+    ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), p));
+    HG_TRY(hipFree(p));
+  }
+  ctx->tile_allocs.clear();
+  const size_t nalloc0 = ctx->allocs.size();
+  rc = dalloc(ctx, &ctx->item_off, tb.off.size());
+  if (!rc) rc = dalloc(ctx, &ctx->items, ni);
+  if (!rc) rc = dalloc(ctx, &ctx->tile_in_off, tb.in_off.size());
+  if (!rc) rc = dalloc(ctx, &ctx->tile_in_list, tb.in_list.size());
+  ctx->tile_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
+  if (!rc) rc = upload(ctx, ctx->item_off, tb.off.data(), tb.off.size());
+  if (!rc) rc = upload(ctx, ctx->items, recs.data(), ni);
+  if (!rc) rc = upload(ctx, ctx->tile_in_off, tb.in_off.data(), tb.in_off.size());
+  if (!rc) rc = upload(ctx, ctx->tile_in_list, tb.in_list.data(), tb.in_list.size());
+  if (rc) return rc;
+  ctx->tileT = T;
+  ctx->ntiles = int32_t(ntiles);
+  ctx->fuse_inlet = ctx->n_inlet > 0 && tb.max_inlets <= kTileInlets;
+  return HG_OK;
+}
+
+int ensure_graph(hg_ctx* ctx, double dt) {
+  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
+  if (drop_graph(ctx)) return HG_ERR_HIP;
   hipGraph_t graph;
   HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
@@ -1308,7 +1419,13 @@ const char* hg_info(void) {
 
 const char* hg_last_error(const hg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
-int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags) {
+}  // extern "C"
+
+namespace {
+// Context creation for one mesh; seg lists the cell offsets of batched
+// instances ({0, N} for a single mesh), within which cells are renumbered.
+int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags,
+                std::vector<int64_t> seg) {
   if (!out || !m || !b || !c) {
     g_create_err = "hg_create: NULL argument";
     return HG_ERR_ARG;
@@ -1394,8 +1511,9 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
     return fail(HG_ERR_HIP);
   }
   // ---- renumbering
+  if (seg.empty()) seg = {0, N};
   if ((flags & HG_REORDER) && m->centroid_xy) {
-    ctx->perm = morton_perm(N, m->centroid_xy);
+    ctx->perm = morton_perm(seg, m->centroid_xy);
   } else {
     ctx->perm.resize(N);
     std::iota(ctx->perm.begin(), ctx->perm.end(), 0);
@@ -1494,56 +1612,139 @@ int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* 
   if (rc) return fail(rc);
 
   // ---- tiled face-once kernel data
-  ctx->tiled = !(flags & HG_CELL_KERNEL);
+  // The tiled kernel's LDS slots are sized for at most 8 faces per cell;
+  // meshes with larger polygons run the per-cell kernel.
+  ctx->h_deg = std::move(h_deg);
+  ctx->h_nbr = std::move(h_nbr);
+  ctx->h_nx = std::move(h_nx);
+  ctx->h_ny = std::move(h_ny);
+  ctx->h_len = std::move(h_len);
+  ctx->h_gkind = std::move(g_kind);
+  ctx->h_gbc = std::move(g_bc);
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
+      ctx->lds_max = size_t(v);
+  }
+  ctx->tiled = !(flags & HG_CELL_KERNEL) && maxdeg <= 8;
   if (ctx->tiled) {
-    const int T = 256;
-    ctx->tileT = T;
     ctx->tileMD = maxdeg <= 3 ? 3 : (maxdeg <= 4 ? 4 : 8);
-    if (maxdeg > 8) {
-      ctx->err = "the tiled kernel supports cells with up to 8 faces (use HG_CELL_KERNEL)";
-      return fail(HG_ERR_MESH);
-    }
-    ctx->ntiles = int32_t((N + T - 1) / T);
-    ctx->fuse_inlet = b->n_inlet > 0 && b->n_inlet <= 32;
-    TileBuild tb = build_tiles(N, T, h_deg, h_nbr, h_nx, h_ny, h_len, g_kind, g_bc);
-    const size_t ni = tb.code.size();
-    {  // every (cell, local face) slot must receive exactly one flux
-      std::vector<uint8_t> hits(size_t(maxdeg) * N, 0);
-      bool ok = true;
-      for (int64_t t = 0; t < ctx->ntiles && ok; ++t)
-        for (int32_t it = tb.off[t]; it < tb.off[t + 1]; ++it) {
-          const int32_t code = tb.code[it];
-          const int32_t ia = code & 1023, ka = (code >> 10) & 15, ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
-          const int32_t type = (code >> 28) & 3;
-          const int64_t ca = t * T + ia;
-          if (ca >= N || ka >= h_deg[ca]) { ok = false; break; }
-          hits[size_t(ka) * N + ca]++;
-          if (type == 0) {
-            const int64_t cb = t * T + ib;
-            if (cb >= N || kb >= h_deg[cb]) { ok = false; break; }
-            hits[size_t(kb) * N + cb]++;
-          }
-        }
-      for (int64_t i = 0; i < N && ok; ++i)
-        for (int32_t k = 0; k < maxdeg; ++k)
-          if (hits[size_t(k) * N + i] != (k < h_deg[i] ? 1 : 0)) { ok = false; break; }
-      if (!ok) {
-        ctx->err = "internal: tiled work items do not cover every face slot exactly once";
-        return fail(HG_ERR_MESH);
-      }
-    }
-    std::vector<hg::ItemRec> recs(ni);
-    for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
-    if (!rc) rc = dalloc(ctx, &ctx->item_off, tb.off.size());
-    if (!rc) rc = dalloc(ctx, &ctx->items, ni);
-    if (!rc) rc = dalloc(ctx, &ctx->tile_inlets, tb.inlets.size());
-    if (!rc) rc = upload(ctx, ctx->item_off, tb.off.data(), tb.off.size());
-    if (!rc) rc = upload(ctx, ctx->items, recs.data(), ni);
-    if (!rc) rc = upload(ctx, ctx->tile_inlets, tb.inlets.data(), tb.inlets.size());
+    const int T = tile_lds_bytes(256, ctx->tileMD) <= ctx->lds_max ? 256 : 128;
+    rc = setup_tiles(ctx, T);
     if (rc) return fail(rc);
   }
   *out = ctx;
   return HG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags) {
+  return create_impl(out, m, b, c, flags, {});
+}
+
+int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const hg_bcs* bcs, const hg_consts* c,
+                    int32_t flags) {
+  if (!out || !meshes || !bcs || !c || n_inst <= 0) {
+    g_create_err = "hg_batch_create: NULL argument or no instances";
+    return HG_ERR_ARG;
+  }
+  *out = nullptr;
+  auto bad = [](int32_t inst, const std::string& what) {
+    g_create_err = "hg_batch_create: instance " + std::to_string(inst) + ": " + what;
+    return HG_ERR_MESH;
+  };
+  // ---- sizes, and every index checked against its own instance before it is offset
+  int64_t N = 0, M = 0, G = 0, nIF = 0;
+  int64_t nI = 0, nE = 0;
+  bool cen = true;
+  for (int32_t r = 0; r < n_inst; ++r) {
+    const hg_mesh& m = meshes[r];
+    const hg_bcs& b = bcs[r];
+    if (m.n_cells <= 0 || b.n_ghosts < 0 || b.n_inlet < 0 || b.n_exit < 0) return bad(r, "bad counts");
+    if (!m.face_off || !m.face_nbr || !m.face_nx || !m.face_ny || !m.face_len || !m.area ||
+        (b.n_ghosts > 0 && (!b.ghost_slot || !b.ghost_kind || !b.ghost_bc)) ||
+        (b.n_inlet > 0 && (!b.inlet_off || !b.inlet_ghost)))
+      return bad(r, "NULL array");
+    if (m.face_off[0] != 0) return bad(r, "face_off[0] != 0");
+    for (int64_t i = 0; i < m.n_cells; ++i)
+      if (m.face_off[i + 1] < m.face_off[i]) return bad(r, "face_off not monotone");
+    const int64_t Mi = m.face_off[m.n_cells];
+    for (int64_t s = 0; s < Mi; ++s) {
+      const int32_t j = m.face_nbr[s];
+      if (j >= 0 ? j >= m.n_cells : ~int64_t(j) >= b.n_ghosts) return bad(r, "face_nbr out of range");
+    }
+    for (int64_t g = 0; g < b.n_ghosts; ++g) {
+      if (b.ghost_slot[g] < 0 || b.ghost_slot[g] >= Mi) return bad(r, "ghost_slot out of range");
+      const int32_t k = b.ghost_kind[g], bi = b.ghost_bc[g];
+      if ((k == HG_BC_INLETQ && (bi < 0 || bi >= b.n_inlet)) || (k == HG_BC_EXITH && (bi < 0 || bi >= b.n_exit)))
+        return bad(r, "ghost_bc out of range");
+    }
+    if (b.n_inlet > 0) {
+      if (b.inlet_off[0] != 0) return bad(r, "inlet_off[0] != 0");
+      for (int32_t k = 0; k < b.n_inlet; ++k)
+        if (b.inlet_off[k + 1] < b.inlet_off[k]) return bad(r, "inlet_off not monotone");
+      for (int32_t f = 0; f < b.inlet_off[b.n_inlet]; ++f)
+        if (b.inlet_ghost[f] < 0 || b.inlet_ghost[f] >= b.n_ghosts) return bad(r, "inlet_ghost out of range");
+      nIF += b.inlet_off[b.n_inlet];
+    }
+    cen = cen && m.centroid_xy;
+    N += m.n_cells;
+    M += Mi;
+    G += b.n_ghosts;
+    nI += b.n_inlet;
+    nE += b.n_exit;
+  }
+  if (N > (int64_t(1) << 30) || M >= (int64_t(1) << 31) || nI >= (int64_t(1) << 31) || nE >= (int64_t(1) << 31)) {
+    g_create_err = "hg_batch_create: batch too large for 32-bit indices";
+    return HG_ERR_ARG;
+  }
+  // ---- concatenation: instance r's cells, slots, ghosts, inlets and exits follow those of r-1
+  std::vector<int32_t> off(N + 1, 0), nbr(M), gslot(G), gkind(G), gbc(G), ioff(nI + 1, 0), ighost(nIF);
+  std::vector<double> nx(M), ny(M), len(M), area(N), xy(cen ? 2 * N : 0);
+  std::vector<int64_t> seg(1, 0);
+  int64_t cN = 0, cM = 0, cG = 0, cI = 0, cE = 0, cF = 0;
+  for (int32_t r = 0; r < n_inst; ++r) {
+    const hg_mesh& m = meshes[r];
+    const hg_bcs& b = bcs[r];
+    const int64_t Mi = m.face_off[m.n_cells];
+    for (int64_t i = 0; i < m.n_cells; ++i) {
+      off[cN + i + 1] = int32_t(cM + m.face_off[i + 1]);
+      area[cN + i] = m.area[i];
+      if (cen) {
+        xy[2 * (cN + i)] = m.centroid_xy[2 * i];
+        xy[2 * (cN + i) + 1] = m.centroid_xy[2 * i + 1];
+      }
+    }
+    for (int64_t s = 0; s < Mi; ++s) {
+      const int32_t j = m.face_nbr[s];
+      nbr[cM + s] = j >= 0 ? int32_t(j + cN) : ~int32_t(~j + cG);
+      nx[cM + s] = m.face_nx[s];
+      ny[cM + s] = m.face_ny[s];
+      len[cM + s] = m.face_len[s];
+    }
+    for (int64_t g = 0; g < b.n_ghosts; ++g) {
+      gslot[cG + g] = int32_t(b.ghost_slot[g] + cM);
+      gkind[cG + g] = b.ghost_kind[g];
+      const int32_t bi = b.ghost_bc[g];
+      gbc[cG + g] = b.ghost_kind[g] == HG_BC_INLETQ ? int32_t(bi + cI)
+                    : b.ghost_kind[g] == HG_BC_EXITH ? int32_t(bi + cE) : bi;
+    }
+    for (int32_t k = 0; k < b.n_inlet; ++k) ioff[cI + k + 1] = int32_t(cF + b.inlet_off[k + 1]);
+    const int64_t nf = b.n_inlet > 0 ? b.inlet_off[b.n_inlet] : 0;
+    for (int64_t f = 0; f < nf; ++f) ighost[cF + f] = int32_t(b.inlet_ghost[f] + cG);
+    cN += m.n_cells;
+    cM += Mi;
+    cG += b.n_ghosts;
+    cI += b.n_inlet;
+    cE += b.n_exit;
+    cF += nf;
+    seg.push_back(cN);
+  }
+  hg_mesh mm{N, off.data(), nbr.data(), nx.data(), ny.data(), len.data(), area.data(), cen ? xy.data() : nullptr};
+  hg_bcs bb{G, gslot.data(), gkind.data(), gbc.data(), int32_t(nI), int32_t(nE), ioff.data(), ighost.data()};
+  return create_impl(out, &mm, &bb, c, flags, seg);
 }
 
 int hg_destroy(hg_ctx* ctx) {
@@ -1757,6 +1958,93 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   (void)hipEventDestroy(e1);
   if (!rc) rc = check_flags(ctx);
   return rc;
+}
+
+int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_map) {
+  if (!ctx) return HG_ERR_ARG;
+  if (!ctx->tiled) {
+    ctx->err = "hg_set_tile_config: this context runs the per-cell kernel";
+    return HG_ERR_STATE;
+  }
+  ctx->xcd_map = xcd_map ? 1 : 0;
+  if (tile_cells != ctx->tileT) return setup_tiles(ctx, tile_cells);
+  return drop_graph(ctx);  // the captured launches hold the old mapping
+}
+
+int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_map) {
+  if (!ctx) return HG_ERR_ARG;
+  if (tile_cells) *tile_cells = ctx->tiled ? ctx->tileT : 0;
+  if (xcd_map) *xcd_map = ctx->xcd_map;
+  return HG_OK;
+}
+
+int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_map, double* ms_per_step) {
+  if (!ctx || !(dt > 0.0) || nsteps <= 0) {
+    if (ctx) ctx->err = "hg_autotune: need dt > 0 and nsteps > 0";
+    return HG_ERR_ARG;
+  }
+  if (!ctx->fields_set) {
+    ctx->err = "hg_autotune: fields not set";
+    return HG_ERR_STATE;
+  }
+  const int64_t N = ctx->N;
+  double best = INFINITY;
+  int32_t bestT = ctx->tileT, bestX = ctx->xcd_map;
+  if (ctx->tiled) {
+    // keep the device state: every candidate starts from it, and it is restored at the end
+    for (int k = 0; k < 3; ++k)
+      HG_TRY(hipMemcpyAsync(ctx->scratch + k * N, ctx->st[ctx->cur][k], N * sizeof(double), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    hipEvent_t e0, e1;
+    HG_TRY(hipEventCreate(&e0));
+    HG_TRY(hipEventCreate(&e1));
+    int rc = HG_OK;
+    for (int T : {128, 256, 512}) {
+      if (tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) continue;
+      for (int x : {1, 0}) {
+        rc = hg_set_tile_config(ctx, T, x);
+        for (int k = 0; k < 3 && !rc; ++k)
+          if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                             ctx->stream) != hipSuccess)
+            rc = HG_ERR_HIP;
+        ctx->cur = 0;
+        if (!rc) rc = clear_flags(ctx);
+        if (!rc) rc = advance_async(ctx, dt, kGraphSteps);  // warm-up, graph instantiation
+        if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+        if (!rc) rc = advance_async(ctx, dt, nsteps);
+        if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = HG_ERR_HIP;
+        if (!rc) rc = check_flags(ctx);
+        if (rc) break;
+        if (double(ms) < best) {
+          best = double(ms);
+          bestT = T;
+          bestX = x;
+        }
+      }
+      if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc) rc = hg_set_tile_config(ctx, bestT, bestX);
+    for (int k = 0; k < 3 && !rc; ++k)
+      if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream) != hipSuccess)
+        rc = HG_ERR_HIP;
+    ctx->cur = 0;
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+    if (rc) {
+      if (ctx->err.empty()) ctx->err = "hg_autotune: HIP error";
+      return rc;
+    }
+    best /= double(nsteps);
+  }
+  if (tile_cells) *tile_cells = ctx->tiled ? bestT : 0;
+  if (xcd_map) *xcd_map = bestX;
+  if (ms_per_step) *ms_per_step = ctx->tiled ? best : 0.0;
+  return HG_OK;
 }
 
 // ------------------------------------------------------------ adjoint ABI --

@@ -24,7 +24,8 @@ HG_CELL_KERNEL = 2
 EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
-    "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes",
+    "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
+    "hg_set_tile_config", "hg_get_tile_config", "hg_autotune",
 ]
 
 
@@ -61,6 +62,12 @@ def lib():
         L.hg_info.restype = C.c_char_p
         L.hg_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(hg_mesh), C.POINTER(hg_bcs), C.POINTER(hg_consts),
                                 C.c_int32]
+        L.hg_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(hg_mesh), C.POINTER(hg_bcs),
+                                      C.POINTER(hg_consts), C.c_int32]
+        L.hg_set_tile_config.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.hg_get_tile_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.hg_autotune.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_double)]
         for name in ("hg_destroy",):
             getattr(L, name).argtypes = [C.c_void_p]
         L.hg_set_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 9
@@ -90,13 +97,16 @@ def _ptr(a):
 
 
 class Context:
-    """Owns one hg_ctx (one HIP stream) built from a ``cases.Case``."""
+    """Owns one hg_ctx (one HIP stream) built from a ``cases.Case``.
+
+    A batch (``cases.concat``) of instances is created with
+    hg_batch_create from the instances' own meshes; every other call then
+    takes the batch's concatenated arrays."""
 
     def __init__(self, case, reorder=True, cell_kernel=False):
         L = lib()
-        m = case.mesh
         self.case = case
-        self.N = m.n_cells
+        self.N = case.mesh.n_cells
         self._keep = []
 
         def k(a, dt):
@@ -106,15 +116,26 @@ class Context:
             self._keep.append(a)
             return _ptr(a)
 
-        mesh = hg_mesh(m.n_cells, k(m.face_off, np.int32), k(m.face_nbr, np.int32), k(m.face_nx, np.float64),
-                       k(m.face_ny, np.float64), k(m.face_len, np.float64), k(m.area, np.float64),
-                       k(m.centroid.ravel(), np.float64))
-        bcs = hg_bcs(m.n_ghosts, k(m.ghost_slot, np.int32), k(m.ghost_kind, np.int32), k(m.ghost_bc, np.int32),
-                     m.n_inlet, m.n_exit, k(m.inlet_off, np.int32), k(m.inlet_ghost, np.int32))
+        def structs(m):
+            mesh = hg_mesh(m.n_cells, k(m.face_off, np.int32), k(m.face_nbr, np.int32), k(m.face_nx, np.float64),
+                           k(m.face_ny, np.float64), k(m.face_len, np.float64), k(m.area, np.float64),
+                           k(m.centroid.ravel(), np.float64))
+            bcs = hg_bcs(m.n_ghosts, k(m.ghost_slot, np.int32), k(m.ghost_kind, np.int32), k(m.ghost_bc, np.int32),
+                         m.n_inlet, m.n_exit, k(m.inlet_off, np.int32), k(m.inlet_ghost, np.int32))
+            return mesh, bcs
+
         consts = hg_consts(case.g, case.k_n, case.h_small)
         h = C.c_void_p()
         flags = (HG_REORDER if reorder else 0) | (HG_CELL_KERNEL if cell_kernel else 0)
-        rc = L.hg_create(C.byref(h), C.byref(mesh), C.byref(bcs), C.byref(consts), flags)
+        inst = case.extra.get("instances") if case.extra else None
+        if inst:
+            pairs = [structs(c.mesh) for c in inst]
+            meshes = (hg_mesh * len(inst))(*[p[0] for p in pairs])
+            bcs = (hg_bcs * len(inst))(*[p[1] for p in pairs])
+            rc = L.hg_batch_create(C.byref(h), len(inst), meshes, bcs, C.byref(consts), flags)
+        else:
+            mesh, bcs = structs(case.mesh)
+            rc = L.hg_create(C.byref(h), C.byref(mesh), C.byref(bcs), C.byref(consts), flags)
         if rc:
             raise HGError(rc, L.hg_last_error(None).decode())
         self.h = h
@@ -233,6 +254,21 @@ class Context:
         self._chk(lib().hg_time_advance(self.h, C.c_double(dt), C.c_int64(nsteps), C.c_int32(reps), C.byref(a),
                                         C.byref(b)))
         return a.value, b.value
+
+    def set_tile_config(self, tile_cells, xcd_map=True):
+        self._chk(lib().hg_set_tile_config(self.h, int(tile_cells), int(bool(xcd_map))))
+
+    def tile_config(self):
+        t, x = C.c_int32(), C.c_int32()
+        self._chk(lib().hg_get_tile_config(self.h, C.byref(t), C.byref(x)))
+        return t.value, bool(x.value)
+
+    def autotune(self, dt, nsteps=64):
+        """Times every launch shape from the device state (restored
+        afterwards) and keeps the fastest: (tile_cells, xcd_map, ms/step)."""
+        t, x, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(lib().hg_autotune(self.h, C.c_double(dt), C.c_int64(nsteps), C.byref(t), C.byref(x), C.byref(ms)))
+        return t.value, bool(x.value), ms.value
 
 
 def probe_copy(ctx, nbytes, reps=1):

@@ -255,3 +255,29 @@ def load_srh_case(case_dir, srhhydro_name, ic=None, dt=0.1, nsteps=10):
         wse, wstill, qx, qy = (np.full(N, float(v)) for v in c)
     return _finish(os.path.splitext(srhhydro_name)[0], mesh, zb, n, wse, wstill, qx, qy, inletQ, exitWSE, dt,
                    nsteps, zone=zone, zone_n=zone_n, z_nodes=xyz[:, 2])
+
+
+def concat(instances, name=None, dt=None):
+    """The batch of independent cases as one Case over their block-diagonal
+    union (instance r's cells, ghosts, inlets, exits follow instance r-1's);
+    one dt for all, by default the smallest instance dt."""
+    cat = np.concatenate
+    m = hm.concat_meshes([c.mesh for c in instances])
+    N = [c.N for c in instances]
+    Q0 = cat([cat([c.Q0[k * n:(k + 1) * n] for c, n in zip(instances, N)]) for k in range(3)])
+    return Case(name=name or f"batch{len(instances)}", mesh=m,
+                hstill=cat([c.hstill for c in instances]), hstill_ghost=cat([c.hstill_ghost for c in instances]),
+                zb=cat([c.zb for c in instances]), zb_ghost=cat([c.zb_ghost for c in instances]),
+                S0x=cat([c.S0x for c in instances]), S0y=cat([c.S0y for c in instances]),
+                n=cat([c.n for c in instances]), inletQ=cat([c.inletQ for c in instances]),
+                exitWSE=cat([c.exitWSE for c in instances]), Q0=Q0,
+                dt=float(min(c.dt for c in instances) if dt is None else dt),
+                nsteps=max(c.nsteps for c in instances), g=instances[0].g, k_n=instances[0].k_n,
+                h_small=instances[0].h_small, extra={"instances": instances, "cell_off": np.cumsum([0] + N)})
+
+
+def split_state(batch, Q):
+    """Per-instance states [xi; q_x; q_y] from a batch state."""
+    off = batch.extra["cell_off"]
+    Nt = int(off[-1])
+    return [np.concatenate([Q[k * Nt + off[r]: k * Nt + off[r + 1]] for k in range(3)]) for r in range(len(off) - 1)]

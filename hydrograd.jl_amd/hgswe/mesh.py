@@ -316,3 +316,61 @@ def morton_order(xy: np.ndarray, bits: int = 21) -> np.ndarray:
 
     code = spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1))
     return np.argsort(code, kind="stable")
+
+
+def concat_meshes(meshes) -> Mesh2D:
+    """Block-diagonal union of independent meshes laid end to end: mesh r's
+    nodes, cells, edges, slots, ghosts, inlets and exits follow those of
+    mesh r-1 (the instance layout of hg_batch_create)."""
+    cat = np.concatenate
+    nn = np.cumsum([0] + [m.node_xy.shape[0] for m in meshes])
+    nc = np.cumsum([0] + [m.n_cells for m in meshes])
+    ne = np.cumsum([0] + [m.n_faces for m in meshes])
+    ns = np.cumsum([0] + [int(m.face_off[-1]) for m in meshes])
+    ng = np.cumsum([0] + [m.n_ghosts for m in meshes])
+    ni = np.cumsum([0] + [m.n_inlet for m in meshes])
+    nx_ = np.cumsum([0] + [m.n_exit for m in meshes])
+    nf = np.cumsum([0] + [int(m.inlet_off[-1]) if m.n_inlet else 0 for m in meshes])
+    maxdeg = max(m.cell_nodes.shape[1] for m in meshes)
+
+    def pad(cn, off):
+        out = np.full((cn.shape[0], maxdeg), -1, dtype=np.int64)
+        out[:, : cn.shape[1]] = np.where(cn >= 0, cn + off, -1)
+        return out
+
+    def nbr(m, r):
+        j = m.face_nbr.astype(np.int64)
+        return np.where(j >= 0, j + nc[r], ~((~j) + ng[r])).astype(np.int32)
+
+    def gbc(m, r):
+        b = m.ghost_bc.astype(np.int64)
+        b = np.where(m.ghost_kind == BC_INLETQ, b + ni[r], np.where(m.ghost_kind == BC_EXITH, b + nx_[r], b))
+        return b.astype(np.int32)
+
+    R = range(len(meshes))
+    inlet_off = np.zeros(int(ni[-1]) + 1, dtype=np.int32)
+    for r, m in zip(R, meshes):
+        if m.n_inlet:
+            inlet_off[ni[r] + 1: ni[r + 1] + 1] = np.asarray(m.inlet_off[1:], dtype=np.int64) + nf[r]
+    return Mesh2D(
+        node_xy=cat([m.node_xy for m in meshes]),
+        cell_nodes=cat([pad(m.cell_nodes, nn[r]) for r, m in zip(R, meshes)]),
+        cell_deg=cat([m.cell_deg for m in meshes]),
+        edge_nodes=cat([m.edge_nodes + nn[r] for r, m in zip(R, meshes)]),
+        face_off=cat([[0]] + [np.asarray(m.face_off[1:], dtype=np.int64) + ns[r] for r, m in zip(R, meshes)]).astype(np.int32),
+        face_nbr=cat([nbr(m, r) for r, m in zip(R, meshes)]),
+        face_edge=cat([np.asarray(m.face_edge, dtype=np.int64) + ne[r] for r, m in zip(R, meshes)]).astype(np.int32),
+        face_nx=cat([m.face_nx for m in meshes]), face_ny=cat([m.face_ny for m in meshes]),
+        face_len=cat([m.face_len for m in meshes]), area=cat([m.area for m in meshes]),
+        centroid=cat([m.centroid for m in meshes]),
+        ghost_edge=cat([np.asarray(m.ghost_edge, dtype=np.int64) + ne[r] for r, m in zip(R, meshes)]),
+        ghost_cell=cat([m.ghost_cell + nc[r] for r, m in zip(R, meshes)]).astype(np.int32),
+        ghost_slot=cat([m.ghost_slot + ns[r] for r, m in zip(R, meshes)]).astype(np.int32),
+        ghost_kind=cat([m.ghost_kind for m in meshes]).astype(np.int32),
+        ghost_bc=cat([gbc(m, r) for r, m in zip(R, meshes)]),
+        ghost_nx=cat([m.ghost_nx for m in meshes]), ghost_ny=cat([m.ghost_ny for m in meshes]),
+        ghost_len=cat([m.ghost_len for m in meshes]),
+        n_inlet=int(ni[-1]), n_exit=int(nx_[-1]), inlet_off=inlet_off,
+        inlet_ghost=cat([np.asarray(m.inlet_ghost, dtype=np.int64) + ng[r] for r, m in zip(R, meshes)
+                         if m.n_inlet] or [np.zeros(0, np.int64)]).astype(np.int32),
+    )

@@ -12,8 +12,14 @@
 #   custom_ODE_solve(ode_f, Q0, params_vector, p)             custom_ODE_solvers.jl:36
 # and read the reference's own structs (mesh_2D, BoundaryConditions2D,
 # SWE2D_Extra_Parameters) unchanged.  ForwardDiff.Dual values cannot cross
-# the C ABI: they are rejected with an error.
+# the C ABI: they are rejected with an error.  Reverse mode (Zygote and the
+# SciMLSensitivity VJP choices that call it) is served by ChainRulesCore
+# rules backed by the library's hand-written adjoint (hg_rhs_vjp, hg_adjoint),
+# with the cotangent of params_vector formed for the active parameter exactly
+# as Zygote would through semi_discretize_swe_2D.jl:114-199.
 module HydrogradHIP
+
+using ChainRulesCore
 
 export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f
 
@@ -197,6 +203,84 @@ function custom_ODE_solve(ode_f::DeviceRHS, Q0, params_vector, swe2d_extra_param
         check(ctx.ptr, ccall((:hg_state_download, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, view(sol, :, k)))
     end
     return sol
+end
+
+# ---------------------------------------------------------------- reverse mode
+
+# Cotangent of params_vector from the per-cell n, zb and per-inlet Q
+# cotangents (semi_discretize_swe_2D.jl:114-199: zb -> bed data, zone n ->
+# cell n gather, Q -> inlet totals).
+function params_bar(ctx, p_extra, prm, nb, zbb, qb)
+    k = ctx.active
+    k == HG_PARAM_ZB && return zbb
+    k == HG_PARAM_N_CELL && return nb
+    k == HG_PARAM_Q && return qb
+    if k == HG_PARAM_N_ZONE
+        zone = p_extra.srh_all_Dict["matID_cells"]
+        out = zeros(length(prm))
+        for i in eachindex(nb)
+            out[zone[i]+1] += nb[i]
+        end
+        return out
+    end
+    return zero(prm)
+end
+
+dense(x, n) = x isa AbstractZero ? zeros(n) : Vector{Float64}(unthunk(x))
+
+# Pullback of one RHS evaluation: (Q, params) -> dQdt.
+function rhs_pullback(ctx, p_extra, q, prm, ybar)
+    N = ctx.N
+    nI = length(p_extra.inletQ_TotalQ)
+    Qb = zeros(3N); nb = zeros(N); zbb = zeros(N); qb = zeros(max(nI, 1))
+    check(ctx.ptr, ccall((:hg_rhs_vjp, libhgswe), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                          Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, q, prm, length(prm), dense(ybar, 3N), Qb, nb, zbb, qb))
+    return Qb, params_bar(ctx, p_extra, prm, nb, zbb, qb[1:nI])
+end
+
+function ChainRulesCore.rrule(f::DeviceRHS, u, p, t)
+    y = f(u, p, t)
+    ctx = context(f.p)
+    q = Vector{Float64}(u); prm = Vector{Float64}(p)
+    function DeviceRHS_pullback(ybar)
+        Qb, pb = rhs_pullback(ctx, f.p, q, prm, ybar)
+        return NoTangent(), Qb, pb, NoTangent()
+    end
+    return y, DeviceRHS_pullback
+end
+
+function ChainRulesCore.rrule(::typeof(swe_2d_rhs), dQdt, Q, params_vector, t, p_extra)
+    y = swe_2d_rhs(dQdt, Q, params_vector, t, p_extra)
+    ctx = context(p_extra)
+    q = Vector{Float64}(Q); prm = Vector{Float64}(params_vector)
+    function swe_2d_rhs_pullback(ybar)
+        Qb, pb = rhs_pullback(ctx, p_extra, q, prm, ybar)
+        return NoTangent(), NoTangent(), Qb, pb, NoTangent(), NoTangent()
+    end
+    return y, swe_2d_rhs_pullback
+end
+
+# Pullback of one Euler step with its dry clamp (custom_ODE_solvers.jl:5-33):
+# the clamp's mask is recorded in the forward pass of hg_adjoint.
+function ChainRulesCore.rrule(::typeof(custom_ODE_update_cells), ode_f::DeviceRHS, Q, params_vector, t, dt,
+                              swe2d_extra_params)
+    y = custom_ODE_update_cells(ode_f, Q, params_vector, t, dt, swe2d_extra_params)
+    ctx = context(swe2d_extra_params)
+    q = Vector{Float64}(Q); prm = Vector{Float64}(params_vector)
+    N = ctx.N
+    nI = length(swe2d_extra_params.inletQ_TotalQ)
+    function update_cells_pullback(ybar)
+        Qb = zeros(3N); nb = zeros(N); zbb = zeros(N); qb = zeros(max(nI, 1))
+        check(ctx.ptr, ccall((:hg_adjoint, libhgswe), Cint,
+                             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64, Int64, Ptr{Float64},
+                              Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                             ctx.ptr, q, prm, length(prm), dt, 1, 1, dense(ybar, 3N), Qb, nb, zbb, qb))
+        pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb[1:nI])
+        return NoTangent(), NoTangent(), Qb, pb, NoTangent(), NoTangent(), NoTangent()
+    end
+    return y, update_cells_pullback
 end
 
 end # module

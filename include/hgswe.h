@@ -104,6 +104,16 @@ int hg_create(hg_ctx** out, const hg_mesh* mesh, const hg_bcs* bcs, const hg_con
 
 int hg_destroy(hg_ctx* ctx);
 
+/* Batched context over n_inst independent meshes (no reference
+ * counterpart: the reference runs one case per process).  The instances are
+ * laid end to end: instance r's cells, ghosts, inlet-Q and exit-H boundaries
+ * follow those of instance r-1, so every other entry point takes
+ * concatenated arrays (state [xi; q_x; q_y] over all cells, fields, params,
+ * probes) and treats the batch as one block-diagonal mesh advanced with one
+ * dt.  Cells are renumbered within each instance. */
+int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const hg_bcs* bcs, const hg_consts* consts,
+                    int32_t flags);
+
 /* Message of the last error on this context (or of the last failed
  * hg_create when ctx is NULL). */
 const char* hg_last_error(const hg_ctx* ctx);
@@ -153,6 +163,18 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
  * context's stream, plus the mean duration of the step kernel alone. */
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel);
+
+/* Launch shape of the tiled step kernel (no reference counterpart): cells
+ * per tile (128, 256 or 512; one work-item per cell) and whether work-groups
+ * are mapped to tiles so that each XCD takes a contiguous run of tiles.
+ * Every shape gives bit-identical results. */
+int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_map);
+int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_map);
+
+/* Autotuner: times nsteps steps of every launch shape from the current
+ * device state (which is restored afterwards), keeps the fastest and
+ * reports it.  Needs hg_set_fields and a device state (hg_state_upload). */
+int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_map, double* ms_per_step);
 
 /* ---------------------------------------------------------- adjoint path */
 

@@ -224,3 +224,39 @@ def test_full_size_channel_parity():
     ctx, orc = capi.Context(c), pyoracle.Oracle(c)
     check_rhs(c, c.Q0, ctx, orc)
     check_steps(c, c.Q0, c.dt, 10, ctx, orc)
+
+
+def decagon_fan():
+    """One 10-sided cell ringed by two layers of triangles, walls outside:
+    a cell degree the tiled kernel's LDS slots do not take, so the library
+    runs the per-cell kernel for it."""
+    from hgswe import mesh as hm
+
+    ang = lambda k, off=0.0: 2 * np.pi * (k + off) / 10  # noqa: E731
+    inner = [(5 * np.cos(ang(k)), 5 * np.sin(ang(k))) for k in range(10)]
+    outer = [(10 * np.cos(ang(k, 0.5)), 10 * np.sin(ang(k, 0.5))) for k in range(10)]
+    xy = np.array(inner + outer)
+    v = lambda k: k % 10  # noqa: E731
+    o = lambda k: 10 + k % 10  # noqa: E731
+    cells = [list(range(10))]
+    for k in range(10):
+        cells.append([v(k + 1), v(k), o(k)] + [-1] * 7)
+        cells.append([v(k + 1), o(k), o(k + 1)] + [-1] * 7)
+    m = hm.build_mesh(xy, np.array(cells), {}, {})
+    N = m.n_cells
+    r = np.hypot(m.centroid[:, 0], m.centroid[:, 1])
+    wse = np.where(r < 3.0, 1.0, 0.4)
+    c = cases._finish("decagon_fan", m, np.zeros(N), np.full(N, 0.03), wse, np.zeros(N), np.zeros(N), np.zeros(N),
+                      [], [], 0.0, 200)
+    return c.copy_fields(dt=cases.cfl_dt(c, 0.4))
+
+
+def test_large_polygon_uses_cell_kernel():
+    c = decagon_fan()
+    assert int(np.max(np.diff(c.mesh.face_off))) == 10
+    ctx, orc = capi.Context(c), pyoracle.Oracle(c)
+    check_rhs(c, c.Q0, ctx, orc)
+    check_steps(c, c.Q0, c.dt, c.nsteps, ctx, orc)
+    a = ctx.step(c.Q0, c.dt, 50)
+    b = capi.Context(c, cell_kernel=True).step(c.Q0, c.dt, 50)
+    assert np.array_equal(a, b)
