@@ -123,6 +123,55 @@ def measure_gradient(args, capi):
     return out
 
 
+def batch_instances(n_inst, seed=7, lo=5_000, hi=500_000):
+    """64 independent channel instances with N_i log-uniform in [5k, 500k]
+    triangles, each with its own bed (seeded bumps), zone n and inlet
+    discharge Q ~ U[50, 1000] m^3/s."""
+    from hgswe import cases
+
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_inst)).astype(int)
+    Qs = rng.uniform(50.0, 1000.0, n_inst)
+    return [cases.channel(int(n), seed=1000 + r, Q_in=float(q), nsteps=100) for r, (n, q) in enumerate(zip(sizes, Qs))]
+
+
+def measure_batch(args, capi, single_value):
+    from hgswe import cases
+
+    t0 = time.perf_counter()
+    inst = batch_instances(args.batch)
+    B = cases.concat(inst)
+    print(f"# batch: {len(inst)} instances, N={B.N}, inlets={B.mesh.n_inlet}, dt={B.dt:.6g} "
+          f"(built in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx = capi.Context(B)
+    ctx.upload(B.Q0)
+    T, x, _ = ctx.autotune(B.dt, 64)
+    ctx.advance(B.dt, args.warmup)
+    t0 = time.perf_counter()
+    ctx.advance(B.dt, args.batch_steps)
+    el = time.perf_counter() - t0
+    out = {"value": B.N * args.batch_steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / args.batch_steps,
+           "single_instance_value": single_value,
+           "config": {"workload": f"batch{len(inst)}_channels_5k_500k_forward_euler", "instances": len(inst),
+                      "cells": B.N, "steps": args.batch_steps, "dt": B.dt, "tile_cells": T, "xcd_chunk": x,
+                      "working_set_MB": round(bytes_per_cell_step(B.mesh) * B.N / 1e6, 1)}}
+    # gradient of the summed per-instance probe loss w.r.t. n, zb and every inlet Q
+    rng = np.random.default_rng(11)
+    off = B.extra["cell_off"]
+    probe = np.concatenate([off[r] + rng.choice(c.N, size=200, replace=False) for r, c in enumerate(inst)])
+    probe = probe.astype(np.int32)
+    QN = ctx.step(B.Q0, B.dt, args.batch_grad_steps)
+    target = QN[probe] + B.hstill[probe] + B.zb[probe] + rng.normal(0.0, 0.02, probe.size)
+    ctx.grad_probes(B.Q0, B.dt, 10, 5, probe, target)  # warm-up
+    t0 = time.perf_counter()
+    ctx.grad_probes(B.Q0, B.dt, args.batch_grad_steps, args.grad_kck, probe, target)
+    el = time.perf_counter() - t0
+    out["gradient"] = {"value": B.N * args.batch_grad_steps / el, "unit": "cell-steps/s",
+                       "ms_per_gradient": el * 1e3, "steps": args.batch_grad_steps,
+                       "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "n, zb, inlet Q"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +184,10 @@ def main():
     ap.add_argument("--grad-steps", type=int, default=2000)
     ap.add_argument("--grad-kck", type=int, default=50)
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched workload (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=400)
+    ap.add_argument("--batch-grad-steps", type=int, default=200)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (256-cell tiles, identity tile mapping)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
                          "(calibrates rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
@@ -153,6 +206,9 @@ def main():
     ctx = capi.Context(case)
     print(f"# {capi.info()}", file=sys.stderr, flush=True)
     ctx.upload(case.Q0)
+    if not args.no_tune:  # launch shapes are bit-identical; keep the fastest
+        T, x, _ = ctx.autotune(case.dt, 64)
+        print(f"# autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     ctx.advance(case.dt, args.warmup)
     # timed region: K device-resident steps, synchronized on both sides
     t0 = time.perf_counter()
@@ -176,7 +232,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "channel_1M_tri_forward_euler", "cells": case.N, "faces": int(case.mesh.n_faces),
-                   "ghosts": case.G, "cfl": 0.5, "dt": case.dt, "parallelism": "single-gpu"},
+                   "ghosts": case.G, "cfl": 0.5, "dt": case.dt, "parallelism": "single-gpu",
+                   "tile_cells": ctx.tile_config()[0], "xcd_chunk": ctx.tile_config()[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "k_swe_tile", "kernel_ms": ms_kernel, "bytes_per_cell_step": bpc},
@@ -188,6 +245,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     if not args.no_grad:
         out["gradient"] = measure_gradient(args, capi)
+    if args.batch > 0:
+        del ctx
+        out["batch"] = measure_batch(args, capi, value)
     print(json.dumps(out), flush=True)
 
 

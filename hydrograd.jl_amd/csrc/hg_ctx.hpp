@@ -113,7 +113,7 @@ struct TileArgs {
   int32_t* flag;
   uint8_t* clamp;  // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
-  int32_t xcd_map;  // 1: XCD-contiguous block -> tile mapping, 0: identity
+  int32_t xcd_chunk;  // block -> tile mapping: 0 identity, G > 0 runs of G tiles per XCD in windows of 8G
 };
 
 // Arguments of the adjoint (VJP) kernel of one RHS evaluation / one step.
@@ -202,7 +202,7 @@ struct hg_ctx {
   int32_t* item_off = nullptr;
   hg::ItemRec* items = nullptr;
   int32_t tileMD = 3;
-  int32_t xcd_map = 1;
+  int32_t xcd_chunk = 0;  // identity block -> tile mapping by default (the autotuner re-checks)
   int32_t *tile_in_off = nullptr, *tile_in_list = nullptr;
   std::vector<void*> tile_allocs;  // device buffers of the current tiling (freed on re-tiling)
   size_t lds_max = 65536;          // dynamic LDS a work-group may use on this device
@@ -220,6 +220,16 @@ struct hg_ctx {
   uint8_t* adj_mask = nullptr;  // kck x N clamp flags
   double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
+
+  // UDE closure for n (0: off; else the precision of its network, 32 or 64)
+  int32_t ude_prec = 0;
+  int32_t ude_nsets = 0;
+  void* ude_w = nullptr;            // [ude_nsets][353] weights in that precision
+  int32_t* ude_set = nullptr;       // weight set of each cell (lib order), or NULL
+  double* ude_nbar = nullptr;       // per-step cotangent of n (N)
+  double* ude_wpart = nullptr;      // per-block weight cotangents
+  double* ude_wbar = nullptr;       // accumulated weight cotangent (353)
+  std::vector<void*> ude_allocs;
 
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;

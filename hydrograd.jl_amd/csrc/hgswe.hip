@@ -28,6 +28,7 @@
 #include "hg_device.hpp"
 #include "hg_roe_pair.hpp"
 #include "hg_adjoint.hpp"
+#include "hg_ude.hpp"
 #include "hgswe.h"
 
 using hg::Side;
@@ -226,14 +227,22 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
 template <bool STEP, int T, int MD>
 __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
   extern __shared__ double lds[];
-  // Work-groups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8);
-  // give each XCD a contiguous run of the Morton-ordered tiles so that
-  // cross-tile neighbours mostly sit in the same XCD's L2.  Bijection:
-  // nt = q * 8 + r, XCD x owns tiles [x q + min(x, r), ...) of length q + (x < r).
-  const int nt = gridDim.x;
-  const int q = nt / kXCD, r = nt % kXCD;
-  const int x = blockIdx.x % kXCD;
-  const int t = a.xcd_map ? x * q + min(x, r) + blockIdx.x / kXCD : int(blockIdx.x);
+  // Work-groups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8)
+  // in launch order.  With xcd_chunk = G > 0, each window of 8G consecutive
+  // Morton-ordered tiles is split so that XCD x takes G consecutive tiles of
+  // it (cross-tile neighbours then tend to share that XCD's L2) while the
+  // window of tiles in flight stays compact; a final partial window keeps
+  // the identity.  G = 0 is the identity mapping.  Bijection on [0, nt).
+  const int G = a.xcd_chunk;
+  int t = int(blockIdx.x);
+  if (G > 0) {
+    const int W = kXCD * G;
+    const int w0 = (t / W) * W;
+    if (w0 + W <= int(gridDim.x)) {
+      const int r = t - w0;
+      t = w0 + (r % kXCD) * G + r / kXCD;
+    }
+  }
   const int li = threadIdx.x;
   const int i = t * T + li;
   const bool valid = i < a.N;
@@ -721,6 +730,212 @@ std::vector<int32_t> morton_perm(const std::vector<int64_t>& seg, const double* 
   return perm;
 }
 
+// ---- UDE closure for Manning's n (matrix cores; one wave = 64 cells) ----
+
+// Hidden layers of 64 cells: features F[64][4] -> A1 = tanh(F W1^T + b1)
+// -> A2 = tanh(A1 W2^T + b2), each layer as 16-row blocks of
+// 16x16x4 matrix-core products; results land in LDS row-major [cell][unit].
+template <class T>
+__device__ __forceinline__ void ude_hidden(const T* __restrict__ W, const T* F, T* A1, T* A2, int l) {
+  using M = hg::Mfma<T>;
+  const int col = l & 15, kq = l >> 4;
+  for (int rb = 0; rb < 4; ++rb) {
+    const T av = F[(16 * rb + col) * 4 + kq];                         // F[16 rb + i][k]
+    const T bv = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : T(0);         // W1[j][k]
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    acc = M::op(av, bv, acc);
+    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + W[hg::kUdeB1 + col]);
+  }
+  hg::wave_sync();
+  for (int rb = 0; rb < 4; ++rb) {
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    for (int kc = 0; kc < 4; ++kc) {
+      const T av = A1[(16 * rb + col) * 16 + 4 * kc + kq];           // A1[16 rb + i][4 kc + k]
+      const T bv = W[hg::kUdeW2 + col * 16 + 4 * kc + kq];           // W2[j][4 kc + k]
+      acc = M::op(av, bv, acc);
+    }
+    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + W[hg::kUdeB2 + col]);
+  }
+  hg::wave_sync();
+}
+
+// Output layer of cell l: z3 = b3 + sum_j A2[l][j] W3[j].
+template <class T>
+__device__ __forceinline__ T ude_out(const T* __restrict__ W, const T* A2, int l) {
+  T z3 = W[hg::kUdeB3];
+  for (int j = 0; j < 16; ++j) z3 = z3 + A2[l * 16 + j] * W[hg::kUdeW3 + j];
+  return z3;
+}
+
+// n = 0.01 + softplus(MLP(features)) for every cell; a wave whose cells use
+// several weight sets makes one pass per set (the matrix-core products need
+// one set across the wave).
+template <class T>
+__global__ __launch_bounds__(64) void k_ude_n(hg::UdeArgs a) {
+  __shared__ T F[64 * 4];
+  __shared__ T A1[64 * 16];
+  __shared__ T A2[64 * 16];
+  const int l = threadIdx.x;
+  const int i = blockIdx.x * 64 + l;
+  const bool valid = i < a.N;
+  double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
+  bool wet = false;
+  int set = 0;
+  if (valid) {
+    hg::ude_features(a, i, h, u, v, um, wet);
+    zb = a.zb[i];
+    if (a.set_of_cell) set = a.set_of_cell[i];
+  }
+  F[l * 4 + 0] = T(h);
+  F[l * 4 + 1] = T(um);
+  F[l * 4 + 2] = T(zb);
+  F[l * 4 + 3] = T(0);
+  hg::wave_sync();
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const int lead = __ffsll((unsigned long long)pending) - 1;
+    const int s = __shfl(set, lead, 64);
+    const bool mine = valid && set == s;
+    pending &= ~uint64_t(__ballot(mine));
+    const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
+    ude_hidden<T>(W, F, A1, A2, l);
+    if (mine) a.n[i] = 0.01 + double(hg::softplus<T>(ude_out<T>(W, A2, l)));
+    hg::wave_sync();  // the next pass rewrites A1, A2
+  }
+}
+
+// Reverse mode of k_ude_n for one weight set: given the cotangent of n,
+// adds the cotangents of the features to the state cotangent (through the
+// clamp and u = q / h, as the forward forms them) and writes this block's
+// 353 weight cotangents to wpart[blockIdx.x].
+template <class T>
+__global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
+  using M = hg::Mfma<T>;
+  __shared__ T F[64 * 4];
+  __shared__ T A1[64 * 16];
+  __shared__ T A2[64 * 16];
+  __shared__ T Z2[64 * 16];
+  __shared__ T Z1[64 * 16];
+  __shared__ T Z3[64];
+  __shared__ double Wb[hg::kUdeW];
+  const int l = threadIdx.x;
+  const int col = l & 15, kq = l >> 4;
+  const int i = blockIdx.x * 64 + l;
+  const bool valid = i < a.N;
+  double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
+  bool wet = false;
+  if (valid) {
+    hg::ude_features(a, i, h, u, v, um, wet);
+    zb = a.zb[i];
+  }
+  F[l * 4 + 0] = T(h);
+  F[l * 4 + 1] = T(um);
+  F[l * 4 + 2] = T(zb);
+  F[l * 4 + 3] = T(0);
+  hg::wave_sync();
+  const T* W = static_cast<const T*>(a.w);
+  ude_hidden<T>(W, F, A1, A2, l);
+  // output layer and softplus, lane = cell
+  const T z3 = ude_out<T>(W, A2, l);
+  const T z3b = valid ? T(a.nbar[i]) * hg::sigmoid<T>(z3) : T(0);
+  Z3[l] = z3b;
+  for (int j = 0; j < 16; ++j) {
+    const T a2 = A2[l * 16 + j];
+    Z2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
+  }
+  hg::wave_sync();
+  // A1bar = Z2 W2 (rows = cells), Z1 = A1bar (1 - A1^2)
+  for (int rb = 0; rb < 4; ++rb) {
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    for (int kc = 0; kc < 4; ++kc) {
+      const T av = Z2[(16 * rb + col) * 16 + 4 * kc + kq];           // Z2[16 rb + i][4 kc + k]
+      const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];         // W2[4 kc + k][j]
+      acc = M::op(av, bv, acc);
+    }
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * rb + M::row(kq, r);
+      const T a1 = A1[row * 16 + col];
+      Z1[row * 16 + col] = acc[r] * (T(1) - a1 * a1);
+    }
+  }
+  hg::wave_sync();
+  // feature cotangents -> state cotangent (lane = cell)
+  T fb0 = T(0), fb1 = T(0), fb2 = T(0);
+  for (int j = 0; j < 16; ++j) {
+    const T z = Z1[l * 16 + j];
+    fb0 = fb0 + z * W[hg::kUdeW1 + j * 3 + 0];
+    fb1 = fb1 + z * W[hg::kUdeW1 + j * 3 + 1];
+    fb2 = fb2 + z * W[hg::kUdeW1 + j * 3 + 2];
+  }
+  if (valid) {
+    double xib = 0.0, qxb = 0.0, qyb = 0.0;
+    if (wet) {  // h = xi + h_still, |u| = sqrt(u^2 + v^2 + eps), u = q_x / h, v = q_y / h
+      const double ub = double(fb1) * u / um, vb = double(fb1) * v / um;
+      xib = double(fb0) - (ub * u + vb * v) / h;
+      qxb = ub / h;
+      qyb = vb / h;
+    }
+    a.ox[i] += xib;
+    a.oqx[i] += qxb;
+    a.oqy[i] += qyb;
+    if (a.zbbar) a.zbbar[i] += double(fb2);
+  }
+  // weight cotangents of these 64 cells: W2bar = Z2^T A1 on the matrix cores
+  {
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    for (int kc = 0; kc < 16; ++kc) {
+      const T av = Z2[(4 * kc + kq) * 16 + col];  // Z2[cell = 4 kc + k][unit i]
+      const T bv = A1[(4 * kc + kq) * 16 + col];  // A1[cell = 4 kc + k][unit j]
+      acc = M::op(av, bv, acc);
+    }
+    for (int r = 0; r < 4; ++r) Wb[hg::kUdeW2 + M::row(kq, r) * 16 + col] = double(acc[r]);
+  }
+  for (int e = l; e < 97; e += 64) {  // W1 (48), b1, b2, W3 (16 each), b3: sums over the cells in order
+    double sacc = 0.0;
+    int dst;
+    if (e < 48) {
+      const int j = e / 3, c = e % 3;
+      for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + j]) * double(F[q * 4 + c]);
+      dst = hg::kUdeW1 + e;
+    } else if (e < 64) {
+      for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + (e - 48)]);
+      dst = hg::kUdeB1 + (e - 48);
+    } else if (e < 80) {
+      for (int q = 0; q < 64; ++q) sacc += double(Z2[q * 16 + (e - 64)]);
+      dst = hg::kUdeB2 + (e - 64);
+    } else if (e < 96) {
+      for (int q = 0; q < 64; ++q) sacc += double(Z3[q]) * double(A2[q * 16 + (e - 80)]);
+      dst = hg::kUdeW3 + (e - 80);
+    } else {
+      for (int q = 0; q < 64; ++q) sacc += double(Z3[q]);
+      dst = hg::kUdeB3;
+    }
+    Wb[dst] = sacc;
+  }
+  __syncthreads();
+  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(blockIdx.x) * hg::kUdeW + k] = Wb[k];
+}
+
+// wbar[k] += sum over blocks of wpart[b][k], in block order (deterministic).
+__global__ __launch_bounds__(128) void k_ude_wsum(int nblocks, const double* wpart, double* wbar) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= hg::kUdeW) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += wpart[size_t(b) * hg::kUdeW + k];
+  wbar[k] += s;
+}
+
+// Layout check of the 16x16x4 matrix-core products used above: D = A B for
+// A (16x4) and B (4x16) given row-major, D returned row-major.
+template <class T>
+__global__ __launch_bounds__(64) void k_mfma_probe(const T* A, const T* B, T* D) {
+  using M = hg::Mfma<T>;
+  const int l = threadIdx.x, col = l & 15, kq = l >> 4;
+  typename M::V acc = {T(0), T(0), T(0), T(0)};
+  acc = M::op(A[col * 4 + kq], B[kq * 16 + col], acc);
+  for (int r = 0; r < 4; ++r) D[M::row(kq, r) * 16 + col] = acc[r];
+}
+
 // Work items of the tiled kernel (see TileArgs): per tile, cross-tile faces
 // first (the kernel prefetches their neighbour states before its first
 // barrier), then boundary faces, then intra-tile faces (owned by the side
@@ -829,7 +1044,7 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.items = ctx->items;
   a.tile_in_off = ctx->tile_in_off;
   a.tile_in_list = ctx->tile_in_list;
-  a.xcd_map = ctx->xcd_map;
+  a.xcd_chunk = ctx->xcd_chunk;
   a.g_kind = ctx->g_kind;
   a.g_bc = ctx->g_bc;
   a.g_hstill = ctx->g_hstill;
@@ -963,9 +1178,44 @@ void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
     launch_tile_t<512>(ctx, a, step);
 }
 
+hg::UdeArgs ude_args(hg_ctx* ctx, const double* const q[3]) {
+  hg::UdeArgs u{};
+  u.N = int32_t(ctx->N);
+  u.n_sets = ctx->ude_nsets;
+  u.xi = q[0];
+  u.qx = q[1];
+  u.qy = q[2];
+  u.hstill = ctx->hstill;
+  u.zb = ctx->zb;
+  u.set_of_cell = ctx->ude_set;
+  u.w = ctx->ude_w;
+  u.n = ctx->n;
+  u.hsmall = ctx->c.h_small;
+  return u;
+}
+
+int ude_blocks(const hg_ctx* ctx) { return int((ctx->N + 63) / 64); }
+
+// n <- 0.01 + softplus(MLP(features of state q)) when the closure is on.
+int enqueue_ude_n(hg_ctx* ctx, const double* const q[3]) {
+  if (!ctx->ude_prec) return HG_OK;
+  const hg::UdeArgs u = ude_args(ctx, q);
+  if (ctx->ude_prec == 32)
+    hipLaunchKernelGGL(k_ude_n<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+  else
+    hipLaunchKernelGGL(k_ude_n<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
 // Enqueue one step (or one RHS evaluation) reading state `src`.
 int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
                  double* o2 = nullptr) {
+  if (ctx->ude_prec) {
+    const double* q[3] = {ctx->st[src][0], ctx->st[src][1], ctx->st[src][2]};
+    int rc = enqueue_ude_n(ctx, q);
+    if (rc) return rc;
+  }
   if (ctx->tiled) {
     if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
       hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
@@ -988,6 +1238,10 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
 // One step (or RHS) from explicit state buffers; optional clamp record.
 int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt, bool step,
                    uint8_t* clamp = nullptr) {
+  if (ctx->ude_prec) {
+    int rc = enqueue_ude_n(ctx, in);
+    if (rc) return rc;
+  }
   if (ctx->tiled) {
     if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
       hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream,
@@ -1082,9 +1336,21 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
 // out and accumulates into adj_nbar / adj_zbbar / adj_Qbar.
 int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
                     double* const out[3], double dt, bool step) {
+  const bool ude = ctx->ude_prec != 0;
+  double* nbar = ude ? ctx->ude_nbar : ctx->adj_nbar;  // with the closure, n-bar of this step only
+  if (ude && ctx->ude_nsets != 1) {
+    ctx->err = "the UDE closure's reverse mode takes one weight set";
+    return HG_ERR_STATE;
+  }
+  if (ude) {
+    int rc = enqueue_ude_n(ctx, q);
+    if (rc) return rc;
+    HG_TRY(hipMemsetAsync(ctx->ude_nbar, 0, ctx->N * sizeof(double), ctx->stream));
+  }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
   hg::AdjArgs a = make_adj_args(ctx, q, lam, mask, out, dt);
+  a.nbar = nbar;
   if (step)
     hipLaunchKernelGGL(k_swe_adj<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   else
@@ -1092,14 +1358,30 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(64), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
                        ctx->in_ghost, ctx->in_cell, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
-                       ctx->totalA, ctx->adj_vbar, ctx->c.h_small, out[0], ctx->adj_nbar, ctx->adj_Qbar);
+                       ctx->totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar, ctx->adj_Qbar);
   hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
                      ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+  if (ude) {  // through the closure: n(Q) adds to the state cotangent, and the weight cotangent accumulates
+    hg::UdeArgs u = ude_args(ctx, q);
+    u.nbar = ctx->ude_nbar;
+    u.ox = out[0];
+    u.oqx = out[1];
+    u.oqy = out[2];
+    u.zbbar = ctx->adj_zbbar;
+    u.wpart = ctx->ude_wpart;
+    if (ctx->ude_prec == 32)
+      hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+    else
+      hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+    hipLaunchKernelGGL(k_ude_wsum, dim3((hg::kUdeW + 127) / 128), dim3(128), 0, ctx->stream, ude_blocks(ctx),
+                       ctx->ude_wpart, ctx->ude_wbar);
+  }
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
 
 int zero_adj_accumulators(hg_ctx* ctx) {
+  if (ctx->ude_prec) HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_Qbar, 0, std::max<int32_t>(ctx->n_inlet, 1) * sizeof(double), ctx->stream));
@@ -1787,6 +2069,10 @@ int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, 
     ctx->err = "unknown active parameter kind";
     return HG_ERR_ARG;
   }
+  if (ctx->ude_prec && (kind == HG_PARAM_N_ZONE || kind == HG_PARAM_N_CELL)) {
+    ctx->err = "Manning n comes from the UDE closure; it cannot also be the active parameter";
+    return HG_ERR_STATE;
+  }
   if (kind == HG_PARAM_N_ZONE) {
     if (!zone_of_cell || n_zones <= 0 || size_t(n_zones) > ctx->scratch_n) {
       ctx->err = "HG_PARAM_N_ZONE needs zone_of_cell and n_zones > 0";
@@ -1888,6 +2174,150 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
   return HG_OK;
 }
 
+int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t* set_of_cell,
+               int32_t precision) {
+  if (!ctx) return HG_ERR_ARG;
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the closure
+  for (void* p : ctx->ude_allocs) {
+    ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), p));
+    HG_TRY(hipFree(p));
+  }
+  ctx->ude_allocs.clear();
+  ctx->ude_prec = 0;
+  ctx->ude_nsets = 0;
+  ctx->ude_w = nullptr;
+  ctx->ude_set = nullptr;
+  if (n_sets == 0) {  // closure off: back to the per-cell n of hg_set_fields
+    HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    return HG_OK;
+  }
+  if (n_sets < 0 || !weights || (precision != 32 && precision != 64)) {
+    ctx->err = "hg_set_ude: need n_sets >= 0, weights and precision 32 or 64";
+    return HG_ERR_ARG;
+  }
+  if (ctx->active == HG_PARAM_N_ZONE || ctx->active == HG_PARAM_N_CELL) {
+    ctx->err = "hg_set_ude: Manning n is the active parameter";
+    return HG_ERR_STATE;
+  }
+  const int64_t N = ctx->N;
+  std::vector<int32_t> set;
+  if (set_of_cell) {
+    set.resize(N);
+    for (int64_t i = 0; i < N; ++i) {
+      const int32_t v = set_of_cell[ctx->perm[i]];
+      if (v < 0 || v >= n_sets) {
+        ctx->err = "hg_set_ude: set_of_cell out of range";
+        return HG_ERR_ARG;
+      }
+      set[i] = v;
+    }
+  }
+  for (int64_t k = 0; k < int64_t(n_sets) * hg::kUdeW; ++k)
+    if (!std::isfinite(weights[k])) {
+      ctx->err = "hg_set_ude: non-finite weight";
+      return HG_ERR_ARG;
+    }
+  const size_t nw = size_t(n_sets) * hg::kUdeW;
+  const size_t nalloc0 = ctx->allocs.size();
+  int rc = HG_OK;
+  if (precision == 32) {
+    std::vector<float> w(nw);
+    for (size_t k = 0; k < nw; ++k) w[k] = float(weights[k]);
+    float* d = nullptr;
+    rc = dalloc(ctx, &d, nw);
+    if (!rc) rc = upload(ctx, d, w.data(), nw);
+    ctx->ude_w = d;
+  } else {
+    double* d = nullptr;
+    rc = dalloc(ctx, &d, nw);
+    if (!rc) rc = upload(ctx, d, weights, nw);
+    ctx->ude_w = d;
+  }
+  if (!rc && set_of_cell) {
+    rc = dalloc(ctx, &ctx->ude_set, N);
+    if (!rc) rc = upload(ctx, ctx->ude_set, set.data(), N);
+  }
+  if (!rc) rc = dalloc(ctx, &ctx->ude_nbar, N);
+  if (!rc) rc = dalloc(ctx, &ctx->ude_wpart, size_t(ude_blocks(ctx)) * hg::kUdeW);
+  if (!rc) rc = dalloc(ctx, &ctx->ude_wbar, hg::kUdeW);
+  ctx->ude_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
+  if (rc) return rc;
+  HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->ude_nsets = n_sets;
+  ctx->ude_prec = precision;
+  return HG_OK;
+}
+
+int hg_ude_n(hg_ctx* ctx, const double* Q, double* n) {
+  if (!ctx || !Q || !n) return ctx ? (ctx->err = "hg_ude_n: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (!ctx->ude_prec) {
+    ctx->err = "hg_ude_n: the UDE closure is off";
+    return HG_ERR_STATE;
+  }
+  int rc = state_in(ctx, Q, 0);
+  if (rc) return rc;
+  ctx->cur = 0;
+  const double* q[3] = {ctx->st[0][0], ctx->st[0][1], ctx->st[0][2]};
+  rc = enqueue_ude_n(ctx, q);
+  if (!rc) rc = cell_out(ctx, ctx->n, n);
+  return rc;
+}
+
+int hg_ude_grad(hg_ctx* ctx, double* w_bar) {
+  if (!ctx || !w_bar) return HG_ERR_ARG;
+  if (!ctx->ude_prec) {
+    ctx->err = "hg_ude_grad: the UDE closure is off";
+    return HG_ERR_STATE;
+  }
+  HG_TRY(hipMemcpyAsync(w_bar, ctx->ude_wbar, hg::kUdeW * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D) {
+  if (!A || !B || !D || (precision != 32 && precision != 64)) return HG_ERR_ARG;
+  const size_t bytes = 512 * sizeof(double);
+  void *dA = nullptr, *dB = nullptr, *dD = nullptr;
+  if (hipMalloc(&dA, bytes) != hipSuccess || hipMalloc(&dB, bytes) != hipSuccess || hipMalloc(&dD, bytes) != hipSuccess) {
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    return HG_ERR_HIP;
+  }
+  hipError_t e = hipSuccess;
+  if (precision == 32) {
+    float a[64], b[64], d[256];
+    for (int k = 0; k < 64; ++k) {
+      a[k] = float(A[k]);
+      b[k] = float(B[k]);
+    }
+    if (e == hipSuccess) e = hipMemcpy(dA, a, sizeof(a), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, b, sizeof(b), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_mfma_probe<float>, dim3(1), dim3(64), 0, 0, static_cast<const float*>(dA),
+                         static_cast<const float*>(dB), static_cast<float*>(dD));
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(d, dD, sizeof(d), hipMemcpyDeviceToHost);
+    for (int k = 0; k < 256 && e == hipSuccess; ++k) D[k] = d[k];
+  } else {
+    if (e == hipSuccess) e = hipMemcpy(dA, A, 64 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dB, B, 64 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, 0, static_cast<const double*>(dA),
+                         static_cast<const double*>(dB), static_cast<double*>(dD));
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(D, dD, 256 * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dD);
+  return e == hipSuccess ? HG_OK : HG_ERR_HIP;
+}
+
 int hg_probe_copy(hg_ctx* ctx, int64_t nbytes, int32_t reps, double* gbps) {
   if (!ctx || nbytes < 16 || reps <= 0) return HG_ERR_ARG;
   const int64_t n2 = nbytes / 16;
@@ -1960,25 +2390,30 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   return rc;
 }
 
-int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_map) {
+int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk) {
   if (!ctx) return HG_ERR_ARG;
   if (!ctx->tiled) {
     ctx->err = "hg_set_tile_config: this context runs the per-cell kernel";
     return HG_ERR_STATE;
   }
-  ctx->xcd_map = xcd_map ? 1 : 0;
+  if (xcd_chunk < 0 || xcd_chunk > 4096) {
+    ctx->err = "hg_set_tile_config: xcd_chunk must be in [0, 4096]";
+    return HG_ERR_ARG;
+  }
+  ctx->xcd_chunk = xcd_chunk;
   if (tile_cells != ctx->tileT) return setup_tiles(ctx, tile_cells);
   return drop_graph(ctx);  // the captured launches hold the old mapping
 }
 
-int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_map) {
+int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk) {
   if (!ctx) return HG_ERR_ARG;
   if (tile_cells) *tile_cells = ctx->tiled ? ctx->tileT : 0;
-  if (xcd_map) *xcd_map = ctx->xcd_map;
+  if (xcd_chunk) *xcd_chunk = ctx->xcd_chunk;
   return HG_OK;
 }
 
-int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_map, double* ms_per_step) {
+int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_chunk,
+                double* ms_per_step) {
   if (!ctx || !(dt > 0.0) || nsteps <= 0) {
     if (ctx) ctx->err = "hg_autotune: need dt > 0 and nsteps > 0";
     return HG_ERR_ARG;
@@ -1989,7 +2424,7 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
   }
   const int64_t N = ctx->N;
   double best = INFINITY;
-  int32_t bestT = ctx->tileT, bestX = ctx->xcd_map;
+  int32_t bestT = ctx->tileT, bestX = ctx->xcd_chunk;
   if (ctx->tiled) {
     // keep the device state: every candidate starts from it, and it is restored at the end
     for (int k = 0; k < 3; ++k)
@@ -2001,7 +2436,7 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
     int rc = HG_OK;
     for (int T : {128, 256, 512}) {
       if (tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) continue;
-      for (int x : {1, 0}) {
+      for (int x : {0, 1, 2, 4, 8, 16, 64}) {
         rc = hg_set_tile_config(ctx, T, x);
         for (int k = 0; k < 3 && !rc; ++k)
           if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
@@ -2042,7 +2477,7 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
     best /= double(nsteps);
   }
   if (tile_cells) *tile_cells = ctx->tiled ? bestT : 0;
-  if (xcd_map) *xcd_map = bestX;
+  if (xcd_chunk) *xcd_chunk = bestX;
   if (ms_per_step) *ms_per_step = ctx->tiled ? best : 0.0;
   return HG_OK;
 }

@@ -25,8 +25,10 @@ EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
-    "hg_set_tile_config", "hg_get_tile_config", "hg_autotune",
+    "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
+    "hg_mfma_probe",
 ]
+UDE_WEIGHTS = 353
 
 
 class HGError(RuntimeError):
@@ -68,6 +70,10 @@ def lib():
         L.hg_get_tile_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.hg_autotune.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_double)]
+        L.hg_set_ude.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
+        L.hg_ude_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_ude_grad.argtypes = [C.c_void_p, C.c_void_p]
+        L.hg_mfma_probe.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         for name in ("hg_destroy",):
             getattr(L, name).argtypes = [C.c_void_p]
         L.hg_set_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 9
@@ -255,20 +261,51 @@ class Context:
                                         C.byref(b)))
         return a.value, b.value
 
-    def set_tile_config(self, tile_cells, xcd_map=True):
-        self._chk(lib().hg_set_tile_config(self.h, int(tile_cells), int(bool(xcd_map))))
+    def set_ude(self, weights, set_of_cell=None, precision=32):
+        """Manning n from the MLP closure; weights (n_sets, 353) or (353,);
+        weights=None turns it off."""
+        if weights is None:
+            self._chk(lib().hg_set_ude(self.h, 0, C.c_void_p(0), C.c_void_p(0), 32))
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1, UDE_WEIGHTS)
+        s = None if set_of_cell is None else np.ascontiguousarray(set_of_cell, dtype=np.int32)
+        self._chk(lib().hg_set_ude(self.h, w.shape[0], _ptr(w), _ptr(s), int(precision)))
+
+    def ude_n(self, Q):
+        n = np.empty(self.N)
+        self._chk(lib().hg_ude_n(self.h, _ptr(np.ascontiguousarray(Q, dtype=np.float64)), _ptr(n)))
+        return n
+
+    def ude_grad(self):
+        g = np.empty(UDE_WEIGHTS)
+        self._chk(lib().hg_ude_grad(self.h, _ptr(g)))
+        return g
+
+    def set_tile_config(self, tile_cells, xcd_chunk=0):
+        self._chk(lib().hg_set_tile_config(self.h, int(tile_cells), int(xcd_chunk)))
 
     def tile_config(self):
         t, x = C.c_int32(), C.c_int32()
         self._chk(lib().hg_get_tile_config(self.h, C.byref(t), C.byref(x)))
-        return t.value, bool(x.value)
+        return t.value, x.value
 
     def autotune(self, dt, nsteps=64):
         """Times every launch shape from the device state (restored
-        afterwards) and keeps the fastest: (tile_cells, xcd_map, ms/step)."""
+        afterwards) and keeps the fastest: (tile_cells, xcd_chunk, ms/step)."""
         t, x, ms = C.c_int32(), C.c_int32(), C.c_double()
         self._chk(lib().hg_autotune(self.h, C.c_double(dt), C.c_int64(nsteps), C.byref(t), C.byref(x), C.byref(ms)))
-        return t.value, bool(x.value), ms.value
+        return t.value, x.value, ms.value
+
+
+def mfma_probe(precision, A, B):
+    """D = A (16x4) B (4x16) on the matrix cores (lane-layout self-check)."""
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(16, 4)
+    B = np.ascontiguousarray(B, dtype=np.float64).reshape(4, 16)
+    D = np.empty((16, 16))
+    rc = lib().hg_mfma_probe(int(precision), _ptr(A), _ptr(B), _ptr(D))
+    if rc:
+        raise HGError(rc, "hg_mfma_probe failed")
+    return D
 
 
 def probe_copy(ctx, nbytes, reps=1):

@@ -165,16 +165,18 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
                     double* ms_step_kernel);
 
 /* Launch shape of the tiled step kernel (no reference counterpart): cells
- * per tile (128, 256 or 512; one work-item per cell) and whether work-groups
- * are mapped to tiles so that each XCD takes a contiguous run of tiles.
+ * per tile (128, 256 or 512; one work-item per cell) and the work-group to
+ * tile mapping: xcd_chunk = 0 maps work-group b to tile b; G > 0 gives each
+ * of the 8 XCDs runs of G consecutive tiles within windows of 8G tiles.
  * Every shape gives bit-identical results. */
-int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_map);
-int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_map);
+int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk);
+int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk);
 
 /* Autotuner: times nsteps steps of every launch shape from the current
  * device state (which is restored afterwards), keeps the fastest and
  * reports it.  Needs hg_set_fields and a device state (hg_state_upload). */
-int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_map, double* ms_per_step);
+int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_chunk,
+                double* ms_per_step);
 
 /* ---------------------------------------------------------- adjoint path */
 
@@ -204,6 +206,35 @@ int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_pa
 int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
                    int64_t kck, int64_t n_probe, const int32_t* probe, const double* target, double* loss,
                    double* n_bar, double* zb_bar, double* inletQ_bar);
+
+/* ------------------------------------------------------- UDE closure */
+
+/* Manning's n from a neural closure, evaluated on the matrix cores every
+ * RHS evaluation: n_i = 0.01 + softplus(MLP(h_i, |u|_i, zb_i)), MLP =
+ * 3 -> 16 -> 16 -> 1 with tanh hidden layers.  It has the role of the
+ * reference's UDE ManningN (update_ManningN_UDE, process_ManningN_2D.jl:
+ * 216-273, called from semi_discretize_swe_2D.jl:165-180) with this network.
+ * Features: h = xi + h_still clamped to h_small (then q = 0), |u| =
+ * sqrt(u^2 + v^2 + eps) with u = q_x / h, v = q_y / h, and zb.
+ * weights: n_sets x 353 values, per set W1[16][3], b1[16], W2[16][16]
+ * (row = output unit), b2[16], W3[16], b3.  set_of_cell (caller order, NULL =
+ * set 0 everywhere) picks each cell's set, e.g. one per batched instance.
+ * precision: 32 (float, the production setting) or 64 (double).  n_sets = 0
+ * turns the closure off and restores the n of hg_set_fields.  While on, n
+ * cannot be the active parameter. */
+int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t* set_of_cell, int32_t precision);
+
+/* The closure's n (N, caller order) for state Q (3N). */
+int hg_ude_n(hg_ctx* ctx, const double* Q, double* n);
+
+/* Cotangent of the weights (353) accumulated by the last hg_rhs_vjp,
+ * hg_adjoint or hg_grad_probes call with the closure on (one weight set). */
+int hg_ude_grad(hg_ctx* ctx, double* w_bar);
+
+/* Self-check with no counterpart in the reference: D = A B on the matrix
+ * cores for A (16x4) and B (4x16) row-major, D (16x16) row-major, in the
+ * given precision (32 or 64); verifies the lane layout the closure uses. */
+int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D);
 
 /* Measurement helper with no counterpart in the reference: copies nbytes
  * between two scratch device buffers `reps` times with a plain streaming

@@ -1,0 +1,196 @@
+"""GPU checks of the UDE closure n = 0.01 + softplus(MLP(h, |u|, zb)).
+
+* The matrix-core lane layout (16x16x4, f32 and f64) against A @ B.
+* The closure's n against a NumPy statement of the same network (f64:
+  1e-13 relative; f32: 1e-5).
+* The per-cell-n solve is the judge of the step: a step with the closure
+  on is bitwise the step of the per-cell-n path given the closure's n, and
+  N steps match the oracle driven by the NumPy closure (1e-10).
+* Reverse mode through the closure (f64): weight and initial-state
+  gradients of the probe loss against central differences; f32 gradients
+  against f64 ones.
+* Batched instances with one weight set each equal separate runs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from hgswe import capi, cases, ude
+
+import pyoracle
+from ude_ref import features, mlp_n
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_case(seed=5, n=4000):
+    return cases.channel(n, seed=seed, Lx=1200.0, Ly=120.0, Q_in=40.0, nsteps=40, n_bumps=5)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_mfma_lane_layout(prec):
+    rng = np.random.default_rng(prec)
+    A, B = rng.normal(size=(16, 4)), rng.normal(size=(4, 16))
+    if prec == 32:
+        A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    D = capi.mfma_probe(prec, A, B)
+    err = float(np.max(np.abs(D - A @ B)))
+    say(f"  mfma f{prec}: max err {err:.2e}")
+    assert err <= (1e-5 if prec == 32 else 1e-14)
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-13), (32, 1e-5)])
+def test_closure_n_matches_numpy(prec, tol):
+    c = small_case()
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 30)
+    W = ude.xavier_weights(3)
+    ctx = capi.Context(c)
+    ctx.set_ude(W, precision=prec)
+    n = ctx.ude_n(Q)
+    ref = mlp_n(W, features(c, Q))
+    rel = float(np.max(np.abs(n - ref) / ref))
+    say(f"  closure n f{prec}: max rel {rel:.2e}, n in [{n.min():.4f}, {n.max():.4f}]")
+    assert rel <= tol
+
+
+def test_closure_step_is_per_cell_n_step():
+    c = small_case()
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 20)
+    W = ude.xavier_weights(4)
+    u = capi.Context(c)
+    u.set_ude(W, precision=32)
+    n = u.ude_n(Q)
+    a = u.step(Q, c.dt, 1)
+    p = capi.Context(c.copy_fields(n=n))
+    b = p.step(Q, c.dt, 1)
+    assert np.array_equal(a, b)
+    r1 = u.rhs(Q)
+    r2 = p.rhs(Q)
+    assert np.array_equal(r1, r2)
+
+
+def test_closure_steps_match_oracle():
+    c = small_case()
+    W = ude.xavier_weights(5)
+    ctx = capi.Context(c)
+    ctx.set_ude(W, precision=64)
+    nsteps = 25
+    qg = ctx.step(c.Q0, c.dt, nsteps)
+    q = c.Q0.copy()
+    orc = pyoracle.Oracle(c)
+    for _ in range(nsteps):
+        orc.set_fields(n=mlp_n(W, features(c, q)))
+        q = orc.run(q, c.dt, 1)
+    rel = float(np.max(np.abs(qg - q)) / np.max(np.abs(q)))
+    say(f"  closure {nsteps} steps vs oracle + NumPy closure: rel {rel:.2e}")
+    assert rel <= 1e-10
+
+
+def fd4(f, x, v, e):
+    """Fourth-order central difference of f at x along v."""
+    return (8.0 * (f(x + e * v) - f(x - e * v)) - (f(x + 2 * e * v) - f(x - 2 * e * v))) / (12.0 * e)
+
+
+def _probe(c, nsteps, seed=9, k=25):
+    rng = np.random.default_rng(seed)
+    probe = rng.choice(c.N, size=k, replace=False).astype(np.int32)
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, nsteps)
+    target = Q[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.02, k)
+    return probe, target
+
+
+def test_closure_weight_gradient_fd():
+    c = small_case()
+    nsteps, kck = 30, 7
+    probe, target = _probe(c, nsteps)
+    W = ude.xavier_weights(6)
+    ctx = capi.Context(c)
+    ctx.set_ude(W, precision=64)
+    loss, _, _, _ = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    gw = ctx.ude_grad()
+    again = capi.Context(c)
+    again.set_ude(W, precision=64)
+    again.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    assert np.array_equal(gw, again.ude_grad())  # deterministic
+
+    def L(w):
+        k = capi.Context(c)
+        k.set_ude(w, precision=64)
+        q = k.step(c.Q0, c.dt, nsteps)
+        r = q[probe] + c.hstill[probe] + c.zb[probe] - target
+        return float(np.sum(r * r))
+
+    rng = np.random.default_rng(2)
+    for trial in range(3):
+        # the loss moves little with the weights, so a small step drowns in
+        # round-off; a fourth-order stencil with a large step is accurate
+        v = rng.normal(size=W.size)
+        fd = fd4(L, W, v, 5e-3)
+        gv = float(np.dot(gw, v))
+        say(f"  closure weights dir {trial}: fd {fd:.10e} adjoint {gv:.10e} rel {abs(fd - gv) / abs(gv):.2e}")
+        assert abs(fd - gv) <= 1e-5 * abs(gv)  # difference-quotient round-off and truncation at this step
+    # initial-state gradient through the closure
+    lb = np.zeros(3 * c.N)
+    rb = 2.0 * (ctx.step(c.Q0, c.dt, nsteps)[probe] + c.hstill[probe] + c.zb[probe] - target)
+    np.add.at(lb, probe, rb)
+    Q0b, _, _, _ = ctx.adjoint(c.Q0, c.dt, nsteps, kck, lb)
+    v = np.zeros(3 * c.N)
+    v[: c.N] = rng.normal(size=c.N) * 1e-2
+
+    def Lq(q0):
+        k = capi.Context(c)
+        k.set_ude(W, precision=64)
+        q = k.step(q0, c.dt, nsteps)
+        r = q[probe] + c.hstill[probe] + c.zb[probe] - target
+        return float(np.sum(r * r))
+
+    fd = fd4(Lq, c.Q0, v, 0.05)
+    gv = float(np.dot(Q0b, v))
+    say(f"  closure Q0 dir: fd {fd:.10e} adjoint {gv:.10e} rel {abs(fd - gv) / abs(gv):.2e}")
+    assert abs(fd - gv) <= 1e-5 * abs(gv)
+
+
+def test_closure_f32_gradient_tracks_f64():
+    c = small_case()
+    nsteps, kck = 30, 10
+    probe, target = _probe(c, nsteps)
+    W = ude.xavier_weights(7)
+    g = {}
+    for prec in (32, 64):
+        ctx = capi.Context(c)
+        ctx.set_ude(W, precision=prec)
+        ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+        g[prec] = ctx.ude_grad()
+    rel = float(np.linalg.norm(g[32] - g[64]) / np.linalg.norm(g[64]))
+    say(f"  closure gradient f32 vs f64: rel {rel:.2e}")
+    assert rel <= 1e-3
+
+
+def test_batched_weight_sets_equal_separate_runs():
+    inst = [small_case(seed=s, n=2500) for s in (1, 2, 3)]
+    B = cases.concat(inst)
+    Ws = ude.perturbed_sets(ude.xavier_weights(8), 3, scale=0.1)
+    set_of_cell = np.repeat(np.arange(3), [c.N for c in inst]).astype(np.int32)
+    ctx = capi.Context(B)
+    ctx.set_ude(Ws, set_of_cell=set_of_cell, precision=32)
+    qb = ctx.step(B.Q0, B.dt, 30)
+    for r, (c, q) in enumerate(zip(inst, cases.split_state(B, qb))):
+        k = capi.Context(c)
+        k.set_ude(Ws[r], precision=32)
+        assert np.array_equal(q, k.step(c.Q0, B.dt, 30)), r
+
+
+def test_closure_rejects_n_as_active_parameter():
+    c = small_case()
+    ctx = capi.Context(c)
+    ctx.set_ude(ude.xavier_weights(1), precision=32)
+    with pytest.raises(capi.HGError):
+        ctx.set_active_param(capi.HG_PARAM_N_CELL)
+    ctx.set_ude(None)
+    ctx.set_active_param(capi.HG_PARAM_N_CELL)
