@@ -172,6 +172,56 @@ def measure_batch(args, capi, single_value):
     return out
 
 
+def measure_ude(args, capi):
+    """UDE closure workload: ~50k-triangle channel, n = 0.01 + softplus(MLP(h,
+    |u|, zb)) in f32 on the matrix cores every step; 32 perturbed weight sets
+    advanced together as a batch; gradient of the probe loss w.r.t. the 353
+    weights."""
+    from hgswe import cases, ude
+
+    c = cases.channel(args.ude_cells, seed=4, nsteps=args.ude_steps)
+    W = ude.xavier_weights(0)
+    ctx = capi.Context(c)
+    ctx.set_ude(W, precision=32)
+    ctx.upload(c.Q0)
+    ctx.advance(c.dt, args.warmup)
+    t0 = time.perf_counter()
+    ctx.advance(c.dt, args.ude_steps)
+    el = time.perf_counter() - t0
+    out = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / args.ude_steps,
+           "config": {"workload": "channel_50k_tri_ude_closure_f32", "cells": c.N, "steps": args.ude_steps,
+                      "dt": c.dt, "network": "3-16-16-1 tanh, 353 weights, f32 matrix cores"}}
+    # 32 weight perturbations batched: 32 copies of the mesh, one weight set each
+    nset = args.ude_sets
+    B = cases.concat([c] * nset)
+    Ws = ude.perturbed_sets(W, nset, scale=0.05)
+    set_of_cell = np.repeat(np.arange(nset), c.N).astype(np.int32)
+    bctx = capi.Context(B)
+    bctx.set_ude(Ws, set_of_cell=set_of_cell, precision=32)
+    bctx.upload(B.Q0)
+    bctx.advance(B.dt, args.warmup)
+    t0 = time.perf_counter()
+    bctx.advance(B.dt, args.ude_steps)
+    el = time.perf_counter() - t0
+    del bctx
+    out["batched_sets"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "sets": nset,
+                           "cells": B.N, "ms_per_step": el * 1e3 / args.ude_steps}
+    # gradient w.r.t. the weights
+    rng = np.random.default_rng(5)
+    probe = rng.choice(c.N, size=200, replace=False).astype(np.int32)
+    QN = ctx.step(c.Q0, c.dt, args.ude_steps)
+    target = QN[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.02, probe.size)
+    ctx.grad_probes(c.Q0, c.dt, 10, 5, probe, target)  # warm-up
+    t0 = time.perf_counter()
+    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.grad_kck, probe, target)
+    gw = ctx.ude_grad()
+    el = time.perf_counter() - t0
+    out["gradient"] = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+                       "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "wrt": "353 closure weights",
+                       "grad_norm": float(np.linalg.norm(gw))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +237,10 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--batch-grad-steps", type=int, default=200)
+    ap.add_argument("--ude-cells", type=int, default=50_000)
+    ap.add_argument("--ude-steps", type=int, default=1000)
+    ap.add_argument("--ude-sets", type=int, default=32)
+    ap.add_argument("--no-ude", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (256-cell tiles, identity tile mapping)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
@@ -248,6 +302,8 @@ def main():
     if args.batch > 0:
         del ctx
         out["batch"] = measure_batch(args, capi, value)
+    if not args.no_ude:
+        out["ude"] = measure_ude(args, capi)
     print(json.dumps(out), flush=True)
 
 

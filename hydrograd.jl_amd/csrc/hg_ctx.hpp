@@ -192,6 +192,8 @@ struct hg_ctx {
   double *g_hstill = nullptr, *g_zb = nullptr, *g_zb0 = nullptr, *g_L23 = nullptr;
   int32_t *in_off = nullptr, *in_cell = nullptr, *in_ghost = nullptr;
   double* in_L53 = nullptr;
+  int32_t* in_next = nullptr;   // inlet face -> next face of the same cell in that inlet, or -1
+  int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
   int32_t* zone = nullptr;

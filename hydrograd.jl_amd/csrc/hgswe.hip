@@ -22,6 +22,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hg_ctx.hpp"
@@ -638,30 +639,53 @@ __global__ __launch_bounds__(kBlock) void k_swe_adj(hg::AdjArgs a) {
 // faces in NodeString order (deterministic; a cell may own several inlet
 // faces), adds d/dh of A_b to the state cotangent and accumulates d/dn and
 // d/dQ_b.
-__global__ void k_inlet_adj(int n_inlet, const int32_t* off, const int32_t* ghost, const int32_t* cell,
-                            const double* L53, const double* g_L23, const double* xi, const double* hstill,
-                            const double* n, const double* inletQ, const double* totalA, const double* vbar,
-                            double hsmall, double* ox, double* nbar, double* Qbar) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t* off, const int32_t* ghost,
+                                                     const int32_t* cell, const int32_t* next,
+                                                     const int32_t* first, const double* L53,
+                                                     const double* g_L23, const double* xi, const double* hstill,
+                                                     const double* n, const double* inletQ, const double* totalA,
+                                                     const double* vbar, double hsmall, double* ox, double* nbar,
+                                                     double* Qbar) {
+  // One work-group for all inlets (a cell may sit on two of them): the
+  // per-face terms are formed in parallel and summed by a fixed tree; each
+  // cell's faces of an inlet are applied, in NodeString order, by the
+  // work-item of its first face (first[f] = 1; next[f]: the cell's next
+  // face in the inlet, or -1).
+  __shared__ double sA[kBlock], sQ[kBlock];
+  const int tid = threadIdx.x;
   for (int b = 0; b < n_inlet; ++b) {
     const double A = totalA[b];
-    double Ab = 0.0, Qb = 0.0;
-    for (int f = off[b]; f < off[b + 1]; ++f) {
+    double pa = 0.0, pq = 0.0;
+    for (int f = off[b] + tid; f < off[b + 1]; f += kBlock) {
       const int gi = ghost[f], c = cell[f];
       const double vb = vbar[gi];
       const double v = inletQ[b] / A * g_L23[gi] / n[c];
-      Ab += vb * (-v / A);
-      Qb += vb * (g_L23[gi] / (A * n[c]));
+      pa += vb * (-v / A);
+      pq += vb * (g_L23[gi] / (A * n[c]));
     }
-    Qbar[b] += Qb;
-    for (int f = off[b]; f < off[b + 1]; ++f) {
-      const int c = cell[f];
-      double h = xi[c] + hstill[c];
-      const bool wet = h > hsmall;
-      if (!wet) continue;  // w = 0 and h clamped: no derivative
-      ox[c] += Ab * L53[f] / n[c];
-      nbar[c] += Ab * (-L53[f] * h / (n[c] * n[c]));
+    sA[tid] = pa;
+    sQ[tid] = pq;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+        sA[tid] += sA[tid + w];
+        sQ[tid] += sQ[tid + w];
+      }
+      __syncthreads();
     }
+    const double Ab = sA[0];
+    if (tid == 0) Qbar[b] += sQ[0];
+    for (int f = off[b] + tid; f < off[b + 1]; f += kBlock) {
+      if (!first[f]) continue;
+      for (int g = f; g >= 0; g = next[g]) {
+        const int c = cell[g];
+        const double h = xi[c] + hstill[c];
+        if (!(h > hsmall)) continue;  // w = 0 and h clamped: no derivative
+        ox[c] += Ab * L53[g] / n[c];
+        nbar[c] += Ab * (-L53[g] * h / (n[c] * n[c]));
+      }
+    }
+    __syncthreads();  // sA / sQ reuse; cells shared by two inlets take them in inlet order
   }
 }
 
@@ -767,40 +791,91 @@ __device__ __forceinline__ T ude_out(const T* __restrict__ W, const T* A2, int l
   return z3;
 }
 
-// n = 0.01 + softplus(MLP(features)) for every cell; a wave whose cells use
-// several weight sets makes one pass per set (the matrix-core products need
-// one set across the wave).
+// One weight set held in registers, laid out for the lane's share of the
+// matrix-core operands (see ude_hidden) and for the output layer.
 template <class T>
-__global__ __launch_bounds__(64) void k_ude_n(hg::UdeArgs a) {
-  __shared__ T F[64 * 4];
-  __shared__ T A1[64 * 16];
-  __shared__ T A2[64 * 16];
-  const int l = threadIdx.x;
-  const int i = blockIdx.x * 64 + l;
-  const bool valid = i < a.N;
-  double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
-  bool wet = false;
-  int set = 0;
-  if (valid) {
-    hg::ude_features(a, i, h, u, v, um, wet);
-    zb = a.zb[i];
-    if (a.set_of_cell) set = a.set_of_cell[i];
+struct UdeRegs {
+  T w1, b1, w2[4], b2, w3[16], b3;
+  __device__ __forceinline__ void load(const T* __restrict__ W, int col, int kq) {
+    w1 = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : T(0);
+    b1 = W[hg::kUdeB1 + col];
+    for (int kc = 0; kc < 4; ++kc) w2[kc] = W[hg::kUdeW2 + col * 16 + 4 * kc + kq];
+    b2 = W[hg::kUdeB2 + col];
+    for (int j = 0; j < 16; ++j) w3[j] = W[hg::kUdeW3 + j];
+    b3 = W[hg::kUdeB3];
   }
-  F[l * 4 + 0] = T(h);
-  F[l * 4 + 1] = T(um);
-  F[l * 4 + 2] = T(zb);
-  F[l * 4 + 3] = T(0);
+};
+
+// ude_hidden with the weights in registers.
+template <class T>
+__device__ __forceinline__ void ude_hidden_r(const UdeRegs<T>& R, const T* F, T* A1, T* A2, int l) {
+  using M = hg::Mfma<T>;
+  const int col = l & 15, kq = l >> 4;
+  for (int rb = 0; rb < 4; ++rb) {
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    acc = M::op(F[(16 * rb + col) * 4 + kq], R.w1, acc);
+    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b1);
+  }
   hg::wave_sync();
-  uint64_t pending = __ballot(valid);
-  while (pending) {
-    const int lead = __ffsll((unsigned long long)pending) - 1;
-    const int s = __shfl(set, lead, 64);
-    const bool mine = valid && set == s;
-    pending &= ~uint64_t(__ballot(mine));
-    const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
-    ude_hidden<T>(W, F, A1, A2, l);
-    if (mine) a.n[i] = 0.01 + double(hg::softplus<T>(ude_out<T>(W, A2, l)));
-    hg::wave_sync();  // the next pass rewrites A1, A2
+  for (int rb = 0; rb < 4; ++rb) {
+    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    for (int kc = 0; kc < 4; ++kc) acc = M::op(A1[(16 * rb + col) * 16 + 4 * kc + kq], R.w2[kc], acc);
+    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b2);
+  }
+  hg::wave_sync();
+}
+
+// n = 0.01 + softplus(MLP(features)) for every cell.  Each wave walks
+// 64-cell chunks (grid-stride) with its weight set in registers; a chunk
+// whose cells use several sets takes one pass per set (the matrix-core
+// products need one set across the wave).
+template <class T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
+  __shared__ T Fs[WPB][64 * 4];
+  __shared__ T A1s[WPB][64 * 16];
+  __shared__ T A2s[WPB][64 * 16];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int col = l & 15, kq = l >> 4;
+  T* F = Fs[w];
+  T* A1 = A1s[w];
+  T* A2 = A2s[w];
+  const int nchunks = (a.N + 63) / 64;
+  UdeRegs<T> R;
+  int cur = -1;
+  for (int ch = blockIdx.x * WPB + w; ch < nchunks; ch += gridDim.x * WPB) {
+    const int i = ch * 64 + l;
+    const bool valid = i < a.N;
+    double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
+    bool wet = false;
+    int set = 0;
+    if (valid) {
+      hg::ude_features(a, i, h, u, v, um, wet);
+      zb = a.zb[i];
+      if (a.set_of_cell) set = a.set_of_cell[i];
+    }
+    F[l * 4 + 0] = T(h);
+    F[l * 4 + 1] = T(um);
+    F[l * 4 + 2] = T(zb);
+    F[l * 4 + 3] = T(0);
+    hg::wave_sync();
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+      const int lead = __ffsll((unsigned long long)pending) - 1;
+      const int s = __shfl(set, lead, 64);
+      const bool mine = valid && set == s;
+      pending &= ~uint64_t(__ballot(mine));
+      if (s != cur) {
+        R.load(static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW, col, kq);
+        cur = s;
+      }
+      ude_hidden_r<T>(R, F, A1, A2, l);
+      if (mine) {
+        T z3 = R.b3;
+        for (int j = 0; j < 16; ++j) z3 = z3 + A2[l * 16 + j] * R.w3[j];
+        a.n[i] = 0.01 + double(hg::softplus<T>(z3));
+      }
+      hg::wave_sync();  // the next pass rewrites A1, A2
+    }
   }
 }
 
@@ -913,16 +988,24 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
     Wb[dst] = sacc;
   }
   __syncthreads();
-  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(blockIdx.x) * hg::kUdeW + k] = Wb[k];
+  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] = Wb[k];
 }
 
-// wbar[k] += sum over blocks of wpart[b][k], in block order (deterministic).
-__global__ __launch_bounds__(128) void k_ude_wsum(int nblocks, const double* wpart, double* wbar) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= hg::kUdeW) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += wpart[size_t(b) * hg::kUdeW + k];
-  wbar[k] += s;
+// wbar[k] += sum over blocks b of wpart[k][b]; one work-group per weight,
+// strided partial sums then a fixed tree (deterministic).
+__global__ __launch_bounds__(kBlock) void k_ude_wsum(int nblocks, const double* wpart, double* wbar) {
+  __shared__ double s[kBlock];
+  const int k = blockIdx.x, tid = threadIdx.x;
+  const double* p = wpart + size_t(k) * nblocks;
+  double acc = 0.0;
+  for (int b = tid; b < nblocks; b += kBlock) acc += p[b];
+  s[tid] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) s[tid] += s[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) wbar[k] += s[0];
 }
 
 // Layout check of the 16x16x4 matrix-core products used above: D = A B for
@@ -1200,10 +1283,14 @@ int ude_blocks(const hg_ctx* ctx) { return int((ctx->N + 63) / 64); }
 int enqueue_ude_n(hg_ctx* ctx, const double* const q[3]) {
   if (!ctx->ude_prec) return HG_OK;
   const hg::UdeArgs u = ude_args(ctx, q);
-  if (ctx->ude_prec == 32)
-    hipLaunchKernelGGL(k_ude_n<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
-  else
-    hipLaunchKernelGGL(k_ude_n<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+  const int chunks = ude_blocks(ctx);  // 64-cell chunks
+  if (ctx->ude_prec == 32) {
+    const int grid = std::min((chunks + 3) / 4, 4096);
+    hipLaunchKernelGGL((k_ude_n<float, 4>), dim3(grid), dim3(256), 0, ctx->stream, u);
+  } else {
+    const int grid = std::min((chunks + 1) / 2, 8192);
+    hipLaunchKernelGGL((k_ude_n<double, 2>), dim3(grid), dim3(128), 0, ctx->stream, u);
+  }
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
@@ -1356,8 +1443,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   else
     hipLaunchKernelGGL(k_swe_adj<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   if (ctx->n_inlet > 0)
-    hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(64), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
-                       ctx->in_ghost, ctx->in_cell, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
+    hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
+                       ctx->in_ghost, ctx->in_cell, ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
                        ctx->totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar, ctx->adj_Qbar);
   hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
                      ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
@@ -1373,8 +1460,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
       hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
     else
       hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
-    hipLaunchKernelGGL(k_ude_wsum, dim3((hg::kUdeW + 127) / 128), dim3(128), 0, ctx->stream, ude_blocks(ctx),
-                       ctx->ude_wpart, ctx->ude_wbar);
+    hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW), dim3(kBlock), 0, ctx->stream, ude_blocks(ctx), ctx->ude_wpart,
+                       ctx->ude_wbar);
   }
   HG_TRY(hipGetLastError());
   return HG_OK;
@@ -1865,6 +1952,8 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = dalloc(ctx, &ctx->in_cell, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_ghost, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_L53, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->in_next, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->in_first, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ, b->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ0, b->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->exitWSE, b->n_exit);
@@ -1890,6 +1979,24 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = upload(ctx, ctx->in_cell, in_cell.data(), ctx->n_inlet_faces);
   if (!rc && ctx->n_inlet_faces) rc = upload(ctx, ctx->in_ghost, b->inlet_ghost, ctx->n_inlet_faces);
   if (!rc) rc = upload(ctx, ctx->in_L53, in_L53.data(), ctx->n_inlet_faces);
+  if (!rc) {  // per inlet, the faces of each cell in NodeString order (see k_inlet_adj)
+    std::vector<int32_t> next(std::max<int32_t>(ctx->n_inlet_faces, 1), -1), first(next.size(), 1);
+    for (int bi = 0; bi < b->n_inlet; ++bi) {
+      std::unordered_map<int32_t, int32_t> last;  // cell -> its latest face in this inlet
+      for (int32_t f = in_off[bi]; f < in_off[bi + 1]; ++f) {
+        auto it = last.find(in_cell[f]);
+        if (it == last.end()) {
+          last.emplace(in_cell[f], f);
+        } else {
+          next[it->second] = f;
+          first[f] = 0;
+          it->second = f;
+        }
+      }
+    }
+    rc = upload(ctx, ctx->in_next, next.data(), ctx->n_inlet_faces);
+    if (!rc) rc = upload(ctx, ctx->in_first, first.data(), ctx->n_inlet_faces);
+  }
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
 
