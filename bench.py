@@ -241,6 +241,7 @@ def main():
     ap.add_argument("--ude-steps", type=int, default=1000)
     ap.add_argument("--ude-sets", type=int, default=32)
     ap.add_argument("--no-ude", action="store_true")
+    ap.add_argument("--tile-config", default="", help="T,G: fixed launch shape (tile cells, XCD chunk); no autotune")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (256-cell tiles, identity tile mapping)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
@@ -260,7 +261,10 @@ def main():
     ctx = capi.Context(case)
     print(f"# {capi.info()}", file=sys.stderr, flush=True)
     ctx.upload(case.Q0)
-    if not args.no_tune:  # launch shapes are bit-identical; keep the fastest
+    if args.tile_config:
+        T, G = (int(v) for v in args.tile_config.split(","))
+        ctx.set_tile_config(T, G)
+    elif not args.no_tune:  # launch shapes are bit-identical; keep the fastest
         T, x, _ = ctx.autotune(case.dt, 64)
         print(f"# autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     ctx.advance(case.dt, args.warmup)

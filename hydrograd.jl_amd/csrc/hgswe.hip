@@ -2475,7 +2475,7 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   HG_TRY(hipEventElapsedTime(&ms, e0, e1));
   if (ms_per_step) *ms_per_step = double(ms) / double(reps * nsteps);
   if (ms_step_kernel && !rc) {  // the step kernel alone, back-to-back on the same stream
-    const int K = 50;
+    const int K = 200;
     HG_TRY(hipEventRecord(e0, ctx->stream));
     for (int k = 0; k < K; ++k) {
       if (ctx->tiled) {
