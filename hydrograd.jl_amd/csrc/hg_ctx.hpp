@@ -233,6 +233,10 @@ struct hg_ctx {
   double* ude_wbar = nullptr;       // accumulated weight cotangent (353)
   std::vector<void*> ude_allocs;
 
+  // Tsit5 work buffers: 7 stages + y, ynew, ytmp (3N each), norm partials
+  double* rk = nullptr;
+  double* rk_part = nullptr;
+
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;
   double gdt = 0.0;

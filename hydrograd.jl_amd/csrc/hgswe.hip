@@ -1019,6 +1019,79 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const T* A, const T* B, T* D)
   for (int r = 0; r < 4; ++r) D[M::row(kq, r) * 16 + col] = acc[r];
 }
 
+// ---- Tsit5: Tsitouras' 5(4) Runge-Kutta pair (Ch. Tsitouras, Comput. Math.
+// Appl. 62 (2011) 770-775), first-same-as-last, the adaptive integrator the
+// reference's examples run through OrdinaryDiffEq (Tsit5()).
+namespace tsit5 {
+constexpr double c[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+constexpr double a[7][6] = {
+    {0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {0.161, 0.0, 0.0, 0.0, 0.0, 0.0},
+    {-0.008480655492356989, 0.335480655492357, 0.0, 0.0, 0.0, 0.0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0.0, 0.0, 0.0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0.0, 0.0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0.0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774}};
+// b = a[6] (fifth order, b7 = 0); btilde = b - bhat (bhat: the embedded fourth-order weights)
+constexpr double btilde[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                              -0.1447110071732629,     0.5823571654525552,    -0.45808210592918697,
+                              0.015151515151515152};
+}  // namespace tsit5
+
+struct RkArgs {
+  int64_t n;             // 3N
+  const double* y;
+  double* out;
+  const double* k[7];
+  double coef[7];
+  int nk;
+};
+
+// out = y + sum_j coef[j] k[j] (coef already scaled by the step).
+__global__ __launch_bounds__(kBlock) void k_rk_combine(RkArgs a) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  double s = 0.0;
+  for (int j = 0; j < a.nk; ++j) s += a.coef[j] * a.k[j][i];
+  a.out[i] = a.y[i] + s;
+}
+
+// Partial sums of ((sum_j coef[j] k[j]) / (abstol + reltol max(|y|, |ynew|)))^2
+// per work-group (fixed tree), for the RMS error norm.
+__global__ __launch_bounds__(kBlock) void k_rk_err(RkArgs a, const double* ynew, double abstol, double reltol,
+                                                   double* part) {
+  __shared__ double s[kBlock];
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  double v = 0.0;
+  if (i < a.n) {
+    double e = 0.0;
+    for (int j = 0; j < a.nk; ++j) e += a.coef[j] * a.k[j][i];
+    const double sc = abstol + reltol * fmax(fabs(a.y[i]), fabs(ynew[i]));
+    v = (e / sc) * (e / sc);
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// out[0] = sum of part[0 .. n) by one work-group (fixed tree).
+__global__ __launch_bounds__(kBlock) void k_sum(int n, const double* part, double* out) {
+  __shared__ double s[kBlock];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n; b += kBlock) acc += part[b];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
 // Work items of the tiled kernel (see TileArgs): per tile, cross-tile faces
 // first (the kernel prefetches their neighbour states before its first
 // barrier), then boundary faces, then intra-tile faces (owned by the side
@@ -2423,6 +2496,143 @@ int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D
   (void)hipFree(dB);
   (void)hipFree(dD);
   return e == hipSuccess ? HG_OK : HG_ERR_HIP;
+}
+
+int hg_tsit5_tableau(double* c, double* a, double* btilde) {
+  for (int i = 0; i < 7; ++i) {
+    if (c) c[i] = tsit5::c[i];
+    if (btilde) btilde[i] = tsit5::btilde[i];
+    for (int j = 0; j < 6; ++j)
+      if (a) a[i * 6 + j] = tsit5::a[i][j];
+  }
+  return HG_OK;
+}
+
+int hg_tsit5_solve(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double t0, double t1,
+                   int64_t n_save, double abstol, double reltol, double dt0, double* Qsave, int64_t* stats) {
+  if (!ctx || !Q0 || !Qsave) return ctx ? (ctx->err = "hg_tsit5_solve: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (!(t1 > t0) || n_save <= 0 || !(abstol > 0.0) || !(reltol >= 0.0) || !(dt0 > 0.0)) {
+    ctx->err = "hg_tsit5_solve: need t1 > t0, n_save > 0, abstol > 0, reltol >= 0, dt0 > 0";
+    return HG_ERR_ARG;
+  }
+  const int64_t N = ctx->N, n3 = 3 * N;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (rc) return rc;
+  if (!ctx->rk) {
+    rc = dalloc(ctx, &ctx->rk, size_t(10) * n3);
+    if (!rc) rc = dalloc(ctx, &ctx->rk_part, size_t((n3 + kBlock - 1) / kBlock) + 1);
+    if (rc) return rc;
+  }
+  double* K[7];
+  for (int j = 0; j < 7; ++j) K[j] = ctx->rk + j * n3;
+  double* y = ctx->rk + 7 * n3;
+  double* ynew = ctx->rk + 8 * n3;
+  double* ytmp = ctx->rk + 9 * n3;
+  double* errv = ctx->rk_part + (n3 + kBlock - 1) / kBlock;
+  const int grid = int((n3 + kBlock - 1) / kBlock);
+  auto trip = [N](double* p, const double** c) {
+    c[0] = p;
+    c[1] = p + N;
+    c[2] = p + 2 * N;
+  };
+  auto rhs = [&](const double* in, double* out) {
+    const double* i3[3];
+    trip(const_cast<double*>(in), i3);
+    double* o3[3] = {out, out + N, out + 2 * N};
+    return enqueue_step_p(ctx, i3, o3, 0.0, false);
+  };
+  auto combine = [&](const double* base, double* out, double h, const double* w, int nk) {
+    RkArgs r{};
+    r.n = n3;
+    r.y = base;
+    r.out = out;
+    r.nk = nk;
+    for (int j = 0; j < nk; ++j) {
+      r.k[j] = K[j];
+      r.coef[j] = h * w[j];
+    }
+    hipLaunchKernelGGL(k_rk_combine, dim3(grid), dim3(kBlock), 0, ctx->stream, r);
+  };
+  auto save = [&](int64_t idx) {
+    double* src[3] = {y, y + N, y + 2 * N};
+    return state_out(ctx, src, Qsave + idx * n3);
+  };
+  rc = state_in(ctx, Q0, 0);
+  for (int c3 = 0; c3 < 3 && !rc; ++c3)
+    if (hipMemcpyAsync(y + c3 * N, ctx->st[0][c3], N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) !=
+        hipSuccess)
+      rc = HG_ERR_HIP;
+  if (!rc) rc = clear_flags(ctx);
+  if (!rc) rc = save(0);
+  if (!rc) rc = rhs(y, K[0]);
+  if (rc) return rc;
+  int64_t accepted = 0, rejected = 0, nrhs = 1;
+  double t = t0, dt = dt0;
+  const int64_t max_steps = 100000000;
+  for (int64_t si = 1; si <= n_save && !rc; ++si) {
+    const double tt = (si == n_save) ? t1 : t0 + (t1 - t0) * double(si) / double(n_save);
+    while (t < tt && !rc) {
+      if (accepted + rejected >= max_steps) {
+        ctx->err = "hg_tsit5_solve: too many steps";
+        return HG_ERR_STATE;
+      }
+      const bool last = dt >= (tt - t) * (1.0 - 1e-12);
+      const double h = last ? tt - t : dt;
+      for (int st = 1; st < 7 && !rc; ++st) {  // stages 2..7; stage 7's input is the new solution
+        double* dst = (st == 6) ? ynew : ytmp;
+        combine(y, dst, h, tsit5::a[st], st);
+        rc = rhs(dst, K[st]);
+      }
+      nrhs += 6;
+      if (rc) break;
+      RkArgs r{};
+      r.n = n3;
+      r.y = y;
+      r.nk = 7;
+      for (int j = 0; j < 7; ++j) {
+        r.k[j] = K[j];
+        r.coef[j] = h * tsit5::btilde[j];
+      }
+      hipLaunchKernelGGL(k_rk_err, dim3(grid), dim3(kBlock), 0, ctx->stream, r, ynew, abstol, reltol,
+                         ctx->rk_part);
+      hipLaunchKernelGGL(k_sum, dim3(1), dim3(kBlock), 0, ctx->stream, grid, ctx->rk_part, errv);
+      double ss = 0.0;
+      if (hipMemcpyAsync(&ss, errv, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        ctx->err = "hg_tsit5_solve: HIP error";
+        return HG_ERR_HIP;
+      }
+      const double err = std::sqrt(ss / double(n3));
+      if (!std::isfinite(err)) {
+        ctx->err = "hg_tsit5_solve: non-finite error estimate";
+        return HG_ERR_NONFINITE;
+      }
+      const double fac = err > 0.0 ? 0.9 * std::pow(err, -0.2) : 10.0;
+      if (err <= 1.0) {  // accept: the new solution and its derivative (first same as last)
+        t = last ? tt : t + h;
+        std::swap(y, ynew);
+        std::swap(K[0], K[6]);
+        ++accepted;
+        const double hn = h * std::min(10.0, std::max(0.2, fac));
+        if (!last || hn < dt) dt = hn;
+      } else {
+        ++rejected;
+        dt = h * std::max(0.2, fac);
+      }
+      if (dt < 1e-14 * (t1 - t0)) {
+        ctx->err = "hg_tsit5_solve: step size underflow";
+        return HG_ERR_STATE;
+      }
+    }
+    if (!rc) rc = save(si);
+  }
+  if (!rc) rc = check_flags(ctx);
+  if (stats) {
+    stats[0] = accepted;
+    stats[1] = rejected;
+    stats[2] = nrhs;
+  }
+  return rc;
 }
 
 int hg_probe_copy(hg_ctx* ctx, int64_t nbytes, int32_t reps, double* gbps) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
-    "hg_mfma_probe",
+    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau",
 ]
 UDE_WEIGHTS = 353
 
@@ -74,6 +74,9 @@ def lib():
         L.hg_ude_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hg_ude_grad.argtypes = [C.c_void_p, C.c_void_p]
         L.hg_mfma_probe.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_tsit5_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
+                                     C.c_int64, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        L.hg_tsit5_tableau.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         for name in ("hg_destroy",):
             getattr(L, name).argtypes = [C.c_void_p]
         L.hg_set_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 9
@@ -261,6 +264,18 @@ class Context:
                                         C.byref(b)))
         return a.value, b.value
 
+    def tsit5(self, Q0, t0, t1, n_save, abstol=1e-6, reltol=1e-3, dt0=1e-3, params=None):
+        """Adaptive Tsit5 from Q0 over [t0, t1]; returns (n_save + 1, 3N) saved
+        states and (accepted, rejected, rhs evaluations)."""
+        Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        out = np.empty((n_save + 1, 3 * self.N))
+        st = np.zeros(3, dtype=np.int64)
+        self._chk(lib().hg_tsit5_solve(self.h, _ptr(Q0), _ptr(p), 0 if p is None else p.size, C.c_double(t0),
+                                       C.c_double(t1), C.c_int64(n_save), C.c_double(abstol), C.c_double(reltol),
+                                       C.c_double(dt0), _ptr(out), _ptr(st)))
+        return out, tuple(int(v) for v in st)
+
     def set_ude(self, weights, set_of_cell=None, precision=32):
         """Manning n from the MLP closure; weights (n_sets, 353) or (353,);
         weights=None turns it off."""
@@ -306,6 +321,13 @@ def mfma_probe(precision, A, B):
     if rc:
         raise HGError(rc, "hg_mfma_probe failed")
     return D
+
+
+def tsit5_tableau():
+    """(c[7], a[7][6], btilde[7]) of the library's Tsit5."""
+    c, a, bt = np.empty(7), np.empty((7, 6)), np.empty(7)
+    lib().hg_tsit5_tableau(_ptr(c), _ptr(a), _ptr(bt))
+    return c, a, bt
 
 
 def probe_copy(ctx, nbytes, reps=1):

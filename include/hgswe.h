@@ -207,6 +207,23 @@ int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
                    int64_t kck, int64_t n_probe, const int32_t* probe, const double* target, double* loss,
                    double* n_bar, double* zb_bar, double* inletQ_bar);
 
+/* Adaptive Tsit5 (Tsitouras 5(4) pair, first-same-as-last) over the device
+ * RHS: solve(ODEProblem(ode_f, Q0, (t0, t1), p), Tsit5(); abstol, reltol,
+ * saveat) as the reference's examples run it (swe_2D_forward_simulation.jl:
+ * 24-53).  Saves n_save + 1 states at t0 + k (t1 - t0) / n_save (the steps
+ * land on the save times) into Qsave ((n_save + 1) x 3N, caller order).
+ * Error norm: RMS of err_i / (abstol + reltol max(|y_i|, |y_new_i|)), as
+ * OrdinaryDiffEq's default.  dt0 is the first trial step.  stats (optional):
+ * accepted steps, rejected steps, RHS evaluations.  No dry clamp is applied
+ * (that belongs to the custom Euler driver). */
+int hg_tsit5_solve(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double t0, double t1,
+                   int64_t n_save, double abstol, double reltol, double dt0, double* Qsave, int64_t* stats);
+
+/* The Tsit5 coefficients the solver uses: c[7], a[7][6] (row i = stage i+1),
+ * btilde[7] (fifth-order minus embedded fourth-order weights; the fifth-order
+ * weights are row 6 of a).  Host-only, no device needed. */
+int hg_tsit5_tableau(double* c, double* a, double* btilde);
+
 /* ------------------------------------------------------- UDE closure */
 
 /* Manning's n from a neural closure, evaluated on the matrix cores every

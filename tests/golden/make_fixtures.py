@@ -9,6 +9,7 @@ LICENSE_Hydrograd.txt):
   examples/SWE_2D/forward_simulation/Savannah_River/                -> savannah/
   examples/SWE_2D/sensitivity_analysis/ManningN/oneD_channel_with_bump/    -> bump_sens/
   examples/SWE_2D/sensitivity_analysis/ManningN/oneD_channel_uniform_flow/ -> uniform_sens/
+    (sensitivity_results.json -> sens.npz, forward_simulation_results.json -> traj.npz)
 
 The SRH-2D case files are copied verbatim; the JSON truth / sensitivity
 files (written by JSON3 with shortest round-trip Float64 text, so parsing is
@@ -78,6 +79,8 @@ def main():
         copy_case(d, os.path.join(OUT, dst), [base + ".srhgeom", base + ".srhhydro", base + ".srhmat",
                                               "run_control.json"])
         json_to_npz(os.path.join(d, "sensitivity_results.json"), os.path.join(OUT, dst, "sens.npz"))
+        # 101 saved Tsit5 frames of the forward pass (3N per frame, t = 0, 2, ..., T)
+        json_to_npz(os.path.join(d, "forward_simulation_results.json"), os.path.join(OUT, dst, "traj.npz"))
     shutil.copyfile(os.path.join(REF, "LICENSE"), os.path.join(OUT, "LICENSE_Hydrograd.txt"))
 
 
