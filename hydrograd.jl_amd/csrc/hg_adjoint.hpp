@@ -6,6 +6,8 @@
 // derivative, the derivative follows the taken branch, clamped depths and
 // masked discharges are constants, and a branch that copies a mirrored state
 // (swe_2D_solvers.jl:23-51) routes the copy's cotangent back to its source.
+// Derivatives are accurate, not bit-reproductions of an AD tape: repeated
+// divisions by one denominator use its reciprocal.
 #pragma once
 #include "hg_roe_pair.hpp"
 
@@ -27,9 +29,10 @@ __host__ __device__ __forceinline__ void ext_zero(ExtBar& e) { e.xi = e.h = e.hu
 // fold the derived cotangents into (xi, h, hu, hv).
 __host__ __device__ __forceinline__ SideBar ext_to_side(const ExtBar& e, const CellQ& c, double half_g) {
   SideBar s;
-  s.hu = e.hu + e.u / c.h;
-  s.hv = e.hv + e.v / c.h;
-  s.h = e.h - e.u * c.u / c.h - e.v * c.v / c.h + e.s / (2.0 * c.s);
+  const double rh = 1.0 / c.h;
+  s.hu = e.hu + e.u * rh;
+  s.hv = e.hv + e.v * rh;
+  s.h = e.h - (e.u * c.u + e.v * c.v) * rh + e.s * (0.5 / c.s);
   s.xi = e.xi + e.p * half_g * (2.0 * (c.xi + EPS) + 2.0 * c.hs);
   return s;
 }
@@ -86,11 +89,13 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   // ---- forward recompute (general branch)
   const double hRoe = (hA + hB) * 0.5;
   const double sAB = sA + sB;
-  const double uRoe = (sA * uA + sB * uB) / sAB;
-  const double vRoe = (sA * vA + sB * vB) / sAB;
+  const double rsAB = 1.0 / sAB;
+  const double uRoe = (sA * uA + sB * uB) * rsAB;
+  const double vRoe = (sA * vA + sB * vB) * rsAB;
   const double un = uRoe * nx + vRoe * ny;
   const double c = sqrt(g * hRoe + EPS);
-  const double o = 0.5 / c;
+  const double rc = 1.0 / c;
+  const double o = 0.5 * rc;
   const double d0 = B.xi - A.xi, d1 = huB - huA, d2 = hvB - hvA;
   const double k0 = -(uRoe * ny - vRoe * nx);
   const double w0 = k0 * d0 + ny * d1 + (-nx) * d2;
@@ -98,6 +103,7 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   const double w2 = (-un * o + 0.5) * d0 + (nx * o) * d1 + (ny * o) * d2;
   const double um = un - c, up = un + c;
   const double l0 = sqrt(un * un + EPS), l1 = sqrt(um * um + EPS), l2 = sqrt(up * up + EPS);
+  const double q0 = un / l0, q1 = um / l1, q2 = up / l2;  // d l_k / d (its argument)
   const double a1 = l1 * w1, a2 = l2 * w2;
   // ---- reverse: F = ((fA + fB) - r) / 2
   const double hf[3] = {0.5 * Fb[0], 0.5 * Fb[1], 0.5 * Fb[2]};
@@ -117,8 +123,8 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   // a_k = l_k w_k
   const double l0b = ab0 * w0, l1b = ab1 * w1, l2b = ab2 * w2;
   const double w0b = ab0 * l0, w1b = ab1 * l1, w2b = ab2 * l2;
-  double unb = l0b * un / l0 + l1b * um / l1 + l2b * up / l2;
-  cb += -l1b * um / l1 + l2b * up / l2;
+  double unb = l0b * q0 + l1b * q1 + l2b * q2;
+  cb += -l1b * q1 + l2b * q2;
   // w's
   const double k0b = w0b * d0;
   const double d0b = w0b * k0 + w1b * (un * o + 0.5) + w2b * (-un * o + 0.5);
@@ -128,16 +134,17 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   const double ob = w1b * (un * d0 - nx * d1 - ny * d2) + w2b * (-un * d0 + nx * d1 + ny * d2);
   uRb += -k0b * ny;
   vRb += k0b * nx;
-  cb += -ob * o / c;
-  const double hRb = cb * g / (2.0 * c);
+  cb += -ob * o * rc;
+  const double hRb = cb * g * (0.5 * rc);
   uRb += unb * nx;
   vRb += unb * ny;
-  ma.s += (uRb * (uA - uRoe) + vRb * (vA - vRoe)) / sAB;
-  mb.s += (uRb * (uB - uRoe) + vRb * (vB - vRoe)) / sAB;
-  ma.u += uRb * sA / sAB;
-  ma.v += vRb * sA / sAB;
-  mb.u += uRb * sB / sAB;
-  mb.v += vRb * sB / sAB;
+  ma.s += (uRb * (uA - uRoe) + vRb * (vA - vRoe)) * rsAB;
+  mb.s += (uRb * (uB - uRoe) + vRb * (vB - vRoe)) * rsAB;
+  const double wa = sA * rsAB, wb = sB * rsAB;
+  ma.u += uRb * wa;
+  ma.v += vRb * wa;
+  mb.u += uRb * wb;
+  mb.v += vRb * wb;
   ma.h += 0.5 * hRb;
   mb.h += 0.5 * hRb;
   // d = B - A (xi of both sides is never mirrored)
@@ -180,13 +187,15 @@ __host__ __device__ __forceinline__ void source_vjp(double g, double kn2, double
   const double fxb = -y1, fyb = -y2;
   const double x = h + hsmall;
   const double p73 = pow_7_3(x);
-  const double cc = g * (n * n) / kn2 / p73;
+  const double rp = 1.0 / (kn2 * p73);
+  const double cc = g * (n * n) * rp;
   const double s = sqrt(qx * qx + qy * qy + EPS);
+  const double rs = 1.0 / s;
   const double cb = fxb * s * qx + fyb * s * qy;
   const double sb = fxb * cc * qx + fyb * cc * qy;
-  qxb += fxb * cc * s + sb * qx / s;
-  qyb += fyb * cc * s + sb * qy / s;
-  nb += cb * 2.0 * g * n / kn2 / p73;
+  qxb += fxb * cc * s + sb * qx * rs;
+  qyb += fyb * cc * s + sb * qy * rs;
+  nb += cb * 2.0 * g * n * rp;
   hb += cb * (-7.0 / 3.0) * cc / x;
 }
 

@@ -110,6 +110,7 @@ struct TileArgs {
   const int32_t* in_cell;
   const double* in_L53;
   const double* totalA;  // used when n_inlet == 0 (conveyance from the separate kernel)
+  double* totalA_out;    // optional record of the fused conveyances (per inlet)
   int32_t* flag;
   uint8_t* clamp;  // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
@@ -221,6 +222,8 @@ struct hg_ctx {
   double* adj_seg = nullptr;    // (kck + 1) x 3N states of one checkpoint segment
   uint8_t* adj_mask = nullptr;  // kck x N clamp flags
   double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
+  double* adj_slot = nullptr;   // 6 x maxdeg x N per-slot cotangents (ELL layout)
+  double* adj_totalA = nullptr; // kck x n_inlet conveyances recorded by a segment's recompute
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
 
   // UDE closure for n (0: off; else the precision of its network, 32 or 64)

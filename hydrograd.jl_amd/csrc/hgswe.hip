@@ -298,6 +298,7 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
       }
       if (li == 0) {
         tA[sl - s0] = acc;
+        if (a.totalA_out) a.totalA_out[b] = acc;  // every tile of inlet b writes the same value
         if (!(acc > 1e-10)) a.flag[0] = 1;
       }
     }
@@ -498,139 +499,193 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(int64_t n2, const double2
 //   * the source term (bed slope, Manning friction) is the cell's own.
 // Derivatives with respect to n, the exit-H bed term, S0 and the inlet
 // ghost velocity are accumulated / written for the follow-up kernels.
+// Reverse mode of one RHS evaluation (or one step), in two passes so that
+// the face VJPs, the bulk of the work, run one work-item per (cell, face)
+// slot.  k_adj_slot: for slot (i, k), the VJP of the flux through that face
+// seen from cell i, with the combined cotangent of both cells (the face's
+// contribution to cell i's raw inputs, by the mirror identity of the
+// pairwise fluxes), routed through the ghost rule on boundary faces; it
+// writes 6 values per slot (cotangents of xi, h, hu, hv, n, zb).
+// k_adj_cell: per cell, the slots summed in local face order, the source
+// VJP, the clamp and the identity part of the step.
+struct CellAdj {  // one cell's quantities the adjoint passes need
+  hg::CellQ me;
+  bool wet;
+  double xi, zb, n, A, lt0, lt1, lt2, y0, y1, y2;
+};
+
 template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_swe_adj(hg::AdjArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
+__device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
+  CellAdj c;
+  c.xi = a.xi[i];
+  c.zb = a.zb[i];
+  c.n = a.n[i];
+  c.A = a.area[i];
+  c.me = hg::make_cellq(c.xi, a.hstill[i], a.qx[i], a.qy[i], c.zb, a.half_g, a.hsmall);
+  c.wet = c.me.h > a.hsmall;
+  c.lt0 = a.lx[i];
+  c.lt1 = a.lqx[i];
+  c.lt2 = a.lqy[i];
+  if (STEP && a.mask && a.mask[i]) c.lt0 = c.lt1 = c.lt2 = 0.0;
+  c.y0 = STEP ? a.dt * c.lt0 : c.lt0;
+  c.y1 = STEP ? a.dt * c.lt1 : c.lt1;
+  c.y2 = STEP ? a.dt * c.lt2 : c.lt2;
+  return c;
+}
+
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) {
+  const int64_t E = int64_t(a.maxdeg) * a.N;
+  const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (s >= E) return;
+  const int i = int(s % a.N), k = int(s / a.N);
+  if (k >= a.deg[i]) return;
   const double hsmall = a.hsmall;
-  const double xi = a.xi[i], hs = a.hstill[i];
-  const double zb_i = a.zb[i], n_i = a.n[i], A_i = a.area[i];
-  const hg::CellQ me = hg::make_cellq(xi, hs, a.qx[i], a.qy[i], zb_i, a.half_g, hsmall);
-  const bool wet = me.h > hsmall;
-  double lt0 = a.lx[i], lt1 = a.lqx[i], lt2 = a.lqy[i];
-  if (STEP && a.mask && a.mask[i]) lt0 = lt1 = lt2 = 0.0;
-  const double y0 = STEP ? a.dt * lt0 : lt0, y1 = STEP ? a.dt * lt1 : lt1, y2 = STEP ? a.dt * lt2 : lt2;
+  const CellAdj ca = cell_adj<STEP>(a, i);
+  const hg::CellQ& me = ca.me;
+  const int j = a.nbr[s];
+  const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
+  const double fi = -(L / ca.A);
   hg::SideBar U{0.0, 0.0, 0.0, 0.0};
   double nb = 0.0, zbb = 0.0;
-  const int d = a.deg[i];
-  for (int k = 0; k < d; ++k) {
-    const int s = k * a.N + i;
-    const int j = a.nbr[s];
-    const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
-    const double fi = -(L / A_i);
-    if (j >= 0) {
-      const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
-      double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
-      if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
-      const double yj0 = STEP ? a.dt * m0 : m0, yj1 = STEP ? a.dt * m1 : m1, yj2 = STEP ? a.dt * m2 : m2;
-      const double fj = L / a.area[j];
-      const double c[3] = {fi * y0 + fj * yj0, fi * y1 + fj * yj1, fi * y2 + fj * yj2};
-      hg::SideBar Ab, Bb;
-      hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
-      U.xi += Ab.xi; U.h += Ab.h; U.hu += Ab.hu; U.hv += Ab.hv;
-    } else {
-      const int gi = ~j;
-      const int kind = a.g_kind[gi];
-      double hgh, qxg, qyg, v = 0.0, w = 0.0;
-      switch (kind) {
-        case HG_BC_INLETQ: {
-          const int b = a.g_bc[gi];
-          w = wet ? 1.0 : 0.0;
-          v = a.inletQ[b] / a.totalA[b] * a.g_L23[gi] / n_i;
-          hgh = me.h;
-          qxg = -me.h * v * nx * w;
-          qyg = -me.h * v * ny * w;
-          break;
-        }
-        case HG_BC_EXITH: {
-          const double hx = a.exitWSE[a.g_bc[gi]] - zb_i;
-          hgh = (hx > hsmall) ? hx : hsmall;
-          qxg = me.hu;
-          qyg = me.hv;
-          break;
-        }
-        case HG_BC_WALL:
-          hgh = me.h;
-          qxg = -me.hu;
-          qyg = -me.hv;
-          break;
-        default: {
-          const double vdn = me.hu * nx + me.hv * ny;
-          hgh = me.h;
-          qxg = me.hu - 2.0 * vdn * nx;
-          qyg = me.hv - 2.0 * vdn * ny;
-          break;
-        }
+  if (j >= 0) {
+    const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
+    double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
+    if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
+    const double yj0 = STEP ? a.dt * m0 : m0, yj1 = STEP ? a.dt * m1 : m1, yj2 = STEP ? a.dt * m2 : m2;
+    const double fj = L / a.area[j];
+    const double c[3] = {fi * ca.y0 + fj * yj0, fi * ca.y1 + fj * yj1, fi * ca.y2 + fj * yj2};
+    hg::SideBar Ab, Bb;
+    hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
+    U = Ab;
+  } else {
+    const int gi = ~j;
+    const int kind = a.g_kind[gi];
+    double hgh, qxg, qyg, v = 0.0, w = 0.0;
+    switch (kind) {
+      case HG_BC_INLETQ: {
+        const int b = a.g_bc[gi];
+        w = ca.wet ? 1.0 : 0.0;
+        v = a.inletQ[b] / a.totalA[b] * a.g_L23[gi] / ca.n;
+        hgh = me.h;
+        qxg = -me.h * v * nx * w;
+        qyg = -me.h * v * ny * w;
+        break;
       }
-      hg::CellQ B;
-      B.hs = a.g_hstill[gi];
-      B.xi = hgh - B.hs;
-      B.h = hgh;
-      B.hu = qxg;
-      B.hv = qyg;
-      B.zb = a.g_zb[gi];
-      B.u = qxg / hgh;
-      B.v = qyg / hgh;
-      B.s = sqrt(hgh + hg::EPS);
-      B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
-      const double c[3] = {fi * y0, fi * y1, fi * y2};
-      hg::SideBar Ab, Bb;
-      hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
-      U.xi += Ab.xi; U.h += Ab.h; U.hu += Ab.hu; U.hv += Ab.hv;
-      const double hgb = Bb.h + Bb.xi;  // xi_g = h_g - hstill_g
-      switch (kind) {
-        case HG_BC_INLETQ: {  // h_g = h, q_g = -h v n w, v = (Q / A) L^(2/3) / n
-          U.h += hgb + Bb.hu * (-v * nx * w) + Bb.hv * (-v * ny * w);
-          const double vb = Bb.hu * (-me.h * nx * w) + Bb.hv * (-me.h * ny * w);
-          nb += vb * (-v / n_i);
-          a.vbar[gi] = vb;
-          break;
-        }
-        case HG_BC_EXITH: {  // h_g = max(h_small, WSE - zb_cell), q_g = q
-          if (a.exitWSE[a.g_bc[gi]] - zb_i > hsmall) zbb -= hgb;
-          U.hu += Bb.hu;
-          U.hv += Bb.hv;
-          break;
-        }
-        case HG_BC_WALL:
-          U.h += hgb;
-          U.hu -= Bb.hu;
-          U.hv -= Bb.hv;
-          break;
-        default: {
-          const double vdnb = -2.0 * (Bb.hu * nx + Bb.hv * ny);
-          U.h += hgb;
-          U.hu += Bb.hu + vdnb * nx;
-          U.hv += Bb.hv + vdnb * ny;
-          break;
-        }
+      case HG_BC_EXITH: {
+        const double hx = a.exitWSE[a.g_bc[gi]] - ca.zb;
+        hgh = (hx > hsmall) ? hx : hsmall;
+        qxg = me.hu;
+        qyg = me.hv;
+        break;
+      }
+      case HG_BC_WALL:
+        hgh = me.h;
+        qxg = -me.hu;
+        qyg = -me.hv;
+        break;
+      default: {
+        const double vdn = me.hu * nx + me.hv * ny;
+        hgh = me.h;
+        qxg = me.hu - 2.0 * vdn * nx;
+        qyg = me.hv - 2.0 * vdn * ny;
+        break;
+      }
+    }
+    hg::CellQ B;
+    B.hs = a.g_hstill[gi];
+    B.xi = hgh - B.hs;
+    B.h = hgh;
+    B.hu = qxg;
+    B.hv = qyg;
+    B.zb = a.g_zb[gi];
+    B.u = qxg / hgh;
+    B.v = qyg / hgh;
+    B.s = sqrt(hgh + hg::EPS);
+    B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
+    const double c[3] = {fi * ca.y0, fi * ca.y1, fi * ca.y2};
+    hg::SideBar Ab, Bb;
+    hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
+    U = Ab;
+    const double hgb = Bb.h + Bb.xi;  // xi_g = h_g - hstill_g
+    switch (kind) {
+      case HG_BC_INLETQ: {  // h_g = h, q_g = -h v n w, v = (Q / A) L^(2/3) / n
+        U.h += hgb + Bb.hu * (-v * nx * w) + Bb.hv * (-v * ny * w);
+        const double vb = Bb.hu * (-me.h * nx * w) + Bb.hv * (-me.h * ny * w);
+        nb += vb * (-v / ca.n);
+        a.vbar[gi] = vb;
+        break;
+      }
+      case HG_BC_EXITH: {  // h_g = max(h_small, WSE - zb_cell), q_g = q
+        if (a.exitWSE[a.g_bc[gi]] - ca.zb > hsmall) zbb -= hgb;
+        U.hu += Bb.hu;
+        U.hv += Bb.hv;
+        break;
+      }
+      case HG_BC_WALL:
+        U.h += hgb;
+        U.hu -= Bb.hu;
+        U.hv -= Bb.hv;
+        break;
+      default: {
+        const double vdnb = -2.0 * (Bb.hu * nx + Bb.hv * ny);
+        U.h += hgb;
+        U.hu += Bb.hu + vdnb * nx;
+        U.hv += Bb.hv + vdnb * ny;
+        break;
       }
     }
   }
-  double xib = U.xi, hb = U.h, qxb = U.hu, qyb = U.hv, S0xb = 0.0, S0yb = 0.0;
-  if (wet)
-    hg::source_vjp(a.g, a.kn2, hsmall, xi, me.h, me.hu, me.hv, n_i, a.S0x[i], a.S0y[i], y1, y2, xib, hb, qxb, qyb,
-                   nb, S0xb, S0yb);
+  sb[0 * E + s] = U.xi;
+  sb[1 * E + s] = U.h;
+  sb[2 * E + s] = U.hu;
+  sb[3 * E + s] = U.hv;
+  sb[4 * E + s] = nb;
+  sb[5 * E + s] = zbb;
+}
+
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a, const double* sb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const int64_t E = int64_t(a.maxdeg) * a.N;
+  const CellAdj ca = cell_adj<STEP>(a, i);
+  const hg::CellQ& me = ca.me;
+  double xib = 0.0, hb = 0.0, qxb = 0.0, qyb = 0.0, nb = 0.0, zbb = 0.0;
+  const int d = a.deg[i];
+  for (int k = 0; k < d; ++k) {  // local face order
+    const int64_t s = int64_t(k) * a.N + i;
+    xib += sb[0 * E + s];
+    hb += sb[1 * E + s];
+    qxb += sb[2 * E + s];
+    qyb += sb[3 * E + s];
+    nb += sb[4 * E + s];
+    zbb += sb[5 * E + s];
+  }
+  double S0xb = 0.0, S0yb = 0.0;
+  if (ca.wet)
+    hg::source_vjp(a.g, a.kn2, a.hsmall, ca.xi, me.h, me.hu, me.hv, ca.n, a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, hb,
+                   qxb, qyb, nb, S0xb, S0yb);
   // clamped depth / masked discharge are constants (semi_discretize_swe_2D.jl:101-106)
   double ox = xib, oqx = 0.0, oqy = 0.0;
-  if (wet) {
+  if (ca.wet) {
     ox += hb;
     oqx = qxb;
     oqy = qyb;
   }
   if (STEP) {
-    ox += lt0;
-    oqx += lt1;
-    oqy += lt2;
+    ox += ca.lt0;
+    oqx += ca.lt1;
+    oqy += ca.lt2;
   }
   a.ox[i] = ox;
   a.oqx[i] = oqx;
   a.oqy[i] = oqy;
   if (a.nbar) a.nbar[i] += nb;
   if (a.zbbar) a.zbbar[i] += zbb;
-  if (a.S0bar) {
-    a.S0bar[i] = S0xb;
-    a.S0bar[a.N + i] = S0yb;
+  if (a.S0bar) {  // summed over the steps; the bed-slope transpose is applied once at the end
+    a.S0bar[i] += S0xb;
+    a.S0bar[a.N + i] += S0yb;
   }
 }
 
@@ -1395,23 +1450,30 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
   return HG_OK;
 }
 
-// One step (or RHS) from explicit state buffers; optional clamp record.
+// One step (or RHS) from explicit state buffers; optional clamp record and
+// record of the inlet conveyances of this evaluation (totalA_out).
 int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt, bool step,
-                   uint8_t* clamp = nullptr) {
+                   uint8_t* clamp = nullptr, double* totalA_out = nullptr) {
   if (ctx->ude_prec) {
     int rc = enqueue_ude_n(ctx, in);
     if (rc) return rc;
   }
+  hg::InletArgs ia = inlet_args_p(ctx, in[0]);
+  if (totalA_out) ia.totalA = totalA_out;
   if (ctx->tiled) {
     if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
-      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream,
-                         inlet_args_p(ctx, in[0]));
-    launch_tile(ctx, make_tile_args_p(ctx, in, out, dt, clamp), step);
+      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, ia);
+    hg::TileArgs a = make_tile_args_p(ctx, in, out, dt, clamp);
+    if (ctx->fuse_inlet)
+      a.totalA_out = totalA_out;
+    else if (totalA_out)
+      a.totalA = totalA_out;
+    launch_tile(ctx, a, step);
   } else {
     if (ctx->n_inlet > 0)
-      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream,
-                         inlet_args_p(ctx, in[0]));
+      hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, ia);
     StepArgs a = make_args_p(ctx, in, out, dt, clamp);
+    if (totalA_out) a.totalA = totalA_out;
     if (step)
       hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
     else
@@ -1435,11 +1497,13 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (!rc) rc = dalloc(ctx, &ctx->adj_S0bar, 2 * N);
     if (!rc) rc = dalloc(ctx, &ctx->adj_vbar, std::max<int64_t>(ctx->G, 1));
     if (!rc) rc = dalloc(ctx, &ctx->adj_Qbar, std::max<int32_t>(ctx->n_inlet, 1));
+    if (!rc) rc = dalloc(ctx, &ctx->adj_slot, size_t(6) * ctx->maxdeg * N);
     if (rc) return rc;
   }
   if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
     rc = dalloc(ctx, &ctx->adj_seg, size_t(kck + 1) * 3 * N);
     if (!rc) rc = dalloc(ctx, &ctx->adj_mask, size_t(kck) * N);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_totalA, size_t(kck) * std::max<int32_t>(ctx->n_inlet, 1));
     if (rc) return rc;
     ctx->adj_seg_cap = size_t(kck);
   }
@@ -1493,9 +1557,11 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
 
 // Cotangent of one RHS evaluation (step = false) or one step (step = true)
 // at state q, given the output cotangent lam; writes the state cotangent to
-// out and accumulates into adj_nbar / adj_zbbar / adj_Qbar.
+// out and accumulates into adj_nbar / adj_zbbar / adj_Qbar / adj_S0bar.
+// totalA: the inlet conveyances of q when the forward pass recorded them
+// (else they are recomputed here).
 int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
-                    double* const out[3], double dt, bool step) {
+                    double* const out[3], double dt, bool step, const double* totalA = nullptr) {
   const bool ude = ctx->ude_prec != 0;
   double* nbar = ude ? ctx->ude_nbar : ctx->adj_nbar;  // with the closure, n-bar of this step only
   if (ude && ctx->ude_nsets != 1) {
@@ -1507,20 +1573,27 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     if (rc) return rc;
     HG_TRY(hipMemsetAsync(ctx->ude_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   }
-  if (ctx->n_inlet > 0)
+  if (ctx->n_inlet > 0 && !totalA) {
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
+    totalA = ctx->totalA;
+  }
   hg::AdjArgs a = make_adj_args(ctx, q, lam, mask, out, dt);
   a.nbar = nbar;
-  if (step)
-    hipLaunchKernelGGL(k_swe_adj<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
-  else
-    hipLaunchKernelGGL(k_swe_adj<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  if (totalA) a.totalA = totalA;
+  const int64_t E = int64_t(ctx->maxdeg) * ctx->N;
+  const int gs = int((E + kBlock - 1) / kBlock);
+  if (step) {
+    hipLaunchKernelGGL(k_adj_slot<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+    hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+  } else {
+    hipLaunchKernelGGL(k_adj_slot<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+    hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+  }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
-                       ctx->in_ghost, ctx->in_cell, ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
-                       ctx->totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar, ctx->adj_Qbar);
-  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
-                     ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+                       ctx->in_ghost, ctx->in_cell, ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0],
+                       ctx->hstill, ctx->n, ctx->inletQ, a.totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar,
+                       ctx->adj_Qbar);
   if (ude) {  // through the closure: n(Q) adds to the state cotangent, and the weight cotangent accumulates
     hg::UdeArgs u = ude_args(ctx, q);
     u.nbar = ctx->ude_nbar;
@@ -1540,8 +1613,18 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   return HG_OK;
 }
 
+// After the last reverse step: the bed-slope transpose of the summed S0
+// cotangent (the map zb -> S0 does not change over the steps).
+int finish_adjoint(hg_ctx* ctx) {
+  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
+                     ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
 int zero_adj_accumulators(hg_ctx* ctx) {
   if (ctx->ude_prec) HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
+  HG_TRY(hipMemsetAsync(ctx->adj_S0bar, 0, 2 * ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_Qbar, 0, std::max<int32_t>(ctx->n_inlet, 1) * sizeof(double), ctx->stream));
@@ -1559,6 +1642,11 @@ int cell_out(hg_ctx* ctx, const double* dsrc, double* hdst) {
 }
 
 double* seg_state(hg_ctx* ctx, int64_t t, int c) { return ctx->adj_seg + (t * 3 + c) * ctx->N; }
+
+// Inlet conveyances of step t of the segment being reversed (nullptr without inlets).
+double* seg_totalA(hg_ctx* ctx, int64_t t) {
+  return ctx->n_inlet > 0 ? ctx->adj_totalA + t * ctx->n_inlet : nullptr;
+}
 
 // Forward nsteps from the device state, storing a checkpoint every kck steps.
 int forward_checkpointed(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck) {
@@ -1587,7 +1675,7 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
     for (int64_t t = 0; t < ns; ++t) {  // recompute the segment, recording the clamp of every step
       const double* in[3] = {seg_state(ctx, t, 0), seg_state(ctx, t, 1), seg_state(ctx, t, 2)};
       double* out[3] = {seg_state(ctx, t + 1, 0), seg_state(ctx, t + 1, 1), seg_state(ctx, t + 1, 2)};
-      int rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask + t * N);
+      int rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask + t * N, seg_totalA(ctx, t));
       if (rc) return rc;
     }
     for (int64_t t = ns - 1; t >= 0; --t) {
@@ -1596,7 +1684,7 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
       double* O = ctx->adj_lam[1 - lam_cur];
       const double* lam[3] = {L, L + N, L + 2 * N};
       double* out[3] = {O, O + N, O + 2 * N};
-      int rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask + t * N, out, dt, true);
+      int rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask + t * N, out, dt, true, seg_totalA(ctx, t));
       if (rc) return rc;
       lam_cur = 1 - lam_cur;
     }
@@ -2844,6 +2932,7 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
   const double* lam[3] = {ctx->adj_lam[0], ctx->adj_lam[0] + N, ctx->adj_lam[0] + 2 * N};
   double* out[3] = {ctx->adj_lam[1], ctx->adj_lam[1] + N, ctx->adj_lam[1] + 2 * N};
   rc = enqueue_adjoint(ctx, ctx->st[0], lam, nullptr, out, 1.0, false);
+  if (!rc) rc = finish_adjoint(ctx);
   if (!rc) rc = adj_download(ctx, ctx->adj_lam[1], Q_bar, n_bar, zb_bar, inletQ_bar);
   if (!rc) rc = check_flags(ctx);
   return rc;
@@ -2873,6 +2962,7 @@ int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_pa
   if (!rc) rc = zero_adj_accumulators(ctx);
   int cur = 0;
   if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
+  if (!rc) rc = finish_adjoint(ctx);
   if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], Q0_bar, n_bar, zb_bar, inletQ_bar);
   if (!rc) rc = check_flags(ctx);
   return rc;
@@ -2910,6 +3000,7 @@ int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
   if (!rc) rc = zero_adj_accumulators(ctx);
   int cur = 0;
   if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
+  if (!rc) rc = finish_adjoint(ctx);
   if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], nullptr, n_bar, zb_bar, inletQ_bar);
   if (!rc) rc = check_flags(ctx);
   if (!rc && zb_bar)  // the loss reads zb directly
