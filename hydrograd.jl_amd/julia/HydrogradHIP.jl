@@ -21,7 +21,7 @@ module HydrogradHIP
 
 using ChainRulesCore
 
-export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f
+export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f, tsit5_solve, set_ude!, ude_grad
 
 const libhgswe = joinpath(@__DIR__, "..", "lib", "libhgswe.so")
 
@@ -203,6 +203,48 @@ function custom_ODE_solve(ode_f::DeviceRHS, Q0, params_vector, swe2d_extra_param
         check(ctx.ptr, ccall((:hg_state_download, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, view(sol, :, k)))
     end
     return sol
+end
+
+# Adaptive Tsit5 on the device, for solve(ODEProblem(ode_f, Q0, tspan, p),
+# Tsit5(); saveat = range(tspan..., length = nsave + 1)) as
+# swe_2D_forward_simulation.jl:24-53 runs it.  Returns a 3N x (nsave + 1)
+# matrix (one column per saved time, like Array(sol)).
+function tsit5_solve(ode_f::DeviceRHS, Q0, params_vector, tspan, nsave; abstol = 1e-6, reltol = 1e-3, dt0 = 1e-3)
+    nodual(Q0); nodual(params_vector)
+    ctx = context(ode_f.p)
+    prm = Vector{Float64}(params_vector)
+    sol = Matrix{Float64}(undef, length(Q0), nsave + 1)
+    stats = zeros(Int64, 3)
+    check(ctx.ptr, ccall((:hg_tsit5_solve, libhgswe), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Int64, Float64, Float64,
+                          Float64, Ptr{Float64}, Ptr{Int64}),
+                         ctx.ptr, Vector{Float64}(Q0), prm, length(prm), Float64(tspan[1]), Float64(tspan[2]), nsave,
+                         abstol, reltol, dt0, sol, stats))
+    return sol
+end
+
+# UDE closure for Manning's n on the device (3 -> 16 -> 16 -> 1 tanh network,
+# 353 weights; the layout is documented in include/hgswe.h).  weights = nothing
+# turns it off.
+function set_ude!(p_extra, weights; precision = 32)
+    ctx = context(p_extra)
+    if weights === nothing
+        check(ctx.ptr, ccall((:hg_set_ude, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Int32}, Int32),
+                             ctx.ptr, 0, C_NULL, C_NULL, 32))
+    else
+        w = Vector{Float64}(weights)
+        check(ctx.ptr, ccall((:hg_set_ude, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Int32}, Int32),
+                             ctx.ptr, Int32(length(w) ÷ 353), w, C_NULL, Int32(precision)))
+    end
+    return nothing
+end
+
+# Cotangent of the closure weights from the last reverse pass (hg_ude_grad).
+function ude_grad(p_extra)
+    ctx = context(p_extra)
+    g = zeros(353)
+    check(ctx.ptr, ccall((:hg_ude_grad, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, g))
+    return g
 end
 
 # ---------------------------------------------------------------- reverse mode
