@@ -236,6 +236,9 @@ struct hg_ctx {
   double* ude_wbar = nullptr;       // accumulated weight cotangent (353)
   std::vector<void*> ude_allocs;
 
+  // forward-mode (JVP) tangents: state (2 x 3N) and parameter fields
+  double* jvp = nullptr;
+
   // Tsit5 work buffers: 7 stages + y, ynew, ytmp (3N each), norm partials
   double* rk = nullptr;
   double* rk_part = nullptr;

@@ -30,6 +30,7 @@
 #include "hg_roe_pair.hpp"
 #include "hg_adjoint.hpp"
 #include "hg_ude.hpp"
+#include "hg_dual.hpp"  // last: its sqrt / cbrt overloads must stay out of the double paths above
 #include "hgswe.h"
 
 using hg::Side;
@@ -1145,6 +1146,152 @@ __global__ __launch_bounds__(kBlock) void k_sum(int n, const double* part, doubl
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// ---- forward mode (JVP) of the Euler step: the per-cell step kernel on
+// dual numbers (value, one tangent), the device counterpart of
+// ForwardDiff.Dual through swe_2d_rhs and custom_ODE_update_cells.
+struct JvpArgs {
+  StepArgs a;  // values (fields after the active parameter is applied)
+  const double* dxi;
+  const double* dqx;
+  const double* dqy;
+  double* dxo;
+  double* dqxo;
+  double* dqyo;
+  const double* dzb;      // tangents of the parameter fields (nullable = 0)
+  const double* dn;
+  const double* dS0x;
+  const double* dS0y;
+  const double* dg_zb;
+  const double* dinletQ;
+  const double* dtotalA;  // tangent of the inlet conveyances of this step
+};
+
+__device__ __forceinline__ hg::Dual1 ldd(const double* v, const double* d, int i) {
+  return hg::Dual1(v[i], d ? d[i] : 0.0);
+}
+
+// Inlet conveyance on dual numbers, sequential fold in NodeString order.
+__global__ void k_inlet_conveyance_jvp(hg::InletArgs a, const double* dxi, const double* dn, double* dtotalA) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n_inlet) return;
+  using hg::Dual1;
+  Dual1 acc(0.0);
+  for (int f = a.off[b]; f < a.off[b + 1]; ++f) {
+    const int c = a.cell[f];
+    Dual1 h = ldd(a.xi, dxi, c) + a.hstill[c];
+    if (h <= a.hsmall) h = Dual1(a.hsmall);
+    const double w = (h > a.hsmall) ? 1.0 : 0.0;
+    const Dual1 term = a.L53[f] * h / ldd(a.n, dn, c) * w;
+    acc = (f == a.off[b]) ? term : acc + term;
+  }
+  a.totalA[b] = acc.v;
+  dtotalA[b] = acc.d;
+  if (!(acc.v > 1e-10)) a.flag[0] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_swe_step_jvp(JvpArgs J) {
+  using hg::Dual1;
+  const StepArgs& a = J.a;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const double hsmall = a.hsmall;
+  const Dual1 xi = ldd(a.xi, J.dxi, i);
+  const double hs = a.hstill[i];
+  const Dual1 qx0 = ldd(a.qx, J.dqx, i), qy0 = ldd(a.qy, J.dqy, i);
+  Dual1 h = xi + hs, qx = qx0, qy = qy0;  // semi_discretize_swe_2D.jl:101-106
+  if (h <= hsmall) {
+    h = Dual1(hsmall);
+    qx = Dual1(0.0);
+    qy = Dual1(0.0);
+  }
+  const Dual1 zb = ldd(a.zb, J.dzb, i);
+  const Dual1 nman = ldd(a.n, J.dn, i);
+  const hg::SideT<Dual1> Ls{xi, Dual1(hs), h, qx, qy, zb};
+  Dual1 acc0(0.0), acc1(0.0), acc2(0.0);
+  const int d = a.deg[i];
+  for (int k = 0; k < d; ++k) {
+    const int s = k * a.N + i;
+    const int j = a.nbr[s];
+    const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
+    hg::SideT<Dual1> Rs;
+    if (j >= 0) {
+      const Dual1 xj = ldd(a.xi, J.dxi, j);
+      const double hsj = a.hstill[j];
+      Dual1 hj = xj + hsj, qxj = ldd(a.qx, J.dqx, j), qyj = ldd(a.qy, J.dqy, j);
+      if (hj <= hsmall) {
+        hj = Dual1(hsmall);
+        qxj = Dual1(0.0);
+        qyj = Dual1(0.0);
+      }
+      Rs = hg::SideT<Dual1>{xj, Dual1(hsj), hj, qxj, qyj, ldd(a.zb, J.dzb, j)};
+    } else {  // ghost (bc_2D.jl:640-834), internal cell = this cell
+      const int gi = ~j;
+      Dual1 hgh, qxg, qyg;
+      switch (a.g_kind[gi]) {
+        case HG_BC_INLETQ: {
+          const int b = a.g_bc[gi];
+          const double w = (h > hsmall) ? 1.0 : 0.0;
+          const Dual1 v = ldd(a.inletQ, J.dinletQ, b) / ldd(a.totalA, J.dtotalA, b) * a.g_L23[gi] / nman;
+          hgh = h;
+          qxg = -h * v * nx * w;
+          qyg = -h * v * ny * w;
+          break;
+        }
+        case HG_BC_EXITH: {
+          const Dual1 hx = a.exitWSE[a.g_bc[gi]] - zb;
+          hgh = (hx > hsmall) ? hx : Dual1(hsmall);
+          qxg = qx;
+          qyg = qy;
+          break;
+        }
+        case HG_BC_WALL:
+          hgh = h;
+          qxg = -qx;
+          qyg = -qy;
+          break;
+        default: {  // HG_BC_SYMM
+          const Dual1 vdn = qx * nx + qy * ny;
+          hgh = h;
+          qxg = qx - 2.0 * vdn * nx;
+          qyg = qy - 2.0 * vdn * ny;
+          break;
+        }
+      }
+      const double hsg = a.g_hstill[gi];
+      Rs = hg::SideT<Dual1>{hgh - hsg, Dual1(hsg), hgh, qxg, qyg, ldd(a.g_zb, J.dg_zb, gi)};
+    }
+    Dual1 f0, f1, f2;
+    hg::roe_flux(Ls, Rs, nx, ny, a.g, a.half_g, hsmall, f0, f1, f2);
+    acc0 = acc0 + f0 * L;
+    acc1 = acc1 + f1 * L;
+    acc2 = acc2 + f2 * L;
+  }
+  const double A = a.area[i];
+  const Dual1 r0 = -acc0 / A, r1 = -acc1 / A, r2 = -acc2 / A;
+  Dual1 s1(0.0), s2(0.0);
+  if (h > hsmall) {  // :463-475
+    Dual1 fx, fy;
+    hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
+    s1 = a.g * xi * ldd(a.S0x, J.dS0x, i) - fx;
+    s2 = a.g * xi * ldd(a.S0y, J.dS0y, i) - fy;
+  }
+  const Dual1 d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;
+  Dual1 xn = xi + a.dt * d0;  // custom_ODE_solvers.jl:16-26
+  Dual1 qxn = qx0 + a.dt * d1;
+  Dual1 qyn = qy0 + a.dt * d2;
+  if (xn < hsmall) {
+    xn = Dual1(hsmall);
+    qxn = Dual1(0.0);
+    qyn = Dual1(0.0);
+  }
+  a.xo[i] = xn.v;
+  a.qxo[i] = qxn.v;
+  a.qyo[i] = qyn.v;
+  J.dxo[i] = xn.d;
+  J.dqxo[i] = qxn.d;
+  J.dqyo[i] = qyn.d;
 }
 
 // Work items of the tiled kernel (see TileArgs): per tile, cross-tile faces
@@ -2584,6 +2731,117 @@ int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D
   (void)hipFree(dB);
   (void)hipFree(dD);
   return e == hipSuccess ? HG_OK : HG_ERR_HIP;
+}
+
+int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* params, int64_t n_params,
+           const double* dparams, double dt, int64_t nsteps, double* QN, double* dQN) {
+  if (!ctx || !Q0 || !QN || !dQN) return ctx ? (ctx->err = "hg_jvp: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (nsteps < 0) {
+    ctx->err = "hg_jvp: nsteps must be >= 0";
+    return HG_ERR_ARG;
+  }
+  if (ctx->ude_prec) {
+    ctx->err = "hg_jvp: forward mode through the UDE closure is not provided";
+    return HG_ERR_STATE;
+  }
+  const int64_t N = ctx->N, G = ctx->G;
+  int rc = hg_set_params(ctx, params, n_params);
+  if (rc) return rc;
+  if (!ctx->jvp) {  // tangents: state (2 x 3N), dn, dzb, dS0x, dS0y (N each), dg_zb (G), dinletQ, dtotalA
+    const size_t nI = size_t(std::max<int32_t>(ctx->n_inlet, 1));
+    rc = dalloc(ctx, &ctx->jvp, size_t(10) * N + size_t(std::max<int64_t>(G, 1)) + 2 * nI);
+    if (rc) return rc;
+  }
+  double* T0 = ctx->jvp;
+  double* T1 = ctx->jvp + 3 * N;
+  double* dn = ctx->jvp + 6 * N;
+  double* dzb = dn + N;
+  double* dS0x = dzb + N;
+  double* dS0y = dS0x + N;
+  double* dgzb = dS0y + N;
+  double* dQin = dgzb + std::max<int64_t>(G, 1);
+  double* dtot = dQin + std::max<int32_t>(ctx->n_inlet, 1);
+  // tangents of the parameter fields (semi_discretize_swe_2D.jl:114-199)
+  const double* pn = nullptr;
+  const double* pzb = nullptr;
+  const double* pQ = nullptr;
+  if (dparams && ctx->active != HG_PARAM_NONE) {
+    switch (ctx->active) {
+      case HG_PARAM_N_ZONE: {
+        std::vector<double> v(N);
+        for (int64_t i = 0; i < N; ++i) v[i] = dparams[ctx->h_zone[i]];
+        rc = upload(ctx, dn, v.data(), N);
+        pn = dn;
+        break;
+      }
+      case HG_PARAM_N_CELL:
+        rc = cell_in(ctx, dn, dparams);
+        pn = dn;
+        break;
+      case HG_PARAM_ZB:  // bed data are linear in zb: their tangent is the same map of dzb
+        rc = cell_in(ctx, dzb, dparams);
+        if (!rc) {
+          hipLaunchKernelGGL(k_bed_data, dim3(grid_of(std::max<int64_t>(N, G))), dim3(kBlock), 0, ctx->stream,
+                             int(N), int(G), dzb, ctx->deg, ctx->nbr, ctx->fnx, ctx->fny, ctx->flen, ctx->area,
+                             ctx->g_cell, dS0x, dS0y, dgzb);
+          if (hipGetLastError() != hipSuccess) rc = HG_ERR_HIP;
+        }
+        pzb = dzb;
+        break;
+      case HG_PARAM_Q:
+        rc = upload(ctx, dQin, dparams, size_t(ctx->n_inlet));
+        pQ = dQin;
+        break;
+      default:
+        break;
+    }
+    if (rc) return rc;
+  }
+  rc = state_in(ctx, Q0, 0);
+  if (!rc) {
+    if (dQ0) {
+      double* dst[3] = {T0, T0 + N, T0 + 2 * N};
+      for (int c = 0; c < 3 && !rc; ++c) rc = cell_in(ctx, dst[c], dQ0 + c * N);
+    } else if (hipMemsetAsync(T0, 0, 3 * N * sizeof(double), ctx->stream) != hipSuccess) {
+      rc = HG_ERR_HIP;
+    }
+  }
+  if (!rc) rc = clear_flags(ctx);
+  if (rc) return rc;
+  int cur = 0;
+  double* Tb[2] = {T0, T1};
+  for (int64_t t = 0; t < nsteps; ++t) {
+    const double* in[3] = {ctx->st[cur][0], ctx->st[cur][1], ctx->st[cur][2]};
+    double* out[3] = {ctx->st[1 - cur][0], ctx->st[1 - cur][1], ctx->st[1 - cur][2]};
+    if (ctx->n_inlet > 0)
+      hipLaunchKernelGGL(k_inlet_conveyance_jvp, dim3((ctx->n_inlet + 63) / 64), dim3(64), 0, ctx->stream,
+                         inlet_args_p(ctx, in[0]), Tb[cur], pn, dtot);
+    JvpArgs J{};
+    J.a = make_args_p(ctx, in, out, dt, nullptr);
+    J.dxi = Tb[cur];
+    J.dqx = Tb[cur] + N;
+    J.dqy = Tb[cur] + 2 * N;
+    J.dxo = Tb[1 - cur];
+    J.dqxo = Tb[1 - cur] + N;
+    J.dqyo = Tb[1 - cur] + 2 * N;
+    J.dzb = pzb;
+    J.dn = pn;
+    J.dS0x = pzb ? dS0x : nullptr;
+    J.dS0y = pzb ? dS0y : nullptr;
+    J.dg_zb = pzb ? dgzb : nullptr;
+    J.dinletQ = pQ;
+    J.dtotalA = dtot;
+    hipLaunchKernelGGL(k_swe_step_jvp, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, J);
+    HG_TRY(hipGetLastError());
+    cur = 1 - cur;
+  }
+  double* src[3] = {ctx->st[cur][0], ctx->st[cur][1], ctx->st[cur][2]};
+  rc = state_out(ctx, src, QN);
+  double* dsrc[3] = {Tb[cur], Tb[cur] + N, Tb[cur] + 2 * N};
+  if (!rc) rc = state_out(ctx, dsrc, dQN);
+  ctx->cur = cur;
+  if (!rc) rc = check_flags(ctx);
+  return rc;
 }
 
 int hg_tsit5_tableau(double* c, double* a, double* btilde) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
-    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau",
+    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp",
 ]
 UDE_WEIGHTS = 353
 
@@ -77,6 +77,8 @@ def lib():
         L.hg_tsit5_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
                                      C.c_int64, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         L.hg_tsit5_tableau.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_jvp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                             C.c_int64, C.c_void_p, C.c_void_p]
         for name in ("hg_destroy",):
             getattr(L, name).argtypes = [C.c_void_p]
         L.hg_set_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 9
@@ -263,6 +265,19 @@ class Context:
         self._chk(lib().hg_time_advance(self.h, C.c_double(dt), C.c_int64(nsteps), C.c_int32(reps), C.byref(a),
                                         C.byref(b)))
         return a.value, b.value
+
+    def jvp(self, Q0, dt, nsteps, dQ0=None, params=None, dparams=None):
+        """Forward-mode derivative of nsteps Euler steps along (dQ0, dparams):
+        returns (QN, dQN)."""
+        Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
+        d0 = None if dQ0 is None else np.ascontiguousarray(dQ0, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        dp = None if dparams is None else np.ascontiguousarray(dparams, dtype=np.float64)
+        QN = np.empty(3 * self.N)
+        dQN = np.empty(3 * self.N)
+        self._chk(lib().hg_jvp(self.h, _ptr(Q0), _ptr(d0), _ptr(p), 0 if p is None else p.size, _ptr(dp),
+                               C.c_double(dt), C.c_int64(nsteps), _ptr(QN), _ptr(dQN)))
+        return QN, dQN
 
     def tsit5(self, Q0, t0, t1, n_save, abstol=1e-6, reltol=1e-3, dt0=1e-3, params=None):
         """Adaptive Tsit5 from Q0 over [t0, t1]; returns (n_save + 1, 3N) saved

@@ -21,7 +21,8 @@ module HydrogradHIP
 
 using ChainRulesCore
 
-export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f, tsit5_solve, set_ude!, ude_grad
+export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f, tsit5_solve, set_ude!, ude_grad,
+       sensitivity_jvp
 
 const libhgswe = joinpath(@__DIR__, "..", "lib", "libhgswe.so")
 
@@ -221,6 +222,23 @@ function tsit5_solve(ode_f::DeviceRHS, Q0, params_vector, tspan, nsave; abstol =
                          ctx.ptr, Vector{Float64}(Q0), prm, length(prm), Float64(tspan[1]), Float64(tspan[2]), nsave,
                          abstol, reltol, dt0, sol, stats))
     return sol
+end
+
+# Forward-mode sensitivity on the device (the ForwardDiff.jacobian columns of
+# swe_2D_sensitivity.jl:33-80, one tangent direction per call): returns the
+# state after nsteps Euler steps and its derivative along (dparams, dQ0).
+function sensitivity_jvp(ode_f::DeviceRHS, Q0, params_vector, dparams, dt, nsteps; dQ0 = nothing)
+    nodual(Q0); nodual(params_vector)
+    ctx = context(ode_f.p)
+    prm = Vector{Float64}(params_vector)
+    dp = dparams === nothing ? C_NULL : Vector{Float64}(dparams)
+    d0 = dQ0 === nothing ? C_NULL : Vector{Float64}(dQ0)
+    QN = zeros(length(Q0)); dQN = zeros(length(Q0))
+    check(ctx.ptr, ccall((:hg_jvp, libhgswe), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Float64, Int64,
+                          Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, Vector{Float64}(Q0), d0, prm, length(prm), dp, dt, nsteps, QN, dQN))
+    return QN, dQN
 end
 
 # UDE closure for Manning's n on the device (3 -> 16 -> 16 -> 1 tanh network,

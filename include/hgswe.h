@@ -224,6 +224,15 @@ int hg_tsit5_solve(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
  * weights are row 6 of a).  Host-only, no device needed. */
 int hg_tsit5_tableau(double* c, double* a, double* btilde);
 
+/* Forward mode (JVP) of nsteps Euler steps (custom_ODE_update_cells) along
+ * one tangent direction of the initial state (dQ0, 3N, NULL = 0) and of the
+ * active parameter (dparams, n_params, NULL = 0): QN and its tangent dQN.
+ * The device counterpart of ForwardDiff.Dual through the step
+ * (swe_2D_sensitivity.jl:33-80, ForwardDiff.jacobian column by column);
+ * branch tests follow the primal values, clamps have zero tangent. */
+int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* params, int64_t n_params,
+           const double* dparams, double dt, int64_t nsteps, double* QN, double* dQN);
+
 /* ------------------------------------------------------- UDE closure */
 
 /* Manning's n from a neural closure, evaluated on the matrix cores every

@@ -197,4 +197,24 @@ int orc_jvp_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const double
   });
 }
 
+// Forward-mode run from an initial-state tangent dQ0 (no parameter tangents):
+// nsteps Euler steps of (Q0, dQ0).
+int orc_jvp_state(const Mesh* m, const Consts* c, const FieldPtrs* f, const double* Q0, const double* dQ0, double dt,
+                  int64_t nsteps, double* Qout, double* dQout) {
+  return guarded([&] {
+    using D = Dual<1>;
+    Fields<D> p = make_fields<D>(*m, *f);
+    std::vector<D> q(3 * m->N);
+    for (int64_t k = 0; k < 3 * m->N; ++k) {
+      q[k] = D(Q0[k]);
+      q[k].d[0] = dQ0[k];
+    }
+    for (int64_t s = 0; s < nsteps; ++s) orc::custom_ODE_update_cells(*m, *c, p, q, dt);
+    for (int64_t k = 0; k < 3 * m->N; ++k) {
+      Qout[k] = q[k].v;
+      dQout[k] = q[k].d[0];
+    }
+  });
+}
+
 }  // extern "C"

@@ -155,6 +155,18 @@ class Oracle:
         return Qout, dQout, loss.value, dloss.value
 
 
+    def jvp_state(self, Q0, dQ0, dt, nsteps):
+        """Dual run of nsteps Euler steps from (Q0, dQ0): (QN, dQN)."""
+        N = self.case.N
+        Q0 = np.ascontiguousarray(Q0, dtype=D)
+        dQ0 = np.ascontiguousarray(dQ0, dtype=D)
+        Qout = np.empty(3 * N)
+        dQout = np.empty(3 * N)
+        self._check(lib().orc_jvp_state(C.byref(self.mesh), C.byref(self.consts), C.byref(self.fields), _p(Q0),
+                                        _p(dQ0), C.c_double(dt), C.c_int64(nsteps), _p(Qout), _p(dQout)))
+        return Qout, dQout
+
+
 def friction(h, qx, qy, n, g=9.81, k_n=1.0, h_small=1e-3):
     arrs = [np.ascontiguousarray(a, dtype=D) for a in (h, qx, qy, n)]
     fx = np.empty_like(arrs[0])
