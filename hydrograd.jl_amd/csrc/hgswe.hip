@@ -36,7 +36,7 @@
 using hg::Side;
 using hg::StepArgs;
 
-static std::string g_create_err;
+static thread_local std::string g_create_err;  // per caller thread, like the contexts
 
 #define HG_TRY(expr)                                                                   \
   do {                                                                                 \
@@ -75,6 +75,15 @@ int upload(hg_ctx* ctx, T* dst, const T* src, size_t n) {
 }
 
 // --------------------------------------------------------------- kernels ---
+
+// flag[1] = 1 when any of a, b, c (n values each) is NaN or infinite.
+__global__ __launch_bounds__(kBlock) void k_nonfinite(int64_t n, const double* a, const double* b, const double* c,
+                                                      int32_t* flag) {
+  bool bad = false;
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+    bad = bad || !isfinite(a[i]) || !isfinite(b[i]) || !isfinite(c[i]);
+  if (bad) flag[1] = 1;
+}
 
 // Inlet conveyance (bc_2D.jl:640-680): total_A_b = sum over the inlet's
 // faces, in NodeString order, of ((L^(5/3) h_c) / n_c) * wet_c.  The terms
@@ -1853,6 +1862,19 @@ int check_flags(hg_ctx* ctx) {
     ctx->err = "Total cross-sectional conveyance for an inlet-q boundary is not positive (<= 1e-10)";
     return HG_ERR_CONVEYANCE;
   }
+  if (f[1]) {
+    ctx->err = "non-finite value (NaN or Inf) in the state";
+    return HG_ERR_NONFINITE;
+  }
+  return HG_OK;
+}
+
+// Finiteness check of the device state (one reduction, at the end of a call).
+int enqueue_finite_check(hg_ctx* ctx) {
+  const int grid = std::min(grid_of(ctx->N), 1024);
+  hipLaunchKernelGGL(k_nonfinite, dim3(grid), dim3(kBlock), 0, ctx->stream, int64_t(ctx->N), ctx->st[ctx->cur][0],
+                     ctx->st[ctx->cur][1], ctx->st[ctx->cur][2], ctx->flags);
+  HG_TRY(hipGetLastError());
   return HG_OK;
 }
 
@@ -2554,6 +2576,7 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps) {
   }
   int rc = clear_flags(ctx);
   if (!rc) rc = advance_async(ctx, dt, nsteps);
+  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
 }

@@ -260,3 +260,16 @@ def test_large_polygon_uses_cell_kernel():
     a = ctx.step(c.Q0, c.dt, 50)
     b = capi.Context(c, cell_kernel=True).step(c.Q0, c.dt, 50)
     assert np.array_equal(a, b)
+
+
+def test_nonfinite_state_is_reported():
+    """A NaN in the state (data, not a fault) ends the call with
+    HG_ERR_NONFINITE instead of returning silently corrupted values."""
+    c = cases.dam_break(nx=20, ny=4)
+    Q = c.Q0.copy()
+    Q[7] = np.nan
+    ctx = capi.Context(c)
+    with pytest.raises(capi.HGError) as e:
+        ctx.step(Q, c.dt, 5)
+    assert e.value.code == capi.HG_ERR_NONFINITE
+    ctx.step(c.Q0, c.dt, 5)  # the context stays usable
