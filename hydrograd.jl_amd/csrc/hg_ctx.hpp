@@ -45,6 +45,7 @@ struct StepArgs {
   const double* totalA;  // per inlet, this step
   uint8_t* clamp;        // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
+  const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
 };
 
 struct InletArgs {
@@ -114,6 +115,7 @@ struct TileArgs {
   int32_t* flag;
   uint8_t* clamp;  // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
+  const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
   int32_t xcd_chunk;  // block -> tile mapping: 0 identity, G > 0 runs of G tiles per XCD in windows of 8G
 };
 
@@ -155,6 +157,7 @@ struct AdjArgs {
   const double* exitWSE;
   const double* totalA;
   double g, half_g, kn2, hsmall, dt;
+  const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
 };
 
 template <class T>
@@ -235,6 +238,12 @@ struct hg_ctx {
   double* ude_wpart = nullptr;      // per-block weight cotangents
   double* ude_wbar = nullptr;       // accumulated weight cotangent (353)
   std::vector<void*> ude_allocs;
+
+  // batches: instance boundaries in caller order, and the per-cell time step
+  // (lib order) when per-instance steps are set
+  std::vector<int64_t> inst_off;
+  double* dt_cell = nullptr;  // dt_buf while per-instance steps are set, else NULL
+  double* dt_buf = nullptr;
 
   // forward-mode (JVP) tangents: state (2 x 3N) and parameter fields
   double* jvp = nullptr;

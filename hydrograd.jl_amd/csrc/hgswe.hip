@@ -202,9 +202,10 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
   }
   const double d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;  // :264
   if (STEP) {  // custom_ODE_solvers.jl:16-26
-    double xn = xi + a.dt * d0;
-    double qxn = qx0 + a.dt * d1;
-    double qyn = qy0 + a.dt * d2;
+    const double dti = a.dtc ? a.dtc[i] : a.dt;  // per-instance step of a batch, when set
+    double xn = xi + dti * d0;
+    double qxn = qx0 + dti * d1;
+    double qyn = qy0 + dti * d2;
     const bool dry = xn < hsmall;
     if (dry) {
       xn = hsmall;
@@ -412,9 +413,10 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
   }
   const double d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;
   if (STEP) {
-    double xn = xi + a.dt * d0;
-    double qxn = qx0 + a.dt * d1;
-    double qyn = qy0 + a.dt * d2;
+    const double dti = a.dtc ? a.dtc[i] : a.dt;  // per-instance step of a batch, when set
+    double xn = xi + dti * d0;
+    double qxn = qx0 + dti * d1;
+    double qyn = qy0 + dti * d2;
     const bool dry = xn < hsmall;
     if (dry) {
       xn = hsmall;
@@ -537,9 +539,10 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
   c.lt1 = a.lqx[i];
   c.lt2 = a.lqy[i];
   if (STEP && a.mask && a.mask[i]) c.lt0 = c.lt1 = c.lt2 = 0.0;
-  c.y0 = STEP ? a.dt * c.lt0 : c.lt0;
-  c.y1 = STEP ? a.dt * c.lt1 : c.lt1;
-  c.y2 = STEP ? a.dt * c.lt2 : c.lt2;
+  const double dti = a.dtc ? a.dtc[i] : a.dt;
+  c.y0 = STEP ? dti * c.lt0 : c.lt0;
+  c.y1 = STEP ? dti * c.lt1 : c.lt1;
+  c.y2 = STEP ? dti * c.lt2 : c.lt2;
   return c;
 }
 
@@ -562,7 +565,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) 
     const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
     double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
     if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
-    const double yj0 = STEP ? a.dt * m0 : m0, yj1 = STEP ? a.dt * m1 : m1, yj2 = STEP ? a.dt * m2 : m2;
+    const double dtj = a.dtc ? a.dtc[j] : a.dt;
+    const double yj0 = STEP ? dtj * m0 : m0, yj1 = STEP ? dtj * m1 : m1, yj2 = STEP ? dtj * m2 : m2;
     const double fj = L / a.area[j];
     const double c[3] = {fi * ca.y0 + fj * yj0, fi * ca.y1 + fj * yj1, fi * ca.y2 + fj * yj2};
     hg::SideBar Ab, Bb;
@@ -1287,9 +1291,10 @@ __global__ __launch_bounds__(kBlock) void k_swe_step_jvp(JvpArgs J) {
     s2 = a.g * xi * ldd(a.S0y, J.dS0y, i) - fy;
   }
   const Dual1 d0 = r0 + 0.0, d1 = r1 + s1, d2 = r2 + s2;
-  Dual1 xn = xi + a.dt * d0;  // custom_ODE_solvers.jl:16-26
-  Dual1 qxn = qx0 + a.dt * d1;
-  Dual1 qyn = qy0 + a.dt * d2;
+  const double dti = a.dtc ? a.dtc[i] : a.dt;
+  Dual1 xn = xi + dti * d0;  // custom_ODE_solvers.jl:16-26
+  Dual1 qxn = qx0 + dti * d1;
+  Dual1 qyn = qy0 + dti * d2;
   if (xn < hsmall) {
     xn = Dual1(hsmall);
     qxn = Dual1(0.0);
@@ -1429,6 +1434,7 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.kn2 = ctx->c.k_n * ctx->c.k_n;
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
+  a.dtc = ctx->dt_cell;
   return a;
 }
 
@@ -1474,6 +1480,7 @@ StepArgs make_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3
   a.kn2 = ctx->c.k_n * ctx->c.k_n;
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
+  a.dtc = ctx->dt_cell;
   return a;
 }
 
@@ -1708,6 +1715,7 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
   a.kn2 = ctx->c.k_n * ctx->c.k_n;
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
+  a.dtc = ctx->dt_cell;
   return a;
 }
 
@@ -2211,6 +2219,7 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   }
   // ---- renumbering
   if (seg.empty()) seg = {0, N};
+  ctx->inst_off = seg;
   if ((flags & HG_REORDER) && m->centroid_xy) {
     ctx->perm = morton_perm(seg, m->centroid_xy);
   } else {
@@ -3074,6 +3083,36 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   (void)hipEventDestroy(e1);
   if (!rc) rc = check_flags(ctx);
   return rc;
+}
+
+int hg_batch_set_dt(hg_ctx* ctx, const double* dt_inst) {
+  if (!ctx) return HG_ERR_ARG;
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  if (drop_graph(ctx)) return HG_ERR_HIP;  // captured launches hold the previous step pointer
+  if (!dt_inst) {
+    ctx->dt_cell = nullptr;  // the buffer stays owned by the context
+    return HG_OK;
+  }
+  const size_t ninst = ctx->inst_off.size() - 1;
+  for (size_t r = 0; r < ninst; ++r)
+    if (!(dt_inst[r] > 0.0) || !std::isfinite(dt_inst[r])) {
+      ctx->err = "hg_batch_set_dt: time steps must be positive and finite";
+      return HG_ERR_ARG;
+    }
+  std::vector<double> v(ctx->N);
+  for (int64_t i = 0; i < ctx->N; ++i) {
+    const int64_t c = ctx->perm[i];
+    const size_t r = size_t(std::upper_bound(ctx->inst_off.begin(), ctx->inst_off.end(), c) - ctx->inst_off.begin()) - 1;
+    v[i] = dt_inst[r];
+  }
+  if (!ctx->dt_buf) {
+    int rc = dalloc(ctx, &ctx->dt_buf, ctx->N);
+    if (rc) return rc;
+  }
+  int rc = upload(ctx, ctx->dt_buf, v.data(), ctx->N);
+  if (rc) return rc;
+  ctx->dt_cell = ctx->dt_buf;
+  return HG_OK;
 }
 
 int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk) {

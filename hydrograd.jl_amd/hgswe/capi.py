@@ -26,7 +26,7 @@ EXPORTS = [
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
-    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp",
+    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt",
 ]
 UDE_WEIGHTS = 353
 
@@ -67,6 +67,7 @@ def lib():
         L.hg_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(hg_mesh), C.POINTER(hg_bcs),
                                       C.POINTER(hg_consts), C.c_int32]
         L.hg_set_tile_config.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.hg_batch_set_dt.argtypes = [C.c_void_p, C.c_void_p]
         L.hg_get_tile_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.hg_autotune.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_double)]
@@ -310,6 +311,11 @@ class Context:
         g = np.empty(UDE_WEIGHTS)
         self._chk(lib().hg_ude_grad(self.h, _ptr(g)))
         return g
+
+    def batch_set_dt(self, dts):
+        """Per-instance time steps of a batch (None: back to the scalar dt)."""
+        d = None if dts is None else np.ascontiguousarray(dts, dtype=np.float64)
+        self._chk(lib().hg_batch_set_dt(self.h, _ptr(d)))
 
     def set_tile_config(self, tile_cells, xcd_chunk=0):
         self._chk(lib().hg_set_tile_config(self.h, int(tile_cells), int(xcd_chunk)))

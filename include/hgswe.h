@@ -114,6 +114,12 @@ int hg_destroy(hg_ctx* ctx);
 int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const hg_bcs* bcs, const hg_consts* consts,
                     int32_t flags);
 
+/* Per-instance time steps for a batch (n_inst values, or NULL to return to
+ * the scalar dt).  While set, every stepping call (hg_step, hg_advance,
+ * hg_adjoint, hg_grad_probes, hg_jvp) advances instance r by dt_inst[r] and
+ * ignores its dt argument. */
+int hg_batch_set_dt(hg_ctx* ctx, const double* dt_inst);
+
 /* Message of the last error on this context (or of the last failed
  * hg_create when ctx is NULL). */
 const char* hg_last_error(const hg_ctx* ctx);

@@ -130,3 +130,32 @@ def test_batched_gradient_is_per_instance_gradients(batch):
         assert np.array_equal(gq[iq:iq + c.mesh.n_inlet], q_)
         iq += c.mesh.n_inlet
     assert abs(L - Ls) <= 1e-12 * abs(Ls)
+
+
+def test_per_instance_time_steps(batch):
+    """With per-instance steps, each instance advances exactly as it does
+    alone with its own dt (forward and gradient)."""
+    B = batch
+    inst = B.extra["instances"]
+    off = B.extra["cell_off"]
+    dts = np.array([0.5 * cases.cfl_dt(c, 0.5) for c in inst])
+    ctx = capi.Context(B)
+    ctx.batch_set_dt(dts)
+    nsteps = 30
+    qb = ctx.step(B.Q0, 123.0, nsteps)  # the scalar dt is ignored
+    for c, q, dt in zip(inst, cases.split_state(B, qb), dts):
+        assert np.array_equal(q, capi.Context(c).step(c.Q0, dt, nsteps)), c.name
+    rng = np.random.default_rng(8)
+    probes, targets = [], []
+    for c in inst:
+        p = rng.choice(c.N, size=6, replace=False).astype(np.int32)
+        probes.append(p)
+        targets.append(c.Q0[p] + c.hstill[p] + c.zb[p] + rng.normal(0.0, 0.05, 6))
+    probe = np.concatenate([p + off[r] for r, p in enumerate(probes)]).astype(np.int32)
+    L, gn, gzb, gq = ctx.grad_probes(B.Q0, 1.0, nsteps, 7, probe, np.concatenate(targets))
+    for r, (c, dt) in enumerate(zip(inst, dts)):
+        _, n_, z_, _ = capi.Context(c).grad_probes(c.Q0, dt, nsteps, 7, probes[r], targets[r])
+        assert np.array_equal(gn[off[r]:off[r + 1]], n_)
+        assert np.array_equal(gzb[off[r]:off[r + 1]], z_)
+    ctx.batch_set_dt(None)
+    assert np.array_equal(ctx.step(B.Q0, B.dt, 5), capi.Context(B).step(B.Q0, B.dt, 5))
