@@ -160,6 +160,13 @@ struct AdjArgs {
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
 };
 
+// A variable Manning-n law (kind HG_NLAW_*; 0: off) and its parameters.
+enum { NLAW_OFF = 0, NLAW_POWER = 1, NLAW_SIGMOID = 2, NLAW_INVERSE = 3, NLAW_H_UMAG_KS = 4 };
+struct NLaw {
+  int32_t kind;
+  double lo, hi, k, hmid;  // n_lower, n_upper, k, h_mid
+};
+
 template <class T>
 struct DBuf {  // owning device buffer
   T* p = nullptr;
@@ -238,6 +245,12 @@ struct hg_ctx {
   double* ude_wpart = nullptr;      // per-block weight cotangents
   double* ude_wbar = nullptr;       // accumulated weight cotangent (353)
   std::vector<void*> ude_allocs;
+
+  // variable Manning-n law (kind 0: off), its per-cell ks and per-step n cotangent
+  hg::NLaw nlaw{};
+  double* nlaw_ks = nullptr;
+  double* nlaw_nbar = nullptr;
+  std::vector<void*> nlaw_allocs;
 
   // batches: instance boundaries in caller order, and the per-cell time step
   // (lib order) when per-instance steps are set

@@ -30,7 +30,8 @@
 #include "hg_roe_pair.hpp"
 #include "hg_adjoint.hpp"
 #include "hg_ude.hpp"
-#include "hg_dual.hpp"  // last: its sqrt / cbrt overloads must stay out of the double paths above
+#include "hg_dual.hpp"  // after the double paths: its sqrt / cbrt overloads must stay out of them
+#include "hg_nlaw.hpp"
 #include "hgswe.h"
 
 using hg::Side;
@@ -1077,6 +1078,68 @@ __global__ __launch_bounds__(kBlock) void k_ude_wsum(int nblocks, const double* 
   if (tid == 0) wbar[k] += s[0];
 }
 
+// ---- variable Manning n: one law evaluation per cell (hg_nlaw.hpp) ----
+
+__device__ __forceinline__ double nlaw_ks(const hg::NLawArgs& a, int i) {
+  return a.law.kind == hg::NLAW_H_UMAG_KS ? a.ks[i] : 1.0;
+}
+
+// n <- law(features of the state), before every RHS evaluation.
+__global__ __launch_bounds__(kBlock) void k_nlaw_n(hg::NLawArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  double h, U;
+  hg::nlaw_features<double>(a.law, a.xi[i], a.qx[i], a.qy[i], a.hstill[i], a.hsmall, h, U);
+  a.n[i] = hg::nlaw_eval<double, false>(a.law, h, U, nlaw_ks(a, i), nullptr, nullptr, nullptr);
+}
+
+// The law at given (h, |U|), with its auxiliary outputs.
+__global__ __launch_bounds__(kBlock) void k_nlaw_eval(hg::NLawArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  double hks, f, Re;
+  a.n[i] = hg::nlaw_eval<double, true>(a.law, a.h_in[i], a.U_in[i], nlaw_ks(a, i), &hks, &f, &Re);
+  if (a.hks) a.hks[i] = hks;
+  if (a.f) a.f[i] = f;
+  if (a.Re) a.Re[i] = Re;
+}
+
+// dn/d(state component s) of cell i: one dual pass seeded on that component.
+__device__ __forceinline__ double nlaw_partial(const hg::NLawArgs& a, int i, int s) {
+  using hg::Dual1;
+  const Dual1 xi(a.xi[i], s == 0 ? 1.0 : 0.0), qx(a.qx[i], s == 1 ? 1.0 : 0.0), qy(a.qy[i], s == 2 ? 1.0 : 0.0);
+  Dual1 h, U;
+  hg::nlaw_features<Dual1>(a.law, xi, qx, qy, a.hstill[i], a.hsmall, h, U);
+  return hg::nlaw_eval<Dual1, false>(a.law, h, U, nlaw_ks(a, i), nullptr, nullptr, nullptr).d;
+}
+
+// Reverse mode: state cotangent += nbar * dn/d(xi, q_x, q_y).  The depth-only
+// laws do not read the discharge.
+__global__ __launch_bounds__(kBlock) void k_nlaw_vjp(hg::NLawArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const double nb = a.nbar[i];
+  if (nb == 0.0) return;
+  a.ox[i] += nb * nlaw_partial(a, i, 0);
+  if (a.law.kind == hg::NLAW_H_UMAG_KS) {
+    a.oqx[i] += nb * nlaw_partial(a, i, 1);
+    a.oqy[i] += nb * nlaw_partial(a, i, 2);
+  }
+}
+
+// Forward mode: n and its tangent along the state tangent.
+__global__ __launch_bounds__(kBlock) void k_nlaw_jvp(hg::NLawArgs a) {
+  using hg::Dual1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const Dual1 xi(a.xi[i], a.dxi[i]), qx(a.qx[i], a.dqx[i]), qy(a.qy[i], a.dqy[i]);
+  Dual1 h, U;
+  hg::nlaw_features<Dual1>(a.law, xi, qx, qy, a.hstill[i], a.hsmall, h, U);
+  const Dual1 n = hg::nlaw_eval<Dual1, false>(a.law, h, U, nlaw_ks(a, i), nullptr, nullptr, nullptr);
+  a.n[i] = n.v;
+  a.dn[i] = n.d;
+}
+
 // Layout check of the 16x16x4 matrix-core products used above: D = A B for
 // A (16x4) and B (4x16) given row-major, D returned row-major.
 template <class T>
@@ -1586,12 +1649,40 @@ int enqueue_ude_n(hg_ctx* ctx, const double* const q[3]) {
   return HG_OK;
 }
 
+hg::NLawArgs nlaw_args(hg_ctx* ctx, const double* const q[3]) {
+  hg::NLawArgs a{};
+  a.law = ctx->nlaw;
+  a.N = int32_t(ctx->N);
+  a.xi = q[0];
+  a.qx = q[1];
+  a.qy = q[2];
+  a.hstill = ctx->hstill;
+  a.ks = ctx->nlaw_ks;
+  a.hsmall = ctx->c.h_small;
+  a.n = ctx->n;
+  return a;
+}
+
+// n <- law(state q) when a variable-n law is set.
+int enqueue_nlaw_n(hg_ctx* ctx, const double* const q[3]) {
+  if (ctx->nlaw.kind == hg::NLAW_OFF) return HG_OK;
+  hipLaunchKernelGGL(k_nlaw_n, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, nlaw_args(ctx, q));
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+// The state-dependent n in force (UDE closure or variable-n law), if any.
+int enqueue_cell_n(hg_ctx* ctx, const double* const q[3]) {
+  if (ctx->ude_prec) return enqueue_ude_n(ctx, q);
+  return enqueue_nlaw_n(ctx, q);
+}
+
 // Enqueue one step (or one RHS evaluation) reading state `src`.
 int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
                  double* o2 = nullptr) {
-  if (ctx->ude_prec) {
+  {
     const double* q[3] = {ctx->st[src][0], ctx->st[src][1], ctx->st[src][2]};
-    int rc = enqueue_ude_n(ctx, q);
+    int rc = enqueue_cell_n(ctx, q);
     if (rc) return rc;
   }
   if (ctx->tiled) {
@@ -1617,8 +1708,8 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
 // record of the inlet conveyances of this evaluation (totalA_out).
 int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt, bool step,
                    uint8_t* clamp = nullptr, double* totalA_out = nullptr) {
-  if (ctx->ude_prec) {
-    int rc = enqueue_ude_n(ctx, in);
+  {
+    int rc = enqueue_cell_n(ctx, in);
     if (rc) return rc;
   }
   hg::InletArgs ia = inlet_args_p(ctx, in[0]);
@@ -1727,7 +1818,9 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
 int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
                     double* const out[3], double dt, bool step, const double* totalA = nullptr) {
   const bool ude = ctx->ude_prec != 0;
-  double* nbar = ude ? ctx->ude_nbar : ctx->adj_nbar;  // with the closure, n-bar of this step only
+  const bool law = ctx->nlaw.kind != hg::NLAW_OFF;
+  // with a state-dependent n, n-bar of this step only
+  double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : ctx->adj_nbar;
   if (ude && ctx->ude_nsets != 1) {
     ctx->err = "the UDE closure's reverse mode takes one weight set";
     return HG_ERR_STATE;
@@ -1736,6 +1829,11 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     int rc = enqueue_ude_n(ctx, q);
     if (rc) return rc;
     HG_TRY(hipMemsetAsync(ctx->ude_nbar, 0, ctx->N * sizeof(double), ctx->stream));
+  }
+  if (law) {
+    int rc = enqueue_nlaw_n(ctx, q);
+    if (rc) return rc;
+    HG_TRY(hipMemsetAsync(ctx->nlaw_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   }
   if (ctx->n_inlet > 0 && !totalA) {
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
@@ -1772,6 +1870,14 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
       hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
     hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW), dim3(kBlock), 0, ctx->stream, ude_blocks(ctx), ctx->ude_wpart,
                        ctx->ude_wbar);
+  }
+  if (law) {  // through the law: n(Q) adds to the state cotangent
+    hg::NLawArgs u = nlaw_args(ctx, q);
+    u.nbar = ctx->nlaw_nbar;
+    u.ox = out[0];
+    u.oqx = out[1];
+    u.oqy = out[2];
+    hipLaunchKernelGGL(k_nlaw_vjp, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, u);
   }
   HG_TRY(hipGetLastError());
   return HG_OK;
@@ -2519,6 +2625,10 @@ int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, 
     ctx->err = "Manning n comes from the UDE closure; it cannot also be the active parameter";
     return HG_ERR_STATE;
   }
+  if (ctx->nlaw.kind != hg::NLAW_OFF && (kind == HG_PARAM_N_ZONE || kind == HG_PARAM_N_CELL)) {
+    ctx->err = "Manning n comes from the variable-n law; it cannot also be the active parameter";
+    return HG_ERR_STATE;
+  }
   if (kind == HG_PARAM_N_ZONE) {
     if (!zone_of_cell || n_zones <= 0 || size_t(n_zones) > ctx->scratch_n) {
       ctx->err = "HG_PARAM_N_ZONE needs zone_of_cell and n_zones > 0";
@@ -2648,6 +2758,10 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
     ctx->err = "hg_set_ude: Manning n is the active parameter";
     return HG_ERR_STATE;
   }
+  if (ctx->nlaw.kind != hg::NLAW_OFF) {
+    ctx->err = "hg_set_ude: a variable-n law is set";
+    return HG_ERR_STATE;
+  }
   const int64_t N = ctx->N;
   std::vector<int32_t> set;
   if (set_of_cell) {
@@ -2722,6 +2836,116 @@ int hg_ude_grad(hg_ctx* ctx, double* w_bar) {
   HG_TRY(hipMemcpyAsync(w_bar, ctx->ude_wbar, hg::kUdeW * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HG_TRY(hipStreamSynchronize(ctx->stream));
   return HG_OK;
+}
+
+int hg_set_n_law(hg_ctx* ctx, int32_t kind, const double* params, const double* ks) {
+  if (!ctx) return HG_ERR_ARG;
+  if (kind < HG_NLAW_OFF || kind > HG_NLAW_H_UMAG_KS) {
+    ctx->err = "hg_set_n_law: unknown law";
+    return HG_ERR_ARG;
+  }
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the law
+  for (void* p : ctx->nlaw_allocs) {
+    ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), p));
+    HG_TRY(hipFree(p));
+  }
+  ctx->nlaw_allocs.clear();
+  ctx->nlaw = hg::NLaw{};
+  ctx->nlaw_ks = nullptr;
+  ctx->nlaw_nbar = nullptr;
+  if (kind == HG_NLAW_OFF) {  // back to the per-cell n of hg_set_fields
+    HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    return HG_OK;
+  }
+  if (ctx->ude_prec) {
+    ctx->err = "hg_set_n_law: the UDE closure is on";
+    return HG_ERR_STATE;
+  }
+  if (ctx->active == HG_PARAM_N_ZONE || ctx->active == HG_PARAM_N_CELL) {
+    ctx->err = "hg_set_n_law: Manning n is the active parameter";
+    return HG_ERR_STATE;
+  }
+  hg::NLaw L{};
+  L.kind = kind;
+  if (kind == HG_NLAW_H_UMAG_KS) {
+    if (!ks) {
+      ctx->err = "hg_set_n_law: h_Umag_ks needs ks per cell";
+      return HG_ERR_ARG;
+    }
+  } else {
+    if (!params) {
+      ctx->err = "hg_set_n_law: params {n_lower, n_upper, k, h_mid} missing";
+      return HG_ERR_ARG;
+    }
+    L.lo = params[0];
+    L.hi = params[1];
+    L.k = params[2];
+    L.hmid = params[3];
+    if (!(L.k > 0.0)) {  // the laws' @assert (process_ManningN_2D.jl:140, 155, 171)
+      ctx->err = "hg_set_n_law: k must be a positive scaling factor";
+      return HG_ERR_ARG;
+    }
+    if (kind == HG_NLAW_SIGMOID && !(L.hmid > 0.0)) {  // :172
+      ctx->err = "hg_set_n_law: h_mid must be a positive water depth";
+      return HG_ERR_ARG;
+    }
+  }
+  const size_t nalloc0 = ctx->allocs.size();
+  int rc = dalloc(ctx, &ctx->nlaw_nbar, ctx->N);
+  if (!rc && ks) {
+    rc = dalloc(ctx, &ctx->nlaw_ks, ctx->N);
+    if (!rc) rc = cell_in(ctx, ctx->nlaw_ks, ks);
+  }
+  ctx->nlaw_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
+  if (rc) return rc;
+  ctx->nlaw = L;
+  return HG_OK;
+}
+
+int hg_n_law_eval(hg_ctx* ctx, const double* h, const double* Umag, double* n, double* h_ks, double* f,
+                  double* Re) {
+  if (!ctx || !h || !Umag) return ctx ? (ctx->err = "hg_n_law_eval: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (ctx->nlaw.kind == hg::NLAW_OFF) {
+    ctx->err = "hg_n_law_eval: no variable-n law is set";
+    return HG_ERR_STATE;
+  }
+  const int64_t N = ctx->N;
+  double* buf = nullptr;  // n, h/ks, f, Re
+  int rc = dalloc(ctx, &buf, size_t(4) * N);
+  if (rc) return rc;
+  ctx->nlaw_allocs.push_back(buf);  // freed with the law
+  if (!rc) rc = cell_in(ctx, ctx->scratch, h);
+  if (!rc) rc = cell_in(ctx, ctx->scratch + N, Umag);
+  if (rc) return rc;
+  hg::NLawArgs a = nlaw_args(ctx, ctx->st[0]);
+  a.h_in = ctx->scratch;
+  a.U_in = ctx->scratch + N;
+  a.n = buf;
+  a.hks = buf + N;
+  a.f = buf + 2 * N;
+  a.Re = buf + 3 * N;
+  hipLaunchKernelGGL(k_nlaw_eval, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, a);
+  HG_TRY(hipGetLastError());
+  double* outs[4] = {n, h_ks, f, Re};
+  for (int k = 0; k < 4 && !rc; ++k)
+    if (outs[k]) rc = cell_out(ctx, buf + k * N, outs[k]);
+  ctx->nlaw_allocs.pop_back();
+  ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), static_cast<void*>(buf)));
+  if (hipFree(buf) != hipSuccess && !rc) rc = HG_ERR_HIP;
+  return rc;
+}
+
+int hg_cell_n(hg_ctx* ctx, const double* Q, double* n) {
+  if (!ctx || !Q || !n) return ctx ? (ctx->err = "hg_cell_n: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  int rc = state_in(ctx, Q, 0);
+  if (rc) return rc;
+  ctx->cur = 0;
+  const double* q[3] = {ctx->st[0][0], ctx->st[0][1], ctx->st[0][2]};
+  rc = enqueue_cell_n(ctx, q);
+  if (!rc) rc = cell_out(ctx, ctx->n, n);
+  return rc;
 }
 
 int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D) {
@@ -2842,9 +3066,19 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
   if (rc) return rc;
   int cur = 0;
   double* Tb[2] = {T0, T1};
+  const bool law = ctx->nlaw.kind != hg::NLAW_OFF;
+  if (law) pn = dn;  // the law's tangent of n, rewritten every step
   for (int64_t t = 0; t < nsteps; ++t) {
     const double* in[3] = {ctx->st[cur][0], ctx->st[cur][1], ctx->st[cur][2]};
     double* out[3] = {ctx->st[1 - cur][0], ctx->st[1 - cur][1], ctx->st[1 - cur][2]};
+    if (law) {
+      hg::NLawArgs u = nlaw_args(ctx, in);
+      u.dxi = Tb[cur];
+      u.dqx = Tb[cur] + N;
+      u.dqy = Tb[cur] + 2 * N;
+      u.dn = dn;
+      hipLaunchKernelGGL(k_nlaw_jvp, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, u);
+    }
     if (ctx->n_inlet > 0)
       hipLaunchKernelGGL(k_inlet_conveyance_jvp, dim3((ctx->n_inlet + 63) / 64), dim3(64), 0, ctx->stream,
                          inlet_args_p(ctx, in[0]), Tb[cur], pn, dtot);

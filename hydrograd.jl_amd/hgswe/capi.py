@@ -19,6 +19,10 @@ HG_OK, HG_ERR_ARG, HG_ERR_HIP, HG_ERR_MESH, HG_ERR_CONVEYANCE, HG_ERR_NONFINITE,
 HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = range(5)
 HG_REORDER = 1
 HG_CELL_KERNEL = 2
+HG_NLAW_OFF, HG_NLAW_POWER, HG_NLAW_SIGMOID, HG_NLAW_INVERSE, HG_NLAW_H_UMAG_KS = range(5)
+# create_manning_function's type names (process_ManningN_2D.jl:122-134)
+NLAW_KINDS = {"power_law": HG_NLAW_POWER, "sigmoid": HG_NLAW_SIGMOID, "inverse": HG_NLAW_INVERSE,
+              "h_Umag_ks": HG_NLAW_H_UMAG_KS}
 
 # every function declared in include/hgswe.h
 EXPORTS = [
@@ -26,7 +30,8 @@ EXPORTS = [
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
-    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt",
+    "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
+    "hg_n_law_eval", "hg_cell_n",
 ]
 UDE_WEIGHTS = 353
 
@@ -74,6 +79,9 @@ def lib():
         L.hg_set_ude.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
         L.hg_ude_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hg_ude_grad.argtypes = [C.c_void_p, C.c_void_p]
+        L.hg_set_n_law.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.hg_n_law_eval.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.hg_cell_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hg_mfma_probe.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.hg_tsit5_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
                                      C.c_int64, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
@@ -311,6 +319,34 @@ class Context:
         g = np.empty(UDE_WEIGHTS)
         self._chk(lib().hg_ude_grad(self.h, _ptr(g)))
         return g
+
+    def set_n_law(self, kind, n_lower=0.0, n_upper=0.0, k=1.0, h_mid=1.0, ks=None):
+        """Variable Manning n (the forward simulation's "variable" option):
+        kind is HG_NLAW_* or a reference type name ("power_law", "sigmoid",
+        "inverse", "h_Umag_ks"); ks per cell for h_Umag_ks.  kind=None or
+        HG_NLAW_OFF turns it off."""
+        kind = HG_NLAW_OFF if kind is None else NLAW_KINDS.get(kind, kind)
+        p = np.array([n_lower, n_upper, k, h_mid], dtype=np.float64)
+        ks = None if ks is None else np.ascontiguousarray(ks, dtype=np.float64)
+        if ks is not None and ks.size != self.N:
+            raise ValueError("ks needs one value per cell")
+        self._chk(lib().hg_set_n_law(self.h, int(kind), _ptr(p), _ptr(ks)))
+
+    def n_law_eval(self, h, Umag):
+        """(n, h/ks, f, Re) of the law at the given depths and velocity magnitudes."""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        U = np.ascontiguousarray(Umag, dtype=np.float64)
+        if h.size != self.N or U.size != self.N:
+            raise ValueError("h and Umag need one value per cell")
+        outs = [np.empty(self.N) for _ in range(4)]
+        self._chk(lib().hg_n_law_eval(self.h, _ptr(h), _ptr(U), *[_ptr(o) for o in outs]))
+        return tuple(outs)
+
+    def cell_n(self, Q):
+        """Manning n per cell that the RHS uses at state Q."""
+        n = np.empty(self.N)
+        self._chk(lib().hg_cell_n(self.h, _ptr(np.ascontiguousarray(Q, dtype=np.float64)), _ptr(n)))
+        return n
 
     def batch_set_dt(self, dts):
         """Per-instance time steps of a batch (None: back to the scalar dt)."""

@@ -136,8 +136,27 @@ function context(p)
                    Vector{Float64}(p.ManningN_cells), Vector{Float64}(p.inletQ_TotalQ), Vector{Float64}(p.exitH_WSE))
         check(ctx.ptr, rc)
         set_active!(ctx, p)
+        set_n_law!(ctx, p)
         ctx
     end
+end
+
+# The forward simulation's variable Manning n (ManningN_option "variable",
+# semi_discretize_swe_2D.jl:129-135): the law of create_manning_function
+# (process_ManningN_2D.jl:122-134) runs on the device before every RHS.
+const HG_NLAW = Dict("power_law" => Int32(1), "sigmoid" => Int32(2), "inverse" => Int32(3), "h_Umag_ks" => Int32(4))
+function set_n_law!(ctx, p)
+    s = p.settings
+    (s.bPerform_Forward_Simulation && s.forward_settings.ManningN_option == "variable") || return nothing
+    fs = s.forward_settings
+    kind = HG_NLAW[fs.ManningN_function_type]
+    prm = fs.ManningN_function_parameters
+    get1(k) = Float64(get(prm, k, 1.0))
+    params = [Float64(get(prm, "n_lower", 0.0)), Float64(get(prm, "n_upper", 0.0)), get1("k"), get1("h_mid")]
+    ks = kind == HG_NLAW["h_Umag_ks"] ? Vector{Float64}(p.ks_cells) : C_NULL
+    check(ctx.ptr, ccall((:hg_set_n_law, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, kind, params, ks))
+    return nothing
 end
 
 # active_param_name -> HG_PARAM_* (semi_discretize_swe_2D.jl:114-199)

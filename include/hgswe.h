@@ -263,6 +263,44 @@ int hg_ude_n(hg_ctx* ctx, const double* Q, double* n);
  * hg_adjoint or hg_grad_probes call with the closure on (one weight set). */
 int hg_ude_grad(hg_ctx* ctx, double* w_bar);
 
+/* ---- Variable Manning n: the forward simulation's "variable" option ---- */
+
+/* Law kinds of create_manning_function (process_ManningN_2D.jl:122-134). */
+enum {
+  HG_NLAW_OFF = 0,
+  HG_NLAW_POWER = 1,     /* n_lower + (n_upper - n_lower) (h + eps)^-k              (:154-165) */
+  HG_NLAW_SIGMOID = 2,   /* n_lower + (n_upper - n_lower) / (1 + exp(k (h - h_mid))) (:170-182) */
+  HG_NLAW_INVERSE = 3,   /* n_lower + (n_upper - n_lower) / (1 + k h)                (:139-150) */
+  HG_NLAW_H_UMAG_KS = 4  /* Cheng (2008) friction factor of h, |U|, ks -> n          (:186-213) */
+};
+
+/* Replaces update_ManningN_forward_simulation inside swe_2d_rhs
+ * (semi_discretize_swe_2D.jl:129-135, process_ManningN_2D.jl:102-117): from
+ * now on every RHS evaluation (hg_rhs, hg_step, hg_advance, hg_tsit5_solve,
+ * and the adjoint / JVP) first sets n per cell from the state by the law,
+ * with h = xi + h_still clamped to h_small and |U| = |q| / h, and the inlet
+ * conveyance and friction use that n.
+ * params: {n_lower, n_upper, k, h_mid} (h_mid is read by the sigmoid only;
+ * all four are ignored by HG_NLAW_H_UMAG_KS).  ks: roughness height per cell
+ * (N, caller order; ks_cells of setup_ManningN, process_ManningN_2D.jl:56-61),
+ * required by HG_NLAW_H_UMAG_KS and ignored otherwise.  Errors like the
+ * reference's assertions: k > 0 (HG_ERR_ARG), h_mid > 0 for the sigmoid.
+ * kind = HG_NLAW_OFF restores the n of hg_set_fields.  The law excludes the
+ * UDE closure and Manning n as the active parameter (HG_ERR_STATE). */
+int hg_set_n_law(hg_ctx* ctx, int32_t kind, const double* params, const double* ks);
+
+/* The law at given depths and velocity magnitudes (N each, caller order):
+ * n and the reference's auxiliary outputs h/ks, friction factor and Reynolds
+ * number (update_ManningN_forward_simulation's four results,
+ * process_ManningN_2D.jl:102-117; zero for the depth-only laws).  Any output
+ * may be NULL.  Evaluated on the device by the same code as the RHS. */
+int hg_n_law_eval(hg_ctx* ctx, const double* h, const double* Umag, double* n, double* h_ks, double* f,
+                  double* Re);
+
+/* Manning n per cell (N, caller order) that the RHS uses at state Q (3N):
+ * the law's or the UDE closure's when one is on, else the current field. */
+int hg_cell_n(hg_ctx* ctx, const double* Q, double* n);
+
 /* Self-check with no counterpart in the reference: D = A B on the matrix
  * cores for A (16x4) and B (4x16) row-major, D (16x16) row-major, in the
  * given precision (32 or 64); verifies the lane layout the closure uses. */

@@ -10,6 +10,11 @@ LICENSE_Hydrograd.txt):
   examples/SWE_2D/sensitivity_analysis/ManningN/oneD_channel_with_bump/    -> bump_sens/
   examples/SWE_2D/sensitivity_analysis/ManningN/oneD_channel_uniform_flow/ -> uniform_sens/
     (sensitivity_results.json -> sens.npz, forward_simulation_results.json -> traj.npz)
+  examples/SWE_2D/forward_simulation/oneD_channel_with_bump_ManningN_h/     -> bump_nh/
+  examples/SWE_2D/forward_simulation/Savannah_River_ManningN_ks_h_Umag/     -> savannah_ks/
+    (variable Manning n: the sigmoid n(h) law and the h_Umag_ks law; their
+    case files and initial condition equal those of bump/ and savannah/, so
+    only run_control.json and the truth are copied)
 
 The SRH-2D case files are copied verbatim; the JSON truth / sensitivity
 files (written by JSON3 with shortest round-trip Float64 text, so parsing is
@@ -81,6 +86,12 @@ def main():
         json_to_npz(os.path.join(d, "sensitivity_results.json"), os.path.join(OUT, dst, "sens.npz"))
         # 101 saved Tsit5 frames of the forward pass (3N per frame, t = 0, 2, ..., T)
         json_to_npz(os.path.join(d, "forward_simulation_results.json"), os.path.join(OUT, dst, "traj.npz"))
+    # variable Manning n laws
+    for src, dst in (("oneD_channel_with_bump_ManningN_h", "bump_nh"),
+                     ("Savannah_River_ManningN_ks_h_Umag", "savannah_ks")):
+        d = os.path.join(fw, src)
+        copy_case(d, os.path.join(OUT, dst), ["run_control.json"])
+        json_to_npz(os.path.join(d, "forward_simulation_solution_truth.json"), os.path.join(OUT, dst, "truth.npz"))
     shutil.copyfile(os.path.join(REF, "LICENSE"), os.path.join(OUT, "LICENSE_Hydrograd.txt"))
 
 
