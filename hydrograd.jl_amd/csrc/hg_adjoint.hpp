@@ -199,4 +199,22 @@ __host__ __device__ __forceinline__ void source_vjp(double g, double kn2, double
   hb += cb * (-7.0 / 3.0) * cc / x;
 }
 
+// VJP of the momentum source with FlowResistance friction m q / |q|_eps
+// (semi_discretize_swe_2D.jl:474-475, 521-531); mb += d/dm.  No depth term.
+__host__ __device__ __forceinline__ void source_vjp_mag(double g, double xi, double qx, double qy, double m,
+                                                        double S0x, double S0y, double y1, double y2, double& xib,
+                                                        double& qxb, double& qyb, double& mb, double& S0xb,
+                                                        double& S0yb) {
+  xib += g * S0x * y1 + g * S0y * y2;
+  S0xb = g * xi * y1;
+  S0yb = g * xi * y2;
+  const double fxb = -y1, fyb = -y2;
+  const double rs = 1.0 / sqrt(qx * qx + qy * qy + EPS);
+  const double dq = fxb * qx + fyb * qy;
+  mb += dq * rs;
+  const double sb = -dq * m * rs * rs;  // through |q|_eps
+  qxb += fxb * m * rs + sb * qx * rs;
+  qyb += fyb * m * rs + sb * qy * rs;
+}
+
 }  // namespace hg

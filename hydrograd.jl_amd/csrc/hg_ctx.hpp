@@ -46,6 +46,7 @@ struct StepArgs {
   uint8_t* clamp;        // optional: 1 where the step's dry clamp fired (adjoint recompute)
   double g, half_g, kn2, hsmall, dt;
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
+  const double* fmag; // friction magnitude per cell (FlowResistance closure), or NULL: Manning friction
 };
 
 struct InletArgs {
@@ -117,6 +118,7 @@ struct TileArgs {
   double g, half_g, kn2, hsmall, dt;
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
   int32_t xcd_chunk;  // block -> tile mapping: 0 identity, G > 0 runs of G tiles per XCD in windows of 8G
+  const double* fmag; // friction magnitude per cell (FlowResistance closure), or NULL: Manning friction
 };
 
 // Arguments of the adjoint (VJP) kernel of one RHS evaluation / one step.
@@ -158,6 +160,8 @@ struct AdjArgs {
   const double* totalA;
   double g, half_g, kn2, hsmall, dt;
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
+  const double* fmag; // FlowResistance closure: friction magnitude per cell (else NULL) ...
+  double* fbar;       // ... and += its cotangent
 };
 
 // A variable Manning-n law (kind HG_NLAW_*; 0: off) and its parameters.
@@ -165,6 +169,23 @@ enum { NLAW_OFF = 0, NLAW_POWER = 1, NLAW_SIGMOID = 2, NLAW_INVERSE = 3, NLAW_H_
 struct NLaw {
   int32_t kind;
   double lo, hi, k, hmid;  // n_lower, n_upper, k, h_mid
+};
+
+// The reference's UDE network (create_NN_model, process_UDE.jl:2-65):
+// per hidden layer Dense(in, w, act) -> LayerNorm(w), then Dense(w, 1) and
+// the output bounds lo + (hi - lo) sigmoid(.).  Offsets index the flat
+// parameter vector (see include/hgswe.h for its order).
+constexpr int kLuxMaxLayers = 4, kLuxMaxW = 16, kLuxMaxP = 2048;
+enum { LUX_OFF = 0, LUX_MANNING_H = 1, LUX_MANNING_H_UMAG_KS = 2, LUX_FLOW_RESISTANCE = 3 };
+enum { ACT_RELU = 0, ACT_LEAKYRELU = 1, ACT_SIGMOID = 2, ACT_TANH = 3, ACT_SOFTPLUS = 4 };
+struct LuxNet {
+  int32_t choice, n_in, n_hidden, P;
+  int32_t width[kLuxMaxLayers], act[kLuxMaxLayers];
+  int32_t off_W[kLuxMaxLayers], off_b[kLuxMaxLayers], off_g[kLuxMaxLayers], off_be[kLuxMaxLayers];
+  int32_t off_Wo, off_bo;
+  double lo, hi;             // output bounds
+  double in_lo[3], in_hi[3]; // input bounds: h, then |U| or |q|, then ks
+  double eps;                // LayerNorm epsilon
 };
 
 template <class T>
@@ -251,6 +272,19 @@ struct hg_ctx {
   double* nlaw_ks = nullptr;
   double* nlaw_nbar = nullptr;
   std::vector<void*> nlaw_allocs;
+
+  // the reference's UDE network (choice 0: off), its weights, per-cell ks,
+  // friction magnitudes (FlowResistance), per-step output cotangent and the
+  // accumulated weight cotangent
+  hg::LuxNet lux{};
+  int32_t lux_mw = 0;          // register width of the kernels (4 or 16)
+  double* lux_w = nullptr;     // P weights (double)
+  double* lux_ks = nullptr;
+  double* lux_fmag = nullptr;  // N (FlowResistance)
+  double* lux_obar = nullptr;  // N
+  double* lux_wpart = nullptr; // P x blocks
+  double* lux_wbar = nullptr;  // P
+  std::vector<void*> lux_allocs;
 
   // batches: instance boundaries in caller order, and the per-cell time step
   // (lib order) when per-instance steps are set

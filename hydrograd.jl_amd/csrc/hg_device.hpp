@@ -106,4 +106,13 @@ __host__ __device__ __forceinline__ void friction(double g, double kn2, double h
   fy = c * s * qy;
 }
 
+// FlowResistance friction (semi_discretize_swe_2D.jl:521-531): the UDE's
+// friction magnitude m along the discharge, m q / sqrt(q_x^2 + q_y^2 + eps).
+template <class T>
+__host__ __device__ __forceinline__ void friction_mag(T m, T qx, T qy, T& fx, T& fy) {
+  const T s = sqrt(qx * qx + qy * qy + EPS);
+  fx = m * qx / s;
+  fy = m * qy / s;
+}
+
 }  // namespace hg

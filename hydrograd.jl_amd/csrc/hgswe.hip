@@ -32,6 +32,7 @@
 #include "hg_ude.hpp"
 #include "hg_dual.hpp"  // after the double paths: its sqrt / cbrt overloads must stay out of them
 #include "hg_nlaw.hpp"
+#include "hg_lux.hpp"
 #include "hgswe.h"
 
 using hg::Side;
@@ -197,7 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
   double s1 = 0.0, s2 = 0.0;
   if (h > hsmall) {  // :463-475
     double fx, fy;
-    hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
+    if (a.fmag)
+      hg::friction_mag(a.fmag[i], qx, qy, fx, fy);
+    else
+      hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
     s1 = a.g * xi * a.S0x[i] - fx;
     s2 = a.g * xi * a.S0y[i] - fy;
   }
@@ -408,7 +412,10 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
   const double h = me.h;
   if (h > hsmall) {
     double fx, fy;
-    hg::friction(a.g, a.kn2, hsmall, h, me.hu, me.hv, n_i, fx, fy);
+    if (a.fmag)
+      hg::friction_mag(a.fmag[i], me.hu, me.hv, fx, fy);
+    else
+      hg::friction(a.g, a.kn2, hsmall, h, me.hu, me.hv, n_i, fx, fy);
     s1 = a.g * xi * S0x_i - fx;
     s2 = a.g * xi * S0y_i - fy;
   }
@@ -678,9 +685,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a, const double
     zbb += sb[5 * E + s];
   }
   double S0xb = 0.0, S0yb = 0.0;
-  if (ca.wet)
+  if (ca.wet && a.fmag) {
+    double mb = 0.0;
+    hg::source_vjp_mag(a.g, ca.xi, me.hu, me.hv, a.fmag[i], a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, qxb, qyb, mb,
+                       S0xb, S0yb);
+    a.fbar[i] += mb;
+  } else if (ca.wet) {
     hg::source_vjp(a.g, a.kn2, a.hsmall, ca.xi, me.h, me.hu, me.hv, ca.n, a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, hb,
                    qxb, qyb, nb, S0xb, S0yb);
+  }
   // clamped depth / masked discharge are constants (semi_discretize_swe_2D.jl:101-106)
   double ox = xib, oqx = 0.0, oqy = 0.0;
   if (ca.wet) {
@@ -1140,6 +1153,62 @@ __global__ __launch_bounds__(kBlock) void k_nlaw_jvp(hg::NLawArgs a) {
   a.dn[i] = n.d;
 }
 
+// ---- the reference's UDE network (hg_lux.hpp) ----
+
+// out <- network(features of the state): n, or the friction magnitude.
+template <int MW>
+__global__ __launch_bounds__(kBlock) void k_lux_fwd(hg::LuxArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  double f[3];
+  hg::lux_features<double>(a.net, a.xi[i], a.qx[i], a.qy[i], a.hstill[i], a.hsmall, hg::lux_ks(a, i), f);
+  a.out[i] = hg::lux_eval<MW>(a.net, a.w, f);
+}
+
+// Reverse mode: one wave per work-group, 64 cells at a time (grid-stride).
+// Weight cotangents: wave sums into LDS, then one slot per work-group in
+// wpart; the state cotangent adds through the features (one dual pass per
+// state component the features read).
+template <int MW>
+__global__ __launch_bounds__(64) void k_lux_vjp(hg::LuxArgs a) {
+  __shared__ double acc[hg::kLuxMaxP];
+  const int lane = threadIdx.x;
+  const int P = a.net.P;
+  for (int k = lane; k < P; k += 64) acc[k] = 0.0;
+  __syncthreads();
+  for (int base = blockIdx.x * 64; base < a.N; base += gridDim.x * 64) {
+    const int i = base + lane;
+    const bool valid = i < a.N;
+    double f[3] = {0.0, 0.0, 0.0};
+    double ob = 0.0;
+    if (valid) {
+      ob = a.obar[i];
+      hg::lux_features<double>(a.net, a.xi[i], a.qx[i], a.qy[i], a.hstill[i], a.hsmall, hg::lux_ks(a, i), f);
+    }
+    double fb[3];
+    hg::lux_backward<MW>(a.net, a.w, f, ob, acc, lane, fb);
+    if (valid && ob != 0.0) {
+      using hg::Dual1;
+      const int ns = a.net.choice == hg::LUX_MANNING_H ? 1 : 3;  // ManningN_h reads the depth only
+      for (int st = 0; st < ns; ++st) {
+        const Dual1 xi(a.xi[i], st == 0 ? 1.0 : 0.0), qx(a.qx[i], st == 1 ? 1.0 : 0.0),
+            qy(a.qy[i], st == 2 ? 1.0 : 0.0);
+        Dual1 fd[3];
+        hg::lux_features<Dual1>(a.net, xi, qx, qy, a.hstill[i], a.hsmall, hg::lux_ks(a, i), fd);
+        const double d = fb[0] * fd[0].d + fb[1] * fd[1].d + fb[2] * fd[2].d;
+        if (st == 0)
+          a.ox[i] += d;
+        else if (st == 1)
+          a.oqx[i] += d;
+        else
+          a.oqy[i] += d;
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = lane; k < P; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] = acc[k];
+}
+
 // Layout check of the 16x16x4 matrix-core products used above: D = A B for
 // A (16x4) and B (4x16) given row-major, D returned row-major.
 template <class T>
@@ -1498,6 +1567,7 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
   a.dtc = ctx->dt_cell;
+  a.fmag = ctx->lux.choice == hg::LUX_FLOW_RESISTANCE ? ctx->lux_fmag : nullptr;
   return a;
 }
 
@@ -1544,6 +1614,7 @@ StepArgs make_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
   a.dtc = ctx->dt_cell;
+  a.fmag = ctx->lux.choice == hg::LUX_FLOW_RESISTANCE ? ctx->lux_fmag : nullptr;
   return a;
 }
 
@@ -1671,9 +1742,40 @@ int enqueue_nlaw_n(hg_ctx* ctx, const double* const q[3]) {
   return HG_OK;
 }
 
-// The state-dependent n in force (UDE closure or variable-n law), if any.
+hg::LuxArgs lux_args(hg_ctx* ctx, const double* const q[3]) {
+  hg::LuxArgs a{};
+  a.net = ctx->lux;
+  a.N = int32_t(ctx->N);
+  a.w = ctx->lux_w;
+  a.xi = q[0];
+  a.qx = q[1];
+  a.qy = q[2];
+  a.hstill = ctx->hstill;
+  a.ks = ctx->lux_ks;
+  a.hsmall = ctx->c.h_small;
+  a.out = ctx->lux.choice == hg::LUX_FLOW_RESISTANCE ? ctx->lux_fmag : ctx->n;
+  return a;
+}
+
+int lux_vjp_blocks(const hg_ctx* ctx) { return int(std::min<int64_t>((ctx->N + 63) / 64, 1024)); }
+
+// n (or the friction magnitude) <- the reference's UDE network at state q.
+int enqueue_lux_fwd(hg_ctx* ctx, const double* const q[3]) {
+  if (ctx->lux.choice == hg::LUX_OFF) return HG_OK;
+  const hg::LuxArgs a = lux_args(ctx, q);
+  if (ctx->lux_mw == 4)
+    hipLaunchKernelGGL(k_lux_fwd<4>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_lux_fwd<hg::kLuxMaxW>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  HG_TRY(hipGetLastError());
+  return HG_OK;
+}
+
+// The state-dependent n (or friction) in force: UDE closure, variable-n law
+// or the reference's UDE network, if any.
 int enqueue_cell_n(hg_ctx* ctx, const double* const q[3]) {
   if (ctx->ude_prec) return enqueue_ude_n(ctx, q);
+  if (ctx->lux.choice) return enqueue_lux_fwd(ctx, q);
   return enqueue_nlaw_n(ctx, q);
 }
 
@@ -1807,6 +1909,10 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
   a.hsmall = ctx->c.h_small;
   a.dt = dt;
   a.dtc = ctx->dt_cell;
+  if (ctx->lux.choice == hg::LUX_FLOW_RESISTANCE) {
+    a.fmag = ctx->lux_fmag;
+    a.fbar = ctx->lux_obar;
+  }
   return a;
 }
 
@@ -1819,8 +1925,10 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
                     double* const out[3], double dt, bool step, const double* totalA = nullptr) {
   const bool ude = ctx->ude_prec != 0;
   const bool law = ctx->nlaw.kind != hg::NLAW_OFF;
+  const bool lux = ctx->lux.choice != hg::LUX_OFF;
+  const bool lux_n = lux && ctx->lux.choice != hg::LUX_FLOW_RESISTANCE;  // the network gives n
   // with a state-dependent n, n-bar of this step only
-  double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : ctx->adj_nbar;
+  double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : lux_n ? ctx->lux_obar : ctx->adj_nbar;
   if (ude && ctx->ude_nsets != 1) {
     ctx->err = "the UDE closure's reverse mode takes one weight set";
     return HG_ERR_STATE;
@@ -1834,6 +1942,11 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     int rc = enqueue_nlaw_n(ctx, q);
     if (rc) return rc;
     HG_TRY(hipMemsetAsync(ctx->nlaw_nbar, 0, ctx->N * sizeof(double), ctx->stream));
+  }
+  if (lux) {
+    int rc = enqueue_lux_fwd(ctx, q);
+    if (rc) return rc;
+    HG_TRY(hipMemsetAsync(ctx->lux_obar, 0, ctx->N * sizeof(double), ctx->stream));
   }
   if (ctx->n_inlet > 0 && !totalA) {
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
@@ -1879,6 +1992,21 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     u.oqy = out[2];
     hipLaunchKernelGGL(k_nlaw_vjp, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, u);
   }
+  if (lux) {  // through the network: its output adds to the state cotangent; weight cotangents accumulate
+    hg::LuxArgs u = lux_args(ctx, q);
+    u.obar = ctx->lux_obar;
+    u.ox = out[0];
+    u.oqx = out[1];
+    u.oqy = out[2];
+    u.wpart = ctx->lux_wpart;
+    const int nb = lux_vjp_blocks(ctx);
+    if (ctx->lux_mw == 4)
+      hipLaunchKernelGGL(k_lux_vjp<4>, dim3(nb), dim3(64), 0, ctx->stream, u);
+    else
+      hipLaunchKernelGGL(k_lux_vjp<hg::kLuxMaxW>, dim3(nb), dim3(64), 0, ctx->stream, u);
+    hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, nb, ctx->lux_wpart,
+                       ctx->lux_wbar);
+  }
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
@@ -1894,6 +2022,7 @@ int finish_adjoint(hg_ctx* ctx) {
 
 int zero_adj_accumulators(hg_ctx* ctx) {
   if (ctx->ude_prec) HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
+  if (ctx->lux.choice) HG_TRY(hipMemsetAsync(ctx->lux_wbar, 0, ctx->lux.P * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_S0bar, 0, 2 * ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
@@ -2076,6 +2205,14 @@ int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
     case HG_PARAM_Q: {
       if (np != ctx->n_inlet || !params) { ctx->err = "HG_PARAM_Q expects n_params == n_inlet"; return HG_ERR_ARG; }
       int rc = upload(ctx, ctx->inletQ, params, size_t(np));
+      if (rc) return rc;
+      break;
+    }
+    case HG_PARAM_UDE: {
+      if (np != ctx->lux.P || !params) { ctx->err = "HG_PARAM_UDE expects the network's weights"; return HG_ERR_ARG; }
+      for (int64_t k = 0; k < np; ++k)
+        if (!std::isfinite(params[k])) { ctx->err = "HG_PARAM_UDE: non-finite weight"; return HG_ERR_ARG; }
+      int rc = upload(ctx, ctx->lux_w, params, size_t(np));
       if (rc) return rc;
       break;
     }
@@ -2617,7 +2754,7 @@ int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost,
 
 int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, int32_t n_zones) {
   if (!ctx) return HG_ERR_ARG;
-  if (kind < HG_PARAM_NONE || kind > HG_PARAM_Q) {
+  if (kind < HG_PARAM_NONE || kind > HG_PARAM_UDE) {
     ctx->err = "unknown active parameter kind";
     return HG_ERR_ARG;
   }
@@ -2627,6 +2764,16 @@ int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, 
   }
   if (ctx->nlaw.kind != hg::NLAW_OFF && (kind == HG_PARAM_N_ZONE || kind == HG_PARAM_N_CELL)) {
     ctx->err = "Manning n comes from the variable-n law; it cannot also be the active parameter";
+    return HG_ERR_STATE;
+  }
+  if (ctx->lux.choice == hg::LUX_MANNING_H || ctx->lux.choice == hg::LUX_MANNING_H_UMAG_KS) {
+    if (kind == HG_PARAM_N_ZONE || kind == HG_PARAM_N_CELL) {
+      ctx->err = "Manning n comes from the UDE network; it cannot also be the active parameter";
+      return HG_ERR_STATE;
+    }
+  }
+  if (kind == HG_PARAM_UDE && ctx->lux.choice == hg::LUX_OFF) {
+    ctx->err = "HG_PARAM_UDE needs the UDE network (hg_set_lux_ude)";
     return HG_ERR_STATE;
   }
   if (kind == HG_PARAM_N_ZONE) {
@@ -2762,6 +2909,10 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
     ctx->err = "hg_set_ude: a variable-n law is set";
     return HG_ERR_STATE;
   }
+  if (ctx->lux.choice) {
+    ctx->err = "hg_set_ude: the UDE network is on";
+    return HG_ERR_STATE;
+  }
   const int64_t N = ctx->N;
   std::vector<int32_t> set;
   if (set_of_cell) {
@@ -2863,6 +3014,10 @@ int hg_set_n_law(hg_ctx* ctx, int32_t kind, const double* params, const double* 
     ctx->err = "hg_set_n_law: the UDE closure is on";
     return HG_ERR_STATE;
   }
+  if (ctx->lux.choice) {
+    ctx->err = "hg_set_n_law: the UDE network is on";
+    return HG_ERR_STATE;
+  }
   if (ctx->active == HG_PARAM_N_ZONE || ctx->active == HG_PARAM_N_CELL) {
     ctx->err = "hg_set_n_law: Manning n is the active parameter";
     return HG_ERR_STATE;
@@ -2948,6 +3103,155 @@ int hg_cell_n(hg_ctx* ctx, const double* Q, double* n) {
   return rc;
 }
 
+// Validates a network description and lays out its weights; NULL when valid.
+const char* lux_build(const hg_lux_net* in, hg::LuxNet& L) {
+  L = hg::LuxNet{};
+  if (!in) return "NULL network";
+  if (in->choice < HG_LUX_MANNING_H || in->choice > HG_LUX_FLOW_RESISTANCE) return "unknown UDE choice";
+  if (in->n_hidden < 1 || in->n_hidden > hg::kLuxMaxLayers) return "need 1..4 hidden layers";
+  L.choice = in->choice;
+  L.n_in = in->choice == HG_LUX_MANNING_H ? 1 : in->choice == HG_LUX_MANNING_H_UMAG_KS ? 3 : 2;
+  L.n_hidden = in->n_hidden;
+  int32_t cur = 0, win = L.n_in;
+  for (int l = 0; l < L.n_hidden; ++l) {
+    const int32_t w = in->width[l];
+    if (w < 1 || w > hg::kLuxMaxW) return "hidden widths must be 1..16";
+    if (in->act[l] < HG_ACT_RELU || in->act[l] > HG_ACT_SOFTPLUS) return "unknown activation";
+    L.width[l] = w;
+    L.act[l] = in->act[l];
+    L.off_W[l] = cur;
+    cur += w * win;
+    L.off_b[l] = cur;
+    cur += w;
+    L.off_g[l] = cur;
+    cur += w;
+    L.off_be[l] = cur;
+    cur += w;
+    win = w;
+  }
+  L.off_Wo = cur;
+  cur += win;
+  L.off_bo = cur;
+  cur += 1;
+  L.P = cur;
+  if (L.P > hg::kLuxMaxP) return "too many weights (at most 2048)";
+  L.lo = in->output_bounds[0];
+  L.hi = in->output_bounds[1];
+  const double* b[3] = {in->h_bounds, in->u_bounds, in->ks_bounds};
+  for (int k = 0; k < L.n_in; ++k) {
+    L.in_lo[k] = b[k][0];
+    L.in_hi[k] = b[k][1];
+    if (!(L.in_hi[k] != L.in_lo[k]) || !std::isfinite(L.in_lo[k]) || !std::isfinite(L.in_hi[k]))
+      return "input bounds must be finite and distinct";
+  }
+  if (L.choice == HG_LUX_FLOW_RESISTANCE) {  // |q| bounds from h and velocity bounds (process_FlowResistance_UDE.jl:14)
+    L.in_lo[1] = in->h_bounds[0] * in->u_bounds[0];
+    L.in_hi[1] = in->h_bounds[1] * in->u_bounds[1];
+    if (!(L.in_hi[1] != L.in_lo[1])) return "h and velocity bounds give an empty |q| range";
+  }
+  L.eps = in->ln_epsilon > 0.0 ? in->ln_epsilon : hg::kLuxLayerNormEps;
+  return nullptr;
+}
+
+int64_t hg_lux_n_params(const hg_lux_net* net) {
+  hg::LuxNet L;
+  return lux_build(net, L) ? -1 : int64_t(L.P);
+}
+
+int hg_set_lux_ude(hg_ctx* ctx, const hg_lux_net* net, const double* params, const double* ks) {
+  if (!ctx) return HG_ERR_ARG;
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the network
+  for (void* p : ctx->lux_allocs) {
+    ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), p));
+    HG_TRY(hipFree(p));
+  }
+  ctx->lux_allocs.clear();
+  ctx->lux = hg::LuxNet{};
+  ctx->lux_w = ctx->lux_ks = ctx->lux_fmag = ctx->lux_obar = ctx->lux_wpart = ctx->lux_wbar = nullptr;
+  ctx->lux_mw = 0;
+  if (ctx->active == HG_PARAM_UDE) ctx->active = HG_PARAM_NONE;
+  if (!net) {  // off: back to the per-cell n of hg_set_fields
+    HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    return HG_OK;
+  }
+  hg::LuxNet L;
+  const char* why = lux_build(net, L);
+  if (why) {
+    ctx->err = std::string("hg_set_lux_ude: ") + why;
+    return HG_ERR_ARG;
+  }
+  if (!params || (L.choice == HG_LUX_MANNING_H_UMAG_KS && !ks)) {
+    ctx->err = "hg_set_lux_ude: weights (and ks for ManningN_h_Umag_ks) required";
+    return HG_ERR_ARG;
+  }
+  for (int32_t k = 0; k < L.P; ++k)
+    if (!std::isfinite(params[k])) {
+      ctx->err = "hg_set_lux_ude: non-finite weight";
+      return HG_ERR_ARG;
+    }
+  if (ctx->ude_prec || ctx->nlaw.kind != hg::NLAW_OFF) {
+    ctx->err = "hg_set_lux_ude: the MLP closure or a variable-n law is on";
+    return HG_ERR_STATE;
+  }
+  if (L.choice != HG_LUX_FLOW_RESISTANCE && (ctx->active == HG_PARAM_N_ZONE || ctx->active == HG_PARAM_N_CELL)) {
+    ctx->err = "hg_set_lux_ude: Manning n is the active parameter";
+    return HG_ERR_STATE;
+  }
+  const int64_t N = ctx->N;
+  const size_t nalloc0 = ctx->allocs.size();
+  int rc = dalloc(ctx, &ctx->lux_w, size_t(L.P));
+  if (!rc) rc = upload(ctx, ctx->lux_w, params, size_t(L.P));
+  if (!rc && L.choice == HG_LUX_MANNING_H_UMAG_KS) {
+    rc = dalloc(ctx, &ctx->lux_ks, N);
+    if (!rc) rc = cell_in(ctx, ctx->lux_ks, ks);
+  }
+  if (!rc && L.choice == HG_LUX_FLOW_RESISTANCE) rc = dalloc(ctx, &ctx->lux_fmag, N);
+  if (!rc) rc = dalloc(ctx, &ctx->lux_obar, N);
+  if (!rc) rc = dalloc(ctx, &ctx->lux_wpart, size_t(L.P) * lux_vjp_blocks(ctx));
+  if (!rc) rc = dalloc(ctx, &ctx->lux_wbar, size_t(L.P));
+  ctx->lux_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
+  if (rc) {
+    ctx->lux_w = ctx->lux_ks = ctx->lux_fmag = ctx->lux_obar = ctx->lux_wpart = ctx->lux_wbar = nullptr;
+    return rc;
+  }
+  HG_TRY(hipMemsetAsync(ctx->lux_wbar, 0, size_t(L.P) * sizeof(double), ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  int maxw = 0;
+  for (int l = 0; l < L.n_hidden; ++l) maxw = std::max(maxw, int(L.width[l]));
+  ctx->lux_mw = maxw <= 4 ? 4 : hg::kLuxMaxW;
+  ctx->lux = L;
+  return HG_OK;
+}
+
+int hg_lux_eval(hg_ctx* ctx, const double* Q, double* out) {
+  if (!ctx || !Q || !out) return ctx ? (ctx->err = "hg_lux_eval: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (!ctx->lux.choice) {
+    ctx->err = "hg_lux_eval: the UDE network is off";
+    return HG_ERR_STATE;
+  }
+  int rc = state_in(ctx, Q, 0);
+  if (rc) return rc;
+  ctx->cur = 0;
+  const double* q[3] = {ctx->st[0][0], ctx->st[0][1], ctx->st[0][2]};
+  rc = enqueue_lux_fwd(ctx, q);
+  if (!rc) rc = cell_out(ctx, ctx->lux.choice == hg::LUX_FLOW_RESISTANCE ? ctx->lux_fmag : ctx->n, out);
+  return rc;
+}
+
+int hg_lux_grad(hg_ctx* ctx, double* params_bar) {
+  if (!ctx || !params_bar) return HG_ERR_ARG;
+  if (!ctx->lux.choice) {
+    ctx->err = "hg_lux_grad: the UDE network is off";
+    return HG_ERR_STATE;
+  }
+  HG_TRY(hipMemcpyAsync(params_bar, ctx->lux_wbar, size_t(ctx->lux.P) * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
 int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D) {
   if (!A || !B || !D || (precision != 32 && precision != 64)) return HG_ERR_ARG;
   const size_t bytes = 512 * sizeof(double);
@@ -2998,6 +3302,10 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
   }
   if (ctx->ude_prec) {
     ctx->err = "hg_jvp: forward mode through the UDE closure is not provided";
+    return HG_ERR_STATE;
+  }
+  if (ctx->lux.choice) {
+    ctx->err = "hg_jvp: forward mode through the UDE network is not provided";
     return HG_ERR_STATE;
   }
   const int64_t N = ctx->N, G = ctx->G;

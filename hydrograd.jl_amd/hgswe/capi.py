@@ -16,10 +16,16 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhgswe.so")
 _LIB = None
 
 HG_OK, HG_ERR_ARG, HG_ERR_HIP, HG_ERR_MESH, HG_ERR_CONVEYANCE, HG_ERR_NONFINITE, HG_ERR_STATE = range(7)
-HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = range(5)
+HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q, HG_PARAM_UDE = range(6)
 HG_REORDER = 1
 HG_CELL_KERNEL = 2
 HG_NLAW_OFF, HG_NLAW_POWER, HG_NLAW_SIGMOID, HG_NLAW_INVERSE, HG_NLAW_H_UMAG_KS = range(5)
+HG_LUX_OFF, HG_LUX_MANNING_H, HG_LUX_MANNING_H_UMAG_KS, HG_LUX_FLOW_RESISTANCE = range(4)
+# UDE_choice names (process_ManningN_2D.jl:216-273, process_FlowResistance_UDE.jl:3-46)
+LUX_CHOICES = {"ManningN_h": HG_LUX_MANNING_H, "ManningN_h_Umag_ks": HG_LUX_MANNING_H_UMAG_KS,
+               "FlowResistance": HG_LUX_FLOW_RESISTANCE}
+# get_activation names (process_UDE.jl:93-107)
+ACTIVATIONS = {"relu": 0, "leakyrelu": 1, "sigmoid": 2, "tanh": 3, "softplus": 4}
 # create_manning_function's type names (process_ManningN_2D.jl:122-134)
 NLAW_KINDS = {"power_law": HG_NLAW_POWER, "sigmoid": HG_NLAW_SIGMOID, "inverse": HG_NLAW_INVERSE,
               "h_Umag_ks": HG_NLAW_H_UMAG_KS}
@@ -31,9 +37,33 @@ EXPORTS = [
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
-    "hg_n_law_eval", "hg_cell_n",
+    "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
 ]
 UDE_WEIGHTS = 353
+
+
+class hg_lux_net(C.Structure):
+    _fields_ = [("choice", C.c_int32), ("n_hidden", C.c_int32), ("width", C.c_int32 * 4), ("act", C.c_int32 * 4),
+                ("output_bounds", C.c_double * 2), ("h_bounds", C.c_double * 2), ("u_bounds", C.c_double * 2),
+                ("ks_bounds", C.c_double * 2), ("ln_epsilon", C.c_double)]
+
+
+def lux_net(choice, hidden, activations, output_bounds, h_bounds, u_bounds=(0.0, 1.0), ks_bounds=(0.0, 1.0),
+            ln_epsilon=0.0):
+    """hg_lux_net from the reference's UDE_NN_config fields (names or codes)."""
+    hidden = list(hidden)
+    acts = [ACTIVATIONS.get(a, a) for a in activations]
+    if len(hidden) != len(acts) or not 1 <= len(hidden) <= 4:
+        raise ValueError("one activation per hidden layer, 1..4 layers")
+    w = (C.c_int32 * 4)(*(hidden + [0] * (4 - len(hidden))))
+    a = (C.c_int32 * 4)(*(acts + [0] * (4 - len(acts))))
+    d2 = lambda b: (C.c_double * 2)(float(b[0]), float(b[1]))
+    return hg_lux_net(LUX_CHOICES.get(choice, choice), len(hidden), w, a, d2(output_bounds), d2(h_bounds),
+                      d2(u_bounds), d2(ks_bounds), float(ln_epsilon))
+
+
+def lux_n_params(net):
+    return int(lib().hg_lux_n_params(C.byref(net)))
 
 
 class HGError(RuntimeError):
@@ -82,6 +112,11 @@ def lib():
         L.hg_set_n_law.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.hg_n_law_eval.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.hg_cell_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_lux_n_params.argtypes = [C.c_void_p]
+        L.hg_lux_n_params.restype = C.c_int64
+        L.hg_set_lux_ude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_lux_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hg_lux_grad.argtypes = [C.c_void_p, C.c_void_p]
         L.hg_mfma_probe.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.hg_tsit5_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_double,
                                      C.c_int64, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
@@ -347,6 +382,35 @@ class Context:
         n = np.empty(self.N)
         self._chk(lib().hg_cell_n(self.h, _ptr(np.ascontiguousarray(Q, dtype=np.float64)), _ptr(n)))
         return n
+
+    def set_lux_ude(self, net, params=None, ks=None):
+        """The reference's UDE network (hg_lux_net from lux_net(...)) with its
+        flat weights; net=None turns it off."""
+        if net is None:
+            self._chk(lib().hg_set_lux_ude(self.h, C.c_void_p(0), C.c_void_p(0), C.c_void_p(0)))
+            self.lux_P = 0
+            return
+        P = lux_n_params(net)
+        w = np.ascontiguousarray(params, dtype=np.float64)
+        if P < 0 or w.size != P:
+            raise ValueError(f"the network takes {P} weights, got {w.size}")
+        ks = None if ks is None else np.ascontiguousarray(ks, dtype=np.float64)
+        if ks is not None and ks.size != self.N:
+            raise ValueError("ks needs one value per cell")
+        self._net = net
+        self._chk(lib().hg_set_lux_ude(self.h, C.byref(net), _ptr(w), _ptr(ks)))
+        self.lux_P = P
+
+    def lux_eval(self, Q):
+        """The network's output per cell (n, or the friction magnitude) at state Q."""
+        out = np.empty(self.N)
+        self._chk(lib().hg_lux_eval(self.h, _ptr(np.ascontiguousarray(Q, dtype=np.float64)), _ptr(out)))
+        return out
+
+    def lux_grad(self):
+        g = np.empty(self.lux_P)
+        self._chk(lib().hg_lux_grad(self.h, _ptr(g)))
+        return g
 
     def batch_set_dt(self, dts):
         """Per-instance time steps of a batch (None: back to the scalar dt)."""

@@ -27,7 +27,7 @@ export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f, tsit5_
 const libhgswe = joinpath(@__DIR__, "..", "lib", "libhgswe.so")
 
 const HG_BC = Dict("inlet-q" => Int32(0), "exit-h" => Int32(1), "wall" => Int32(2), "symm" => Int32(3))
-const HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q = Int32.(0:4)
+const HG_PARAM_NONE, HG_PARAM_ZB, HG_PARAM_N_ZONE, HG_PARAM_N_CELL, HG_PARAM_Q, HG_PARAM_UDE = Int32.(0:5)
 
 struct HgMesh
     n_cells::Int64
@@ -61,6 +61,20 @@ mutable struct Context
     ptr::Ptr{Cvoid}
     N::Int
     active::Int32
+    lux_hidden::Int  # hidden layers of the UDE network when its weights are params_vector
+end
+
+# hg_lux_net: the reference's UDE network (create_NN_model, process_UDE.jl:2-65)
+struct HgLuxNet
+    choice::Int32
+    n_hidden::Int32
+    width::NTuple{4,Int32}
+    act::NTuple{4,Int32}
+    output_bounds::NTuple{2,Float64}
+    h_bounds::NTuple{2,Float64}
+    u_bounds::NTuple{2,Float64}
+    ks_bounds::NTuple{2,Float64}
+    ln_epsilon::Float64
 end
 
 check(ctx, rc) = rc == 0 || error("libhgswe: " * unsafe_string(ccall((:hg_last_error, libhgswe), Cstring, (Ptr{Cvoid},), ctx)))
@@ -125,7 +139,7 @@ function context(p)
                        out, mesh, bc, k, Int32(1))
             check(C_NULL, rc)
         end
-        ctx = Context(out[], d.N, HG_PARAM_NONE)
+        ctx = Context(out[], d.N, HG_PARAM_NONE, 0)
         finalizer(x -> ccall((:hg_destroy, libhgswe), Cint, (Ptr{Cvoid},), x.ptr), ctx)
         S0 = p.S0_cells
         rc = ccall((:hg_set_fields, libhgswe), Cint,
@@ -137,6 +151,7 @@ function context(p)
         check(ctx.ptr, rc)
         set_active!(ctx, p)
         set_n_law!(ctx, p)
+        set_lux_ude!(ctx, p)
         ctx
     end
 end
@@ -178,13 +193,73 @@ function set_active!(ctx, p)
     ctx.active = kind
 end
 
+# The reference's UDE network on the device (UDE_choice ManningN_h,
+# ManningN_h_Umag_ks or FlowResistance; semi_discretize_swe_2D.jl:145-158,
+# 521-531): its weights are params_vector (HG_PARAM_UDE).
+const HG_LUX = Dict("ManningN_h" => Int32(1), "ManningN_h_Umag_ks" => Int32(2), "FlowResistance" => Int32(3))
+const HG_ACT = Dict("relu" => Int32(0), "leakyrelu" => Int32(1), "sigmoid" => Int32(2), "tanh" => Int32(3),
+                    "softplus" => Int32(4))
+function set_lux_ude!(ctx, p)
+    s = p.settings
+    (s.bPerform_UDE && haskey(HG_LUX, s.UDE_settings.UDE_choice)) || return nothing
+    cfg = s.UDE_settings.UDE_NN_config
+    hidden = Int32.(cfg["hidden_layers"])
+    acts = [HG_ACT[a] for a in cfg["activations"]]
+    pad(v, z) = ntuple(i -> i <= length(v) ? v[i] : z, 4)
+    b2(k) = haskey(cfg, k) ? (Float64(cfg[k][1]), Float64(cfg[k][2])) : (0.0, 1.0)
+    ub = haskey(cfg, "Umag_bounds") ? b2("Umag_bounds") : b2("velocity_magnitude_bounds")
+    net = HgLuxNet(HG_LUX[s.UDE_settings.UDE_choice], Int32(length(hidden)), pad(hidden, Int32(0)),
+                   pad(acts, Int32(0)), b2("output_bounds"), b2("h_bounds"), ub, b2("ks_bounds"), 0.0)
+    w = lux_flat(p.ude_model_params, length(hidden))
+    ks = s.UDE_settings.UDE_choice == "ManningN_h_Umag_ks" ? Vector{Float64}(p.ks_cells) : C_NULL
+    check(ctx.ptr, ccall((:hg_set_lux_ude, libhgswe), Cint, (Ptr{Cvoid}, Ref{HgLuxNet}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, Ref(net), w, ks))
+    check(ctx.ptr, ccall((:hg_set_active_param, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Int32}, Int32),
+                         ctx.ptr, HG_PARAM_UDE, C_NULL, Int32(0)))
+    ctx.active = HG_PARAM_UDE
+    ctx.lux_hidden = length(hidden)
+    return nothing
+end
+
+# Lux parameters (layer_1 Dense, layer_2 LayerNorm, ..., last Dense) in the
+# library's flat order, by field name.
+function lux_flat(ps, nh)
+    v = Float64[]
+    for l in 1:nh
+        d = getproperty(ps, Symbol(:layer_, 2l - 1))
+        n = getproperty(ps, Symbol(:layer_, 2l))
+        append!(v, vec(d.weight), vec(d.bias), vec(n.scale), vec(n.bias))
+    end
+    o = getproperty(ps, Symbol(:layer_, 2nh + 1))
+    append!(v, vec(o.weight), vec(o.bias))
+    return v
+end
+
+# The flat weight cotangent back into the structure of ps (a NamedTuple of
+# arrays or a ComponentArray; every array is overwritten).
+function lux_unflat(ps, g, nh)
+    pb = deepcopy(ps)
+    k = 0
+    put!(a) = (n = length(a); a .= reshape(g[k+1:k+n], size(a)); k += n)
+    for l in 1:nh
+        d = getproperty(pb, Symbol(:layer_, 2l - 1))
+        n = getproperty(pb, Symbol(:layer_, 2l))
+        put!(d.weight); put!(d.bias); put!(n.scale); put!(n.bias)
+    end
+    o = getproperty(pb, Symbol(:layer_, 2nh + 1))
+    put!(o.weight); put!(o.bias)
+    return pb
+end
+
+flatparams(ctx, p) = ctx.active == HG_PARAM_UDE ? lux_flat(p, ctx.lux_hidden) : Vector{Float64}(p)
+
 nodual(x) = eltype(x) <: Float64 || error("HydrogradHIP: only Float64 arrays cross the C ABI (got $(eltype(x)))")
 
 function swe_2d_rhs(dQdt::AbstractVector, Q::AbstractVector, params_vector::AbstractVector, t::Float64, p_extra)
     nodual(Q); nodual(params_vector)
     ctx = context(p_extra)
     q = Vector{Float64}(Q)
-    prm = Vector{Float64}(params_vector)
+    prm = flatparams(ctx, params_vector)
     out = p_extra.bInPlaceODE ? dQdt : similar(q)
     check(ctx.ptr, ccall((:hg_rhs, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}),
                          ctx.ptr, q, prm, length(prm), out))
@@ -202,7 +277,7 @@ function custom_ODE_update_cells(ode_f::DeviceRHS, Q, params_vector, t, dt, swe2
     nodual(Q); nodual(params_vector)
     ctx = context(swe2d_extra_params)
     q = Vector{Float64}(Q)
-    prm = Vector{Float64}(params_vector)
+    prm = flatparams(ctx, params_vector)
     check(ctx.ptr, ccall((:hg_step, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64),
                          ctx.ptr, q, prm, length(prm), dt, 1))
     return q
@@ -214,7 +289,7 @@ function custom_ODE_solve(ode_f::DeviceRHS, Q0, params_vector, swe2d_extra_param
     c = swe2d_extra_params.swe_2D_constants
     steps = c.tspan[1]:c.dt:c.tspan[2]
     ctx = context(swe2d_extra_params)
-    prm = Vector{Float64}(params_vector)
+    prm = flatparams(ctx, params_vector)
     check(ctx.ptr, ccall((:hg_set_params, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), ctx.ptr, prm, length(prm)))
     check(ctx.ptr, ccall((:hg_state_upload, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, Vector{Float64}(Q0)))
     sol = Matrix{Float64}(undef, length(Q0), length(steps))
@@ -232,7 +307,7 @@ end
 function tsit5_solve(ode_f::DeviceRHS, Q0, params_vector, tspan, nsave; abstol = 1e-6, reltol = 1e-3, dt0 = 1e-3)
     nodual(Q0); nodual(params_vector)
     ctx = context(ode_f.p)
-    prm = Vector{Float64}(params_vector)
+    prm = flatparams(ctx, params_vector)
     sol = Matrix{Float64}(undef, length(Q0), nsave + 1)
     stats = zeros(Int64, 3)
     check(ctx.ptr, ccall((:hg_tsit5_solve, libhgswe), Cint,
@@ -249,7 +324,7 @@ end
 function sensitivity_jvp(ode_f::DeviceRHS, Q0, params_vector, dparams, dt, nsteps; dQ0 = nothing)
     nodual(Q0); nodual(params_vector)
     ctx = context(ode_f.p)
-    prm = Vector{Float64}(params_vector)
+    prm = flatparams(ctx, params_vector)
     dp = dparams === nothing ? C_NULL : Vector{Float64}(dparams)
     d0 = dQ0 === nothing ? C_NULL : Vector{Float64}(dQ0)
     QN = zeros(length(Q0)); dQN = zeros(length(Q0))
@@ -289,8 +364,13 @@ end
 # Cotangent of params_vector from the per-cell n, zb and per-inlet Q
 # cotangents (semi_discretize_swe_2D.jl:114-199: zb -> bed data, zone n ->
 # cell n gather, Q -> inlet totals).
-function params_bar(ctx, p_extra, prm, nb, zbb, qb)
+function params_bar(ctx, p_extra, prm, nb, zbb, qb, p = nothing)
     k = ctx.active
+    if k == HG_PARAM_UDE  # weight cotangent of the network (hg_lux_grad)
+        g = zeros(length(prm))
+        check(ctx.ptr, ccall((:hg_lux_grad, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, g))
+        return p === nothing ? g : lux_unflat(p, g, ctx.lux_hidden)
+    end
     k == HG_PARAM_ZB && return zbb
     k == HG_PARAM_N_CELL && return nb
     k == HG_PARAM_Q && return qb
@@ -308,7 +388,7 @@ end
 dense(x, n) = x isa AbstractZero ? zeros(n) : Vector{Float64}(unthunk(x))
 
 # Pullback of one RHS evaluation: (Q, params) -> dQdt.
-function rhs_pullback(ctx, p_extra, q, prm, ybar)
+function rhs_pullback(ctx, p_extra, q, prm, ybar, p = nothing)
     N = ctx.N
     nI = length(p_extra.inletQ_TotalQ)
     Qb = zeros(3N); nb = zeros(N); zbb = zeros(N); qb = zeros(max(nI, 1))
@@ -316,15 +396,15 @@ function rhs_pullback(ctx, p_extra, q, prm, ybar)
                          (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                           Ptr{Float64}, Ptr{Float64}),
                          ctx.ptr, q, prm, length(prm), dense(ybar, 3N), Qb, nb, zbb, qb))
-    return Qb, params_bar(ctx, p_extra, prm, nb, zbb, qb[1:nI])
+    return Qb, params_bar(ctx, p_extra, prm, nb, zbb, qb[1:nI], p)
 end
 
 function ChainRulesCore.rrule(f::DeviceRHS, u, p, t)
     y = f(u, p, t)
     ctx = context(f.p)
-    q = Vector{Float64}(u); prm = Vector{Float64}(p)
+    q = Vector{Float64}(u); prm = flatparams(ctx, p)
     function DeviceRHS_pullback(ybar)
-        Qb, pb = rhs_pullback(ctx, f.p, q, prm, ybar)
+        Qb, pb = rhs_pullback(ctx, f.p, q, prm, ybar, p)
         return NoTangent(), Qb, pb, NoTangent()
     end
     return y, DeviceRHS_pullback
@@ -333,9 +413,9 @@ end
 function ChainRulesCore.rrule(::typeof(swe_2d_rhs), dQdt, Q, params_vector, t, p_extra)
     y = swe_2d_rhs(dQdt, Q, params_vector, t, p_extra)
     ctx = context(p_extra)
-    q = Vector{Float64}(Q); prm = Vector{Float64}(params_vector)
+    q = Vector{Float64}(Q); prm = flatparams(ctx, params_vector)
     function swe_2d_rhs_pullback(ybar)
-        Qb, pb = rhs_pullback(ctx, p_extra, q, prm, ybar)
+        Qb, pb = rhs_pullback(ctx, p_extra, q, prm, ybar, params_vector)
         return NoTangent(), NoTangent(), Qb, pb, NoTangent(), NoTangent()
     end
     return y, swe_2d_rhs_pullback
@@ -347,7 +427,7 @@ function ChainRulesCore.rrule(::typeof(custom_ODE_update_cells), ode_f::DeviceRH
                               swe2d_extra_params)
     y = custom_ODE_update_cells(ode_f, Q, params_vector, t, dt, swe2d_extra_params)
     ctx = context(swe2d_extra_params)
-    q = Vector{Float64}(Q); prm = Vector{Float64}(params_vector)
+    q = Vector{Float64}(Q); prm = flatparams(ctx, params_vector)
     N = ctx.N
     nI = length(swe2d_extra_params.inletQ_TotalQ)
     function update_cells_pullback(ybar)
@@ -356,7 +436,7 @@ function ChainRulesCore.rrule(::typeof(custom_ODE_update_cells), ode_f::DeviceRH
                              (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64, Int64, Ptr{Float64},
                               Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
                              ctx.ptr, q, prm, length(prm), dt, 1, 1, dense(ybar, 3N), Qb, nb, zbb, qb))
-        pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb[1:nI])
+        pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb[1:nI], params_vector)
         return NoTangent(), NoTangent(), Qb, pb, NoTangent(), NoTangent(), NoTangent()
     end
     return y, update_cells_pullback

@@ -46,7 +46,9 @@ enum {
   HG_PARAM_ZB = 1,     /* params = zb per cell; bed data rebuilt (process_bed_2D.jl:46-66) */
   HG_PARAM_N_ZONE = 2, /* params = Manning n per zone; n_cell = params[zone[i]] (process_ManningN_2D.jl:71-98) */
   HG_PARAM_N_CELL = 3, /* params = Manning n per cell */
-  HG_PARAM_Q = 4       /* params = total discharge per inlet-Q boundary */
+  HG_PARAM_Q = 4,      /* params = total discharge per inlet-Q boundary */
+  HG_PARAM_UDE = 5     /* params = the UDE network's weights (hg_set_lux_ude; the reference
+                          passes them as params_vector, semi_discretize_swe_2D.jl:145-158) */
 };
 
 /* hg_create flags */
@@ -300,6 +302,62 @@ int hg_n_law_eval(hg_ctx* ctx, const double* h, const double* Umag, double* n, d
 /* Manning n per cell (N, caller order) that the RHS uses at state Q (3N):
  * the law's or the UDE closure's when one is on, else the current field. */
 int hg_cell_n(hg_ctx* ctx, const double* Q, double* n);
+
+/* ---- The reference's UDE network (UDE_choice, Lux model of create_NN_model) ---- */
+
+enum {
+  HG_LUX_OFF = 0,
+  HG_LUX_MANNING_H = 1,         /* n = NN(h)                 (process_ManningN_2D.jl:216-273) */
+  HG_LUX_MANNING_H_UMAG_KS = 2, /* n = NN(h, |U|, ks)        (same) */
+  HG_LUX_FLOW_RESISTANCE = 3    /* friction magnitude = NN(h, |q|) (process_FlowResistance_UDE.jl:3-46,
+                                   friction m q / sqrt(q_x^2 + q_y^2 + eps), semi_discretize_swe_2D.jl:521-531) */
+};
+/* Hidden-layer activations (get_activation, process_UDE.jl:93-107; NNlib definitions). */
+enum { HG_ACT_RELU = 0, HG_ACT_LEAKYRELU = 1, HG_ACT_SIGMOID = 2, HG_ACT_TANH = 3, HG_ACT_SOFTPLUS = 4 };
+
+/* The network of create_NN_model (process_UDE.jl:2-65): per hidden layer
+ * Dense(in, width[l], act[l]) -> LayerNorm(width[l]); then Dense(width[last], 1)
+ * and lo + (hi - lo) sigmoid(.) with output_bounds = {lo, hi}.  Inputs are
+ * normalised to [-1, 1] by their bounds: h by h_bounds; |U| by u_bounds
+ * (Umag_bounds, ManningN_h_Umag_ks); |q| by {h_lo u_lo, h_hi u_hi}
+ * (velocity_magnitude_bounds, FlowResistance); ks by ks_bounds.
+ * ln_epsilon: LayerNorm epsilon, 0 for Lux's default Float32(1e-5). */
+typedef struct {
+  int32_t choice;          /* HG_LUX_* */
+  int32_t n_hidden;        /* 1..4 hidden layers */
+  int32_t width[4];        /* 1..16 units each */
+  int32_t act[4];          /* HG_ACT_* each */
+  double output_bounds[2];
+  double h_bounds[2];
+  double u_bounds[2];
+  double ks_bounds[2];
+  double ln_epsilon;
+} hg_lux_net;
+
+/* Number of weights of the network (or -1 if the description is invalid).
+ * Flat order, per hidden layer l: Dense weight (width[l] x in, column-major,
+ * as Lux stores it), Dense bias, LayerNorm scale, LayerNorm bias; then the
+ * output Dense weight (width[last]) and bias.  The Julia binding builds this
+ * vector from Lux's parameter NamedTuple by field name. */
+int64_t hg_lux_n_params(const hg_lux_net* net);
+
+/* Turns the network on (net NULL: off, restoring the n of hg_set_fields).
+ * From then on every RHS evaluation first evaluates the network per cell
+ * from the state (clamped h, masked q) and uses its output as Manning n
+ * (inlet conveyance and friction) or, for HG_LUX_FLOW_RESISTANCE, as the
+ * friction magnitude (the inlet conveyance keeps the field n).  params: the
+ * weights (hg_lux_n_params of them); ks: per cell (caller order), required by
+ * HG_LUX_MANNING_H_UMAG_KS.  Excludes the MLP closure, the variable-n laws
+ * and n as the active parameter.  With HG_PARAM_UDE active, the params of
+ * each call replace the weights. */
+int hg_set_lux_ude(hg_ctx* ctx, const hg_lux_net* net, const double* params, const double* ks);
+
+/* The network's output per cell (N, caller order) at state Q (3N). */
+int hg_lux_eval(hg_ctx* ctx, const double* Q, double* out);
+
+/* Cotangent of the weights accumulated by the last hg_rhs_vjp, hg_adjoint or
+ * hg_grad_probes call with the network on. */
+int hg_lux_grad(hg_ctx* ctx, double* params_bar);
 
 /* Self-check with no counterpart in the reference: D = A B on the matrix
  * cores for A (16x4) and B (4x16) row-major, D (16x16) row-major, in the

@@ -138,6 +138,7 @@ template <class T>
 struct Fields {
   std::vector<T> hstill, hstill_ghost, zb, zb_ghost, S0x, S0y, n;
   std::vector<T> inletQ, exitWSE;
+  std::vector<T> fmag;  // FlowResistance UDE: friction magnitude per cell (empty: Manning friction)
 };
 
 template <class T>
@@ -345,16 +346,28 @@ inline void friction_terms(const Consts& c, const T& h, const T& q_x, const T& q
   fy = c.g * (n * n) / (c.k_n * c.k_n) / dpow(h + c.h_small, 7.0 / 3.0) * smooth_sqrt(q_x * q_x + q_y * q_y) * q_y;
 }
 
+// FlowResistance branch (:521-531): the UDE gives the friction magnitude m,
+//   mag_q = smooth_sqrt(q_x^2 + q_y^2), friction = m * q / mag_q.
+template <class T>
+inline void friction_terms_mag(const T& m, const T& q_x, const T& q_y, T& fx, T& fy) {
+  const T mag_q = smooth_sqrt(q_x * q_x + q_y * q_y);
+  fx = m * q_x / mag_q;
+  fy = m * q_y / mag_q;
+}
+
 template <class T>
 void compute_source_terms(const Mesh& m, const Consts& c, const std::vector<T>& xi, const std::vector<T>& h,
                           const std::vector<T>& q_x, const std::vector<T>& q_y, const std::vector<T>& S0x,
                           const std::vector<T>& S0y, const std::vector<T>& ManningN_cells,
-                          std::vector<T>& out /*3N*/) {
+                          std::vector<T>& out /*3N*/, const std::vector<T>* fmag = nullptr) {
   const int64_t N = m.N;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     T fx, fy;
-    friction_terms(c, h[i], q_x[i], q_y[i], ManningN_cells[i], fx, fy);
+    if (fmag)
+      friction_terms_mag((*fmag)[i], q_x[i], q_y[i], fx, fy);
+    else
+      friction_terms(c, h[i], q_x[i], q_y[i], ManningN_cells[i], fx, fy);
     const bool above_small_h = val(h[i]) > c.h_small;
     out[i] = T(0.0);
     out[N + i] = bool_times(above_small_h, c.g * xi[i] * S0x[i] - fx);
@@ -412,7 +425,7 @@ void swe_2d_rhs(const Mesh& m, const Consts& c, const Fields<T>& p, const std::v
 
   std::vector<T> inv(3 * N), src(3 * N);
   compute_inviscid_fluxes(m, c, xi, p.hstill, h, q_x, q_y, xi_ghost, p.hstill_ghost, gh, p.zb, p.zb_ghost, inv);
-  compute_source_terms(m, c, xi, h, q_x, q_y, p.S0x, p.S0y, p.n, src);
+  compute_source_terms(m, c, xi, h, q_x, q_y, p.S0x, p.S0y, p.n, src, p.fmag.empty() ? nullptr : &p.fmag);
   dQdt.resize(3 * N);
   for (int64_t k = 0; k < 3 * N; ++k) dQdt[k] = inv[k] + src[k];  // (:264)
 }

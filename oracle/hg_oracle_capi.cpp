@@ -21,6 +21,7 @@ thread_local std::string g_err;
 
 struct FieldPtrs {  // plain-pointer form of the parameter bundle
   const double *hstill, *hstill_ghost, *zb, *zb_ghost, *S0x, *S0y, *n, *inletQ, *exitWSE;
+  const double* fmag;  // nullable: FlowResistance friction magnitudes
 };
 
 template <class T>
@@ -40,6 +41,7 @@ Fields<T> make_fields(const Mesh& m, const FieldPtrs& f) {
   p.n = cp(f.n, m.N);
   p.inletQ = cp(f.inletQ, m.n_inlet);
   p.exitWSE = cp(f.exitWSE, m.n_exit);
+  if (f.fmag) p.fmag = cp(f.fmag, m.N);
   return p;
 }
 

@@ -36,7 +36,7 @@ class _Consts(C.Structure):
 
 class _Fields(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in
-                ("hstill", "hstill_ghost", "zb", "zb_ghost", "S0x", "S0y", "n", "inletQ", "exitWSE")]
+                ("hstill", "hstill_ghost", "zb", "zb_ghost", "S0x", "S0y", "n", "inletQ", "exitWSE", "fmag")]
 
 
 def lib():
@@ -94,6 +94,12 @@ class Oracle:
                 a = np.zeros(1)
             self.keep["f_" + name] = a
             ptrs.append(C.c_void_p(a.ctypes.data))
+        # fmag: FlowResistance friction magnitudes per cell (None: Manning friction)
+        fm = vals.get("fmag")
+        if fm is not None:
+            fm = np.ascontiguousarray(fm, dtype=D)
+            self.keep["f_fmag"] = fm
+        ptrs.append(C.c_void_p(0) if fm is None else C.c_void_p(fm.ctypes.data))
         self.fields = _Fields(*ptrs)
 
     def _check(self, rc):
