@@ -1,8 +1,10 @@
-"""Parameter inversion over the device path: the reference's inversion loss
-on the final state (compute_loss_inversion, swe_2D_inversion.jl:247-483),
-its gradient through the library's adjoint (hg_adjoint), and the Adam loop
-the reference drives through Optimization.jl (swe_2D_inversion.jl:600-732,
-Optimisers.jl Adam).
+"""Parameter inversion and UDE training over the device path: the
+reference's inversion loss on the final state (compute_loss_inversion,
+swe_2D_inversion.jl:247-483) and its UDE loss (compute_loss_UDE,
+swe_2D_UDE.jl:522-599: the same WSE and uv terms), their gradients through
+the library's adjoint (hg_adjoint; hg_lux_grad for the network weights), and
+the Adam loop the reference drives through Optimization.jl
+(swe_2D_inversion.jl:600-732, swe_2D_UDE.jl:360-420; Optimisers.jl Adam).
 
 The forward model is the custom Euler driver (custom_ODE_solve) the adjoint
 differentiates; the loss terms and their normalisations follow the
@@ -195,6 +197,44 @@ def invert(ctx, case, truth, params0, active, dt, nsteps, kck=50, lr=1e-3, iters
     res = InversionResult(params=p)
     for it in range(iters):
         L, parts, g = loss_and_grad(ctx, case, truth, p, active, dt, nsteps, kck, settings, zone=zone, G=G)
+        res.loss.append(L)
+        res.parts.append(parts)
+        res.history.append(p.copy())
+        if callback:
+            callback(it, L, p)
+        p = opt.step(p, g)
+    res.params = p
+    return res
+
+
+def ude_settings():
+    """The UDE loss: WSE and uv terms only (UDE_bWSE_loss, UDE_b_uv_loss)."""
+    return LossSettings(wse=True, uv=True, bound=False, slope=False)
+
+
+def ude_loss_and_grad(ctx, case, truth, weights, dt, nsteps, kck, settings=None):
+    """compute_loss_UDE on the final state after nsteps Euler steps and its
+    gradient w.r.t. the network weights (HG_PARAM_UDE must be active).
+    truth: WSE_truth, u_truth, v_truth and optionally zb_cell_truth (the bed
+    the reference adds to the predicted depth)."""
+    settings = settings or ude_settings()
+    QN = ctx.step(case.Q0, dt, nsteps, params=weights)
+    zb = truth.get("zb_cell_truth", case.zb)
+    total, parts, QN_bar, _, _ = final_state_loss(case, QN, truth, settings, zb=zb)
+    ctx.adjoint(case.Q0, dt, nsteps, kck, QN_bar, params=weights)
+    return total, parts, ctx.lux_grad()
+
+
+def train_ude(ctx, case, truth, weights0, dt, nsteps, kck=50, lr=0.1, iters=100, settings=None, callback=None):
+    """Adam on the reference's UDE loss (the training loop of
+    swe_2D_UDE.jl:360-420 with its Adam stage); returns the weights and loss
+    history."""
+    ctx.set_active_param(capi.HG_PARAM_UDE)
+    opt = Adam(lr=lr)
+    p = np.array(weights0, dtype=np.float64)
+    res = InversionResult(params=p)
+    for it in range(iters):
+        L, parts, g = ude_loss_and_grad(ctx, case, truth, p, dt, nsteps, kck, settings)
         res.loss.append(L)
         res.parts.append(parts)
         res.history.append(p.copy())

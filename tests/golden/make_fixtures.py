@@ -15,6 +15,13 @@ LICENSE_Hydrograd.txt):
     (variable Manning n: the sigmoid n(h) law and the h_Umag_ks law; their
     case files and initial condition equal those of bump/ and savannah/, so
     only run_control.json and the truth are copied)
+  examples/SWE_2D/sensitivity_analysis/ManningN/Savana_River/               -> savannah_sens/
+    (ForwardDiff's dQ(200 s)/dn_zone on the Savannah mesh; case files and
+    initial condition equal those of savannah/: run_control.json, sens.npz)
+  examples/SWE_2D/UDE/ManningN/oneD_channel_with_bump/training/             -> ude_bump/
+    (the reference's UDE training run on the n(h) bump truth: its 100 loss
+    values, WSE and uv parts, and the predicted final states h, u = q_x/h,
+    v = q_y/h of iterations 0, 1, 25, 50 and 99; run_control.json)
 
 The SRH-2D case files are copied verbatim; the JSON truth / sensitivity
 files (written by JSON3 with shortest round-trip Float64 text, so parsing is
@@ -92,6 +99,22 @@ def main():
         d = os.path.join(fw, src)
         copy_case(d, os.path.join(OUT, dst), ["run_control.json"])
         json_to_npz(os.path.join(d, "forward_simulation_solution_truth.json"), os.path.join(OUT, dst, "truth.npz"))
+    # Savannah River sensitivities
+    d = os.path.join(sa, "Savana_River")
+    copy_case(d, os.path.join(OUT, "savannah_sens"), ["run_control.json"])
+    json_to_npz(os.path.join(d, "sensitivity_results.json"), os.path.join(OUT, "savannah_sens", "sens.npz"))
+    # UDE training history
+    d = os.path.join(EX, "UDE", "ManningN", "oneD_channel_with_bump", "training")
+    copy_case(d, os.path.join(OUT, "ude_bump"), ["run_control.json"])
+    with open(os.path.join(d, "UDE_results_losses_and_ODE_solution.json")) as fh:
+        r = json.load(fh)
+    its = np.array([0, 1, 25, 50, 99])
+    arrs = {k: np.asarray(r[k], dtype=np.float64) for k in ("loss_totals", "loss_pred_WSEs", "loss_pred_uvs",
+                                                           "ManningN_cells", "ManningN_cell_truth")}
+    for k in ("inverted_hs", "inverted_us", "inverted_vs"):
+        arrs[k] = np.asarray(r[k], dtype=np.float64)[its]
+    arrs["iterations"] = its
+    np.savez_compressed(os.path.join(OUT, "ude_bump", "history.npz"), **arrs)
     shutil.copyfile(os.path.join(REF, "LICENSE"), os.path.join(OUT, "LICENSE_Hydrograd.txt"))
 
 

@@ -109,3 +109,49 @@ def test_jvp_matches_forwarddiff_fixture():
             errs.append(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     say(f"  jvp vs ForwardDiff fixture: max rel {max(errs):.2e}")
     assert max(errs) < 1e-3
+
+
+def test_savannah_sensitivities_match_forwarddiff_fixture(tmp_path):
+    """dQ(200 s)/dn_zone on the Savannah River mesh (1306 mixed cells, 6
+    zones): the device Dual run and the device adjoint against ForwardDiff
+    through Tsit5 (sensitivity_analysis/ManningN/Savana_River).  The Euler
+    runs converge to the fixture at first order in dt (L1 error halves with
+    dt); the max-norm error sits at the fixture's own reltol 1e-3 level."""
+    from test_oracle_golden import savannah_ic_path
+
+    with open(os.path.join(GOLDEN, "savannah_sens", "run_control.json")) as fh:
+        rc = json.load(fh)
+    zn = np.asarray(rc["sensitivity_analysis_options"]["sensitivity_ManningN_values"], dtype=np.float64)
+    c = load_golden_case("savannah", ic=savannah_ic_path(tmp_path))
+    T = rc["time_settings"]["tspan"][1]
+    S = np.load(os.path.join(GOLDEN, "savannah_sens", "sens.npz"))["sensitivity_results"].reshape(len(zn), 3 * c.N).T
+    N = c.N
+    ctx = capi.Context(c)
+    ctx.set_active_param(capi.HG_PARAM_N_ZONE, zone=c.zone, n_zones=len(zn))
+    l1 = {}
+    for dt in (0.05, 0.02):
+        nsteps = int(round(T / dt))
+        e1, emax = [], []
+        J = np.zeros((3 * N, len(zn)))
+        for z in range(len(zn)):
+            dz = np.zeros(len(zn))
+            dz[z] = 1.0
+            _, J[:, z] = ctx.jvp(c.Q0, dt, nsteps, params=zn, dparams=dz)
+            ref = S[:, z]
+            if not np.any(c.zone == z):
+                assert np.max(np.abs(ref)) == 0.0 and np.max(np.abs(J[:, z])) == 0.0
+                continue
+            for k in range(3):
+                a, b = J[k * N:(k + 1) * N, z], ref[k * N:(k + 1) * N]
+                e1.append(np.mean(np.abs(a - b)) / np.mean(np.abs(b)))
+                emax.append(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        l1[dt] = max(e1)
+        say(f"  Savannah dt {dt}: jvp vs ForwardDiff L1 rel {max(e1):.2e}, max rel {max(emax):.2e}")
+        assert max(emax) < 1e-2
+    assert l1[0.02] < 1e-3 and l1[0.02] < 0.6 * l1[0.05]
+    # the adjoint of w . Q(T) gives the same zone gradient as the tangents
+    w = np.random.default_rng(0).normal(size=3 * N)
+    _, nb, _, _ = ctx.adjoint(c.Q0, 0.02, int(round(T / 0.02)), 50, w, params=zn)
+    gz = np.bincount(c.zone, weights=nb, minlength=len(zn))
+    assert np.allclose(gz, w @ J, rtol=1e-9, atol=1e-12 * np.max(np.abs(gz)))
+    say(f"  adjoint zone gradient vs fixture w.S: {np.max(np.abs(gz - w @ S)) / np.max(np.abs(w @ S)):.2e}")
