@@ -105,6 +105,11 @@ def measure_gradient(args, capi):
     c, n_true, probe = gradient_case(args.grad_cells, args.grad_steps)
     ctx = capi.Context(c)
     ctx.set_active_param(capi.HG_PARAM_N_CELL)
+    if not (args.no_tune or args.tile_config):
+        ctx.set_params(n_true)
+        ctx.upload(c.Q0)
+        T, x, _ = ctx.autotune(c.dt, 64)
+        print(f"# gradient autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     QT = ctx.step(c.Q0, c.dt, args.grad_steps, params=n_true)
     target = QT[probe] + c.hstill[probe] + c.zb[probe]
     ctx.grad_probes(c.Q0, c.dt, 20, args.grad_kck, probe, target, params=c.n)  # warm-up
@@ -268,12 +273,17 @@ def main():
         T, x, _ = ctx.autotune(case.dt, 64)
         print(f"# autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     ctx.advance(case.dt, args.warmup)
-    # timed region: K device-resident steps, synchronized on both sides
+    ctx.advance_timed(case.dt, 0)  # creates the dispatch events (no steps)
+    # timed region: K device-resident steps (hipGraph replays), synchronized
+    # on both sides
     t0 = time.perf_counter()
     ctx.advance(case.dt, args.steps)
     el = time.perf_counter() - t0
     value = case.N * args.steps / el
-    ms_step, ms_kernel = ctx.time_advance(case.dt, 100, 2)
+    # the step kernel's duration: K more launches of the same step, each
+    # dispatch carrying its own start / stop events (graph kernel nodes
+    # cannot), the per-dispatch figure rocprofv3's kernel trace reports
+    ms_kernel = ctx.advance_timed(case.dt, args.steps)
     bpc = bytes_per_cell_step(case.mesh)
     achieved = bpc * case.N / (ms_kernel * 1e-3) / 1e9
     out = {

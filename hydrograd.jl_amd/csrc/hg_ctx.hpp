@@ -254,6 +254,9 @@ struct hg_ctx {
   uint8_t* adj_mask = nullptr;  // kck x N clamp flags
   double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
   double* adj_slot = nullptr;   // 6 x maxdeg x N per-slot cotangents (ELL layout)
+  int32_t* adj_items = nullptr;  // face-once work list: boundary slots and owned interior slots
+  int32_t* adj_mirror = nullptr; // interior slot -> the same face's slot in the neighbour
+  int32_t adj_n_items = 0;       // 0: no face-once list (a cell pair shares more than one face)
   double* adj_totalA = nullptr; // kck x n_inlet conveyances recorded by a segment's recompute
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
 
@@ -298,6 +301,9 @@ struct hg_ctx {
   // Tsit5 work buffers: 7 stages + y, ynew, ytmp (3N each), norm partials
   double* rk = nullptr;
   double* rk_part = nullptr;
+
+  // events attached to timed step dispatches (hg_advance_timed), reused
+  std::vector<hipEvent_t> ev_pool;
 
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;

@@ -14,6 +14,7 @@
 // because a ghost's internal cell is the cell that owns the boundary face;
 // only the inlet conveyance (a sequential sum over an inlet's faces) needs a
 // separate tiny kernel per step.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -554,13 +555,14 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
   return c;
 }
 
+// The VJP of the face in slot s (cell i = s % N, local face k = s / N),
+// written to slot s; with `mirror`, an interior face's other side goes to
+// slot mirror[s] of the neighbour too (face-once), else the neighbour's own
+// slot computes it.
 template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) {
+__device__ __forceinline__ void adj_face(const hg::AdjArgs& a, double* sb, int64_t s, const int32_t* mirror) {
   const int64_t E = int64_t(a.maxdeg) * a.N;
-  const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (s >= E) return;
-  const int i = int(s % a.N), k = int(s / a.N);
-  if (k >= a.deg[i]) return;
+  const int i = int(s % a.N);
   const double hsmall = a.hsmall;
   const CellAdj ca = cell_adj<STEP>(a, i);
   const hg::CellQ& me = ca.me;
@@ -580,6 +582,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) 
     hg::SideBar Ab, Bb;
     hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
     U = Ab;
+    if (mirror) {  // the same face seen from j: the B-side cotangent (mirror identity)
+      const int64_t m = mirror[s];
+      sb[0 * E + m] = Bb.xi;
+      sb[1 * E + m] = Bb.h;
+      sb[2 * E + m] = Bb.hu;
+      sb[3 * E + m] = Bb.hv;
+      sb[4 * E + m] = 0.0;
+      sb[5 * E + m] = 0.0;
+    }
   } else {
     const int gi = ~j;
     const int kind = a.g_kind[gi];
@@ -664,6 +675,27 @@ __global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) 
   sb[3 * E + s] = U.hv;
   sb[4 * E + s] = nb;
   sb[5 * E + s] = zbb;
+}
+
+// One work-item per (cell, face) slot: every interior face evaluated from
+// both of its sides.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) {
+  const int64_t E = int64_t(a.maxdeg) * a.N;
+  const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (s >= E) return;
+  if (int(s / a.N) >= a.deg[int(s % a.N)]) return;
+  adj_face<STEP>(a, sb, s, nullptr);
+}
+
+// Face-once: one work-item per boundary slot and per interior face (listed
+// at its owner's slot), writing both sides.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_face(hg::AdjArgs a, double* sb, const int32_t* items, int n_items,
+                                                     const int32_t* mirror) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_items) return;
+  adj_face<STEP>(a, sb, items[t], mirror);
 }
 
 template <bool STEP>
@@ -1659,31 +1691,35 @@ int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
 // Tile kernel instances: T in {128, 256, 512} cells per tile (one work-item
 // per cell), MD = 3 (triangles), 4 (quads or mixed), 8 (up to octagons).
 template <int T, int MD>
-void launch_tile_tm(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+void launch_tile_tm(hg_ctx* ctx, const hg::TileArgs& a, bool step, hipEvent_t e0, hipEvent_t e1) {
   const size_t lds = tile_lds_bytes(T, MD);
-  if (step)
+  if (e0)  // start / stop events attached to the dispatch itself
+    hipExtLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, e0, e1, 0, a);
+  else if (step)
     hipLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
   else
     hipLaunchKernelGGL((k_swe_tile<false, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
 }
 
 template <int T>
-void launch_tile_t(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+void launch_tile_t(hg_ctx* ctx, const hg::TileArgs& a, bool step, hipEvent_t e0, hipEvent_t e1) {
   if (ctx->tileMD == 3)
-    launch_tile_tm<T, 3>(ctx, a, step);
+    launch_tile_tm<T, 3>(ctx, a, step, e0, e1);
   else if (ctx->tileMD == 4)
-    launch_tile_tm<T, 4>(ctx, a, step);
+    launch_tile_tm<T, 4>(ctx, a, step, e0, e1);
   else
-    launch_tile_tm<T, 8>(ctx, a, step);
+    launch_tile_tm<T, 8>(ctx, a, step, e0, e1);
 }
 
-void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step) {
+// Launches the tiled step (or RHS) kernel; with events e0 / e1 (a step
+// only) they record the dispatch's own start and end.
+void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (ctx->tileT == 128)
-    launch_tile_t<128>(ctx, a, step);
+    launch_tile_t<128>(ctx, a, step, e0, e1);
   else if (ctx->tileT == 256)
-    launch_tile_t<256>(ctx, a, step);
+    launch_tile_t<256>(ctx, a, step, e0, e1);
   else
-    launch_tile_t<512>(ctx, a, step);
+    launch_tile_t<512>(ctx, a, step, e0, e1);
 }
 
 hg::UdeArgs ude_args(hg_ctx* ctx, const double* const q[3]) {
@@ -1781,7 +1817,7 @@ int enqueue_cell_n(hg_ctx* ctx, const double* const q[3]) {
 
 // Enqueue one step (or one RHS evaluation) reading state `src`.
 int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0 = nullptr, double* o1 = nullptr,
-                 double* o2 = nullptr) {
+                 double* o2 = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   {
     const double* q[3] = {ctx->st[src][0], ctx->st[src][1], ctx->st[src][2]};
     int rc = enqueue_cell_n(ctx, q);
@@ -1791,14 +1827,16 @@ int enqueue_step(hg_ctx* ctx, int src, int dst, double dt, bool step, double* o0
     if (ctx->n_inlet > 0 && !ctx->fuse_inlet)
       hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
     hg::TileArgs a = make_tile_args(ctx, src, dst, dt, o0, o1, o2);
-    launch_tile(ctx, a, step);
+    launch_tile(ctx, a, step, e0, e1);
     HG_TRY(hipGetLastError());
     return HG_OK;
   }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args(ctx, src));
   StepArgs a = make_args(ctx, src, dst, dt, o0, o1, o2);
-  if (step)
+  if (e0 && step)
+    hipExtLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, e0, e1, 0, a);
+  else if (step)
     hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL(k_swe_step<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
@@ -1841,6 +1879,45 @@ int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3],
 
 // ---- adjoint host side
 int advance_async(hg_ctx* ctx, double dt, int64_t nsteps);
+int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms);
+
+// Face-once work list of the adjoint: every boundary slot, and each interior
+// face once, at the slot of its owner, the side with the smaller caller
+// index (so the list, and the gradient bits, do not depend on renumbering);
+// mirror[s] is the same face's slot in the neighbour.  Left empty when a
+// pair of cells shares more than one face (the slot kernel then runs).
+int build_adj_items(hg_ctx* ctx) {
+  const int64_t N = ctx->N, E = int64_t(ctx->maxdeg) * N;
+  std::vector<int32_t> items, mirror(E, -1);
+  items.reserve(size_t(E / 2 + ctx->G + 1));
+  for (int k = 0; k < ctx->maxdeg; ++k)
+    for (int64_t i = 0; i < N; ++i) {
+      if (k >= ctx->h_deg[i]) continue;
+      const int64_t s = int64_t(k) * N + i;
+      const int32_t j = ctx->h_nbr[s];
+      if (j < 0) {
+        items.push_back(int32_t(s));
+        continue;
+      }
+      int64_t m = -1;
+      int hits = 0;
+      for (int kk = 0; kk < ctx->h_deg[j]; ++kk)
+        if (ctx->h_nbr[int64_t(kk) * N + j] == int32_t(i)) {
+          m = int64_t(kk) * N + j;
+          ++hits;
+        }
+      if (hits != 1) return HG_OK;  // not a conforming pair: keep the slot kernel
+      mirror[s] = int32_t(m);
+      if (ctx->perm[i] < ctx->perm[j]) items.push_back(int32_t(s));
+    }
+  int rc = dalloc(ctx, &ctx->adj_items, std::max<size_t>(items.size(), 1));
+  if (!rc) rc = upload(ctx, ctx->adj_items, items.data(), items.size());
+  if (!rc) rc = dalloc(ctx, &ctx->adj_mirror, size_t(E));
+  if (!rc) rc = upload(ctx, ctx->adj_mirror, mirror.data(), size_t(E));
+  if (rc) return rc;
+  ctx->adj_n_items = int32_t(items.size());
+  return HG_OK;
+}
 
 int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
   const int64_t N = ctx->N;
@@ -1854,6 +1931,7 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (!rc) rc = dalloc(ctx, &ctx->adj_vbar, std::max<int64_t>(ctx->G, 1));
     if (!rc) rc = dalloc(ctx, &ctx->adj_Qbar, std::max<int32_t>(ctx->n_inlet, 1));
     if (!rc) rc = dalloc(ctx, &ctx->adj_slot, size_t(6) * ctx->maxdeg * N);
+    if (!rc) rc = build_adj_items(ctx);
     if (rc) return rc;
   }
   if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
@@ -1957,11 +2035,20 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   if (totalA) a.totalA = totalA;
   const int64_t E = int64_t(ctx->maxdeg) * ctx->N;
   const int gs = int((E + kBlock - 1) / kBlock);
+  const int gi = (ctx->adj_n_items + kBlock - 1) / kBlock;
   if (step) {
-    hipLaunchKernelGGL(k_adj_slot<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+    if (ctx->adj_n_items)
+      hipLaunchKernelGGL(k_adj_face<true>, dim3(gi), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot, ctx->adj_items,
+                         ctx->adj_n_items, ctx->adj_mirror);
+    else
+      hipLaunchKernelGGL(k_adj_slot<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
   } else {
-    hipLaunchKernelGGL(k_adj_slot<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
+    if (ctx->adj_n_items)
+      hipLaunchKernelGGL(k_adj_face<false>, dim3(gi), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot, ctx->adj_items,
+                         ctx->adj_n_items, ctx->adj_mirror);
+    else
+      hipLaunchKernelGGL(k_adj_slot<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
     hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
   }
   if (ctx->n_inlet > 0)
@@ -2346,6 +2433,32 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
   return HG_OK;
 }
 
+// nsteps single-launch steps, each step kernel with its own start / stop
+// events on its dispatch; *kernel_ms = their mean duration.  Synchronous.
+int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
+  std::vector<hipEvent_t>& ev = ctx->ev_pool;  // grown once (at least 2 x 2048), reused by later calls
+  int rc = HG_OK;
+  const size_t need = size_t(2 * std::max<int64_t>(nsteps, 2048));
+  while (!rc && ev.size() < need) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) rc = HG_ERR_HIP;
+    else ev.push_back(e);
+  }
+  for (int64_t k = 0; k < nsteps && !rc; ++k) {
+    rc = enqueue_step(ctx, ctx->cur, 1 - ctx->cur, dt, true, nullptr, nullptr, nullptr, ev[2 * k], ev[2 * k + 1]);
+    ctx->cur = 1 - ctx->cur;
+  }
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+  double sum = 0.0;
+  for (int64_t k = 0; k < nsteps && !rc; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) rc = HG_ERR_HIP;
+    sum += ms;
+  }
+  if (!rc && kernel_ms) *kernel_ms = nsteps > 0 ? sum / double(nsteps) : 0.0;
+  return rc;
+}
+
 }  // namespace
 
 // ================================================================ C ABI ====
@@ -2600,7 +2713,13 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   ctx->tiled = !(flags & HG_CELL_KERNEL) && maxdeg <= 8;
   if (ctx->tiled) {
     ctx->tileMD = maxdeg <= 3 ? 3 : (maxdeg <= 4 ? 4 : 8);
-    const int T = tile_lds_bytes(256, ctx->tileMD) <= ctx->lds_max ? 256 : 128;
+    // 256-cell tiles, unless that leaves fewer than two work-groups per CU
+    // (a small mesh fills the chip better with 128-cell tiles)
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu <= 0)
+      ncu = 256;
+    const bool small = ctx->N < int64_t(2) * ncu * 256;
+    const int T = (!small && tile_lds_bytes(256, ctx->tileMD) <= ctx->lds_max) ? 256 : 128;
     rc = setup_tiles(ctx, T);
     if (rc) return fail(rc);
   }
@@ -2722,6 +2841,7 @@ int hg_destroy(hg_ctx* ctx) {
   if (!ctx) return HG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2842,6 +2962,19 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps) {
   }
   int rc = clear_flags(ctx);
   if (!rc) rc = advance_async(ctx, dt, nsteps);
+  if (!rc) rc = enqueue_finite_check(ctx);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+int hg_advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
+  if (!ctx || nsteps < 0) return HG_ERR_ARG;
+  if (!ctx->fields_set) {
+    ctx->err = "hg_set_fields has not been called";
+    return HG_ERR_STATE;
+  }
+  int rc = clear_flags(ctx);
+  if (!rc) rc = advance_timed(ctx, dt, nsteps, kernel_ms);
   if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
@@ -3604,22 +3737,9 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   float ms = 0.f;
   HG_TRY(hipEventElapsedTime(&ms, e0, e1));
   if (ms_per_step) *ms_per_step = double(ms) / double(reps * nsteps);
-  if (ms_step_kernel && !rc) {  // the step kernel alone, back-to-back on the same stream
-    const int K = 200;
-    HG_TRY(hipEventRecord(e0, ctx->stream));
-    for (int k = 0; k < K; ++k) {
-      if (ctx->tiled) {
-        launch_tile(ctx, make_tile_args(ctx, k & 1, (k + 1) & 1, dt), true);
-      } else {
-        StepArgs a = make_args(ctx, k & 1, (k + 1) & 1, dt);
-        hipLaunchKernelGGL(k_swe_step<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
-      }
-    }
-    HG_TRY(hipEventRecord(e1, ctx->stream));
-    HG_TRY(hipEventSynchronize(e1));
-    HG_TRY(hipEventElapsedTime(&ms, e0, e1));
-    *ms_step_kernel = double(ms) / K;
+  if (ms_step_kernel && !rc) {  // the step kernel alone: events attached to each of K dispatches
     ctx->cur = 0;
+    rc = advance_timed(ctx, dt, 200, ms_step_kernel);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -166,9 +166,19 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps);
  * custom_ODE_solvers.jl:39). */
 int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
 
+/* Measurement helper with no counterpart in the reference: hg_advance with
+ * every step launched singly and start / stop events attached to each step
+ * kernel's dispatch (hipExtLaunchKernelGGL); *kernel_ms is the mean duration
+ * of those dispatches, the figure a profiler's kernel trace reports.  The
+ * events are created by the first call (for at least 2048 steps) and reused,
+ * so later calls of up to that many steps create none; nsteps = 0 only
+ * creates them. */
+int hg_advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms);
+
 /* Timing helper: runs hg_advance(dt, nsteps) `reps` times on the device
  * state and reports the mean time per step measured with HIP events on the
- * context's stream, plus the mean duration of the step kernel alone. */
+ * context's stream, plus the mean duration of the step kernel alone (200
+ * single launches with events on their dispatches). */
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel);
 
