@@ -108,7 +108,7 @@ def measure_gradient(args, capi):
     if not (args.no_tune or args.tile_config):
         ctx.set_params(n_true)
         ctx.upload(c.Q0)
-        T, x, _ = ctx.autotune(c.dt, 64)
+        T, x, _ = ctx.autotune(c.dt, 200)
         print(f"# gradient autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     QT = ctx.step(c.Q0, c.dt, args.grad_steps, params=n_true)
     target = QT[probe] + c.hstill[probe] + c.zb[probe]
@@ -150,7 +150,7 @@ def measure_batch(args, capi, single_value):
           f"(built in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     ctx = capi.Context(B)
     ctx.upload(B.Q0)
-    T, x, _ = ctx.autotune(B.dt, 64)
+    T, x, _ = ctx.autotune(B.dt, 100)
     ctx.advance(B.dt, args.warmup)
     t0 = time.perf_counter()
     ctx.advance(B.dt, args.batch_steps)
@@ -270,7 +270,7 @@ def main():
         T, G = (int(v) for v in args.tile_config.split(","))
         ctx.set_tile_config(T, G)
     elif not args.no_tune:  # launch shapes are bit-identical; keep the fastest
-        T, x, _ = ctx.autotune(case.dt, 64)
+        T, x, _ = ctx.autotune(case.dt, 200)
         print(f"# autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     ctx.advance(case.dt, args.warmup)
     ctx.advance_timed(case.dt, 0)  # creates the dispatch events (no steps)

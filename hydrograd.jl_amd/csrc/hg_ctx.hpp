@@ -226,6 +226,7 @@ struct hg_ctx {
   double* in_L53 = nullptr;
   int32_t* in_next = nullptr;   // inlet face -> next face of the same cell in that inlet, or -1
   int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
+  bool inlets_disjoint = true;  // no cell sits on two inlets: the inlet adjoint runs one work-group per inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
   int32_t* zone = nullptr;
@@ -253,10 +254,6 @@ struct hg_ctx {
   double* adj_seg = nullptr;    // (kck + 1) x 3N states of one checkpoint segment
   uint8_t* adj_mask = nullptr;  // kck x N clamp flags
   double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
-  double* adj_slot = nullptr;   // 6 x maxdeg x N per-slot cotangents (ELL layout)
-  int32_t* adj_items = nullptr;  // face-once work list: boundary slots and owned interior slots
-  int32_t* adj_mirror = nullptr; // interior slot -> the same face's slot in the neighbour
-  int32_t adj_n_items = 0;       // 0: no face-once list (a cell pair shares more than one face)
   double* adj_totalA = nullptr; // kck x n_inlet conveyances recorded by a segment's recompute
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
 

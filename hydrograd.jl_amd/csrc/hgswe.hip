@@ -520,15 +520,8 @@ __global__ __launch_bounds__(kBlock) void k_copy_probe(int64_t n2, const double2
 //   * the source term (bed slope, Manning friction) is the cell's own.
 // Derivatives with respect to n, the exit-H bed term, S0 and the inlet
 // ghost velocity are accumulated / written for the follow-up kernels.
-// Reverse mode of one RHS evaluation (or one step), in two passes so that
-// the face VJPs, the bulk of the work, run one work-item per (cell, face)
-// slot.  k_adj_slot: for slot (i, k), the VJP of the flux through that face
-// seen from cell i, with the combined cotangent of both cells (the face's
-// contribution to cell i's raw inputs, by the mirror identity of the
-// pairwise fluxes), routed through the ghost rule on boundary faces; it
-// writes 6 values per slot (cotangents of xi, h, hu, hv, n, zb).
-// k_adj_cell: per cell, the slots summed in local face order, the source
-// VJP, the clamp and the identity part of the step.
+// (One work-item per (cell, face) slot with a second pass for the sums was
+// slower at every size: the slot values round-trip through memory.)
 struct CellAdj {  // one cell's quantities the adjoint passes need
   hg::CellQ me;
   bool wet;
@@ -555,22 +548,22 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
   return c;
 }
 
-// The VJP of the face in slot s (cell i = s % N, local face k = s / N),
-// written to slot s; with `mirror`, an interior face's other side goes to
-// slot mirror[s] of the neighbour too (face-once), else the neighbour's own
-// slot computes it.
+// The VJP of the face in slot s (cell i = s % N, local face k = s / N) seen
+// from cell i: its contribution to the cotangent of the cell's state (U),
+// of n and of the exit-H bed; the neighbour computes the other side.
+// (Writing both sides from one work-item halves the face evaluations but
+// scatters the neighbour's writes; on a 6.4M-cell batch that is 30 % slower.)
 template <bool STEP>
-__device__ __forceinline__ void adj_face(const hg::AdjArgs& a, double* sb, int64_t s, const int32_t* mirror) {
-  const int64_t E = int64_t(a.maxdeg) * a.N;
-  const int i = int(s % a.N);
+__device__ __forceinline__ void adj_face(const hg::AdjArgs& a, const CellAdj& ca, int64_t s, hg::SideBar& U,
+                                         double& nb, double& zbb) {
   const double hsmall = a.hsmall;
-  const CellAdj ca = cell_adj<STEP>(a, i);
   const hg::CellQ& me = ca.me;
   const int j = a.nbr[s];
   const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
   const double fi = -(L / ca.A);
-  hg::SideBar U{0.0, 0.0, 0.0, 0.0};
-  double nb = 0.0, zbb = 0.0;
+  U = hg::SideBar{0.0, 0.0, 0.0, 0.0};
+  nb = 0.0;
+  zbb = 0.0;
   if (j >= 0) {
     const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
     double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
@@ -582,15 +575,6 @@ __device__ __forceinline__ void adj_face(const hg::AdjArgs& a, double* sb, int64
     hg::SideBar Ab, Bb;
     hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
     U = Ab;
-    if (mirror) {  // the same face seen from j: the B-side cotangent (mirror identity)
-      const int64_t m = mirror[s];
-      sb[0 * E + m] = Bb.xi;
-      sb[1 * E + m] = Bb.h;
-      sb[2 * E + m] = Bb.hu;
-      sb[3 * E + m] = Bb.hv;
-      sb[4 * E + m] = 0.0;
-      sb[5 * E + m] = 0.0;
-    }
   } else {
     const int gi = ~j;
     const int kind = a.g_kind[gi];
@@ -669,53 +653,43 @@ __device__ __forceinline__ void adj_face(const hg::AdjArgs& a, double* sb, int64
       }
     }
   }
-  sb[0 * E + s] = U.xi;
-  sb[1 * E + s] = U.h;
-  sb[2 * E + s] = U.hu;
-  sb[3 * E + s] = U.hv;
-  sb[4 * E + s] = nb;
-  sb[5 * E + s] = zbb;
-}
-
-// One work-item per (cell, face) slot: every interior face evaluated from
-// both of its sides.
-template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_slot(hg::AdjArgs a, double* sb) {
-  const int64_t E = int64_t(a.maxdeg) * a.N;
-  const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (s >= E) return;
-  if (int(s / a.N) >= a.deg[int(s % a.N)]) return;
-  adj_face<STEP>(a, sb, s, nullptr);
-}
-
-// Face-once: one work-item per boundary slot and per interior face (listed
-// at its owner's slot), writing both sides.
-template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_face(hg::AdjArgs a, double* sb, const int32_t* items, int n_items,
-                                                     const int32_t* mirror) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n_items) return;
-  adj_face<STEP>(a, sb, items[t], mirror);
 }
 
 template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a, const double* sb) {
+__device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, const CellAdj& ca, double xib,
+                                                double hb, double qxb, double qyb, double nb, double zbb);
+
+// Reverse mode of one RHS evaluation or one step, one work-item per cell:
+// the VJPs of the cell's faces (each seen from the cell), summed in local
+// face order, then the source VJP, the clamp and the identity part of the
+// step.  The neighbour's side of a face is its own work-item's.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
-  const int64_t E = int64_t(a.maxdeg) * a.N;
   const CellAdj ca = cell_adj<STEP>(a, i);
-  const hg::CellQ& me = ca.me;
   double xib = 0.0, hb = 0.0, qxb = 0.0, qyb = 0.0, nb = 0.0, zbb = 0.0;
   const int d = a.deg[i];
   for (int k = 0; k < d; ++k) {  // local face order
-    const int64_t s = int64_t(k) * a.N + i;
-    xib += sb[0 * E + s];
-    hb += sb[1 * E + s];
-    qxb += sb[2 * E + s];
-    qyb += sb[3 * E + s];
-    nb += sb[4 * E + s];
-    zbb += sb[5 * E + s];
+    hg::SideBar U;
+    double fn, fz;
+    adj_face<STEP>(a, ca, int64_t(k) * a.N + i, U, fn, fz);
+    xib += U.xi;
+    hb += U.h;
+    qxb += U.hu;
+    qyb += U.hv;
+    nb += fn;
+    zbb += fz;
   }
+  adj_cell_finish<STEP>(a, i, ca, xib, hb, qxb, qyb, nb, zbb);
+}
+
+// The source VJP, the clamp and the identity part of the step, and the
+// writes of one cell, given its summed face cotangents.
+template <bool STEP>
+__device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, const CellAdj& ca, double xib,
+                                                double hb, double qxb, double qyb, double nb, double zbb) {
+  const hg::CellQ& me = ca.me;
   double S0xb = 0.0, S0yb = 0.0;
   if (ca.wet && a.fmag) {
     double mb = 0.0;
@@ -761,14 +735,17 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t
                                                      const double* n, const double* inletQ, const double* totalA,
                                                      const double* vbar, double hsmall, double* ox, double* nbar,
                                                      double* Qbar) {
-  // One work-group for all inlets (a cell may sit on two of them): the
-  // per-face terms are formed in parallel and summed by a fixed tree; each
-  // cell's faces of an inlet are applied, in NodeString order, by the
-  // work-item of its first face (first[f] = 1; next[f]: the cell's next
-  // face in the inlet, or -1).
+  // Per inlet: the per-face terms are formed in parallel and summed by a
+  // fixed tree; each cell's faces of the inlet are applied, in NodeString
+  // order, by the work-item of its first face (first[f] = 1; next[f]: the
+  // cell's next face in the inlet, or -1).  With one work-group, the inlets
+  // are taken in order (a cell may sit on two of them); when no cell does,
+  // work-group b takes inlet b alone (same arithmetic, same bits).
   __shared__ double sA[kBlock], sQ[kBlock];
   const int tid = threadIdx.x;
-  for (int b = 0; b < n_inlet; ++b) {
+  const int b0 = gridDim.x > 1 ? int(blockIdx.x) : 0;
+  const int b1 = gridDim.x > 1 ? int(blockIdx.x) + 1 : n_inlet;
+  for (int b = b0; b < b1; ++b) {
     const double A = totalA[b];
     double pa = 0.0, pq = 0.0;
     for (int f = off[b] + tid; f < off[b + 1]; f += kBlock) {
@@ -1881,44 +1858,6 @@ int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3],
 int advance_async(hg_ctx* ctx, double dt, int64_t nsteps);
 int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms);
 
-// Face-once work list of the adjoint: every boundary slot, and each interior
-// face once, at the slot of its owner, the side with the smaller caller
-// index (so the list, and the gradient bits, do not depend on renumbering);
-// mirror[s] is the same face's slot in the neighbour.  Left empty when a
-// pair of cells shares more than one face (the slot kernel then runs).
-int build_adj_items(hg_ctx* ctx) {
-  const int64_t N = ctx->N, E = int64_t(ctx->maxdeg) * N;
-  std::vector<int32_t> items, mirror(E, -1);
-  items.reserve(size_t(E / 2 + ctx->G + 1));
-  for (int k = 0; k < ctx->maxdeg; ++k)
-    for (int64_t i = 0; i < N; ++i) {
-      if (k >= ctx->h_deg[i]) continue;
-      const int64_t s = int64_t(k) * N + i;
-      const int32_t j = ctx->h_nbr[s];
-      if (j < 0) {
-        items.push_back(int32_t(s));
-        continue;
-      }
-      int64_t m = -1;
-      int hits = 0;
-      for (int kk = 0; kk < ctx->h_deg[j]; ++kk)
-        if (ctx->h_nbr[int64_t(kk) * N + j] == int32_t(i)) {
-          m = int64_t(kk) * N + j;
-          ++hits;
-        }
-      if (hits != 1) return HG_OK;  // not a conforming pair: keep the slot kernel
-      mirror[s] = int32_t(m);
-      if (ctx->perm[i] < ctx->perm[j]) items.push_back(int32_t(s));
-    }
-  int rc = dalloc(ctx, &ctx->adj_items, std::max<size_t>(items.size(), 1));
-  if (!rc) rc = upload(ctx, ctx->adj_items, items.data(), items.size());
-  if (!rc) rc = dalloc(ctx, &ctx->adj_mirror, size_t(E));
-  if (!rc) rc = upload(ctx, ctx->adj_mirror, mirror.data(), size_t(E));
-  if (rc) return rc;
-  ctx->adj_n_items = int32_t(items.size());
-  return HG_OK;
-}
-
 int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
   const int64_t N = ctx->N;
   int rc = 0;
@@ -1930,8 +1869,6 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (!rc) rc = dalloc(ctx, &ctx->adj_S0bar, 2 * N);
     if (!rc) rc = dalloc(ctx, &ctx->adj_vbar, std::max<int64_t>(ctx->G, 1));
     if (!rc) rc = dalloc(ctx, &ctx->adj_Qbar, std::max<int32_t>(ctx->n_inlet, 1));
-    if (!rc) rc = dalloc(ctx, &ctx->adj_slot, size_t(6) * ctx->maxdeg * N);
-    if (!rc) rc = build_adj_items(ctx);
     if (rc) return rc;
   }
   if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
@@ -2033,26 +1970,13 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   hg::AdjArgs a = make_adj_args(ctx, q, lam, mask, out, dt);
   a.nbar = nbar;
   if (totalA) a.totalA = totalA;
-  const int64_t E = int64_t(ctx->maxdeg) * ctx->N;
-  const int gs = int((E + kBlock - 1) / kBlock);
-  const int gi = (ctx->adj_n_items + kBlock - 1) / kBlock;
-  if (step) {
-    if (ctx->adj_n_items)
-      hipLaunchKernelGGL(k_adj_face<true>, dim3(gi), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot, ctx->adj_items,
-                         ctx->adj_n_items, ctx->adj_mirror);
-    else
-      hipLaunchKernelGGL(k_adj_slot<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
-    hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
-  } else {
-    if (ctx->adj_n_items)
-      hipLaunchKernelGGL(k_adj_face<false>, dim3(gi), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot, ctx->adj_items,
-                         ctx->adj_n_items, ctx->adj_mirror);
-    else
-      hipLaunchKernelGGL(k_adj_slot<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
-    hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a, ctx->adj_slot);
-  }
+  if (step)
+    hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   if (ctx->n_inlet > 0)
-    hipLaunchKernelGGL(k_inlet_adj, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->n_inlet, ctx->in_off,
+    hipLaunchKernelGGL(k_inlet_adj, dim3(ctx->inlets_disjoint ? ctx->n_inlet : 1), dim3(kBlock), 0, ctx->stream,
+                       ctx->n_inlet, ctx->in_off,
                        ctx->in_ghost, ctx->in_cell, ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0],
                        ctx->hstill, ctx->n, ctx->inletQ, a.totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar,
                        ctx->adj_Qbar);
@@ -2691,6 +2615,13 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
     }
     rc = upload(ctx, ctx->in_next, next.data(), ctx->n_inlet_faces);
     if (!rc) rc = upload(ctx, ctx->in_first, first.data(), ctx->n_inlet_faces);
+    std::unordered_map<int32_t, int32_t> owner;  // cell -> the inlet it sits on
+    ctx->inlets_disjoint = true;
+    for (int bi = 0; bi < b->n_inlet && ctx->inlets_disjoint; ++bi)
+      for (int32_t f = in_off[bi]; f < in_off[bi + 1]; ++f) {
+        auto it = owner.emplace(in_cell[f], bi).first;
+        if (it->second != bi) ctx->inlets_disjoint = false;
+      }
   }
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
@@ -3817,9 +3748,6 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
     for (int k = 0; k < 3; ++k)
       HG_TRY(hipMemcpyAsync(ctx->scratch + k * N, ctx->st[ctx->cur][k], N * sizeof(double), hipMemcpyDeviceToDevice,
                             ctx->stream));
-    hipEvent_t e0, e1;
-    HG_TRY(hipEventCreate(&e0));
-    HG_TRY(hipEventCreate(&e1));
     int rc = HG_OK;
     for (int T : {128, 256, 512}) {
       if (tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) continue;
@@ -3831,25 +3759,19 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
             rc = HG_ERR_HIP;
         ctx->cur = 0;
         if (!rc) rc = clear_flags(ctx);
-        if (!rc) rc = advance_async(ctx, dt, kGraphSteps);  // warm-up, graph instantiation
-        if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
-        if (!rc) rc = advance_async(ctx, dt, nsteps);
-        if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
-        float ms = 0.f;
-        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-          rc = HG_ERR_HIP;
+        double warm = 0.0, ms = 0.0;  // step kernel alone: events on each dispatch, after a short warm-up
+        if (!rc) rc = advance_timed(ctx, dt, 16, &warm);
+        if (!rc) rc = advance_timed(ctx, dt, nsteps, &ms);
         if (!rc) rc = check_flags(ctx);
         if (rc) break;
-        if (double(ms) < best) {
-          best = double(ms);
+        if (ms < best) {
+          best = ms;
           bestT = T;
           bestX = x;
         }
       }
       if (rc) break;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (!rc) rc = hg_set_tile_config(ctx, bestT, bestX);
     for (int k = 0; k < 3 && !rc; ++k)
       if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
@@ -3861,7 +3783,6 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
       if (ctx->err.empty()) ctx->err = "hg_autotune: HIP error";
       return rc;
     }
-    best /= double(nsteps);
   }
   if (tile_cells) *tile_cells = ctx->tiled ? bestT : 0;
   if (xcd_chunk) *xcd_chunk = bestX;

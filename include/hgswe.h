@@ -190,9 +190,11 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
 int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk);
 int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk);
 
-/* Autotuner: times nsteps steps of every launch shape from the current
- * device state (which is restored afterwards), keeps the fastest and
- * reports it.  Needs hg_set_fields and a device state (hg_state_upload). */
+/* Autotuner: times the step kernel over nsteps steps of every launch shape
+ * (events on each dispatch, after 16 warm-up steps) from the current device
+ * state (which is restored afterwards), keeps the fastest and reports its
+ * mean kernel time in *ms_per_step.  Needs hg_set_fields and a device state
+ * (hg_state_upload). */
 int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_chunk,
                 double* ms_per_step);
 
