@@ -251,10 +251,12 @@ struct hg_ctx {
   // adjoint work buffers (allocated on first use)
   double* adj_lam[2] = {nullptr, nullptr};  // 3N each
   double *adj_nbar = nullptr, *adj_zbbar = nullptr, *adj_S0bar = nullptr, *adj_vbar = nullptr, *adj_Qbar = nullptr;
-  double* adj_seg = nullptr;    // (kck + 1) x 3N states of one checkpoint segment
-  uint8_t* adj_mask = nullptr;  // kck x N clamp flags
+  double* adj_seg[2] = {nullptr, nullptr};    // (kck + 1) x 3N states of a checkpoint segment (two buffers)
+  uint8_t* adj_mask[2] = {nullptr, nullptr};  // kck x N clamp flags of each
   double* adj_ckpt = nullptr;   // nseg x 3N checkpoints
-  double* adj_totalA = nullptr; // kck x n_inlet conveyances recorded by a segment's recompute
+  double* adj_totalA[2] = {nullptr, nullptr};  // kck x n_inlet conveyances recorded by each segment's recompute
+  hipStream_t stream2 = nullptr;  // the adjoint's segment recompute, overlapping the reverse sweep
+  hipEvent_t ev_seg[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr}, ev_fwd = nullptr;
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
 
   // UDE closure for n (0: off; else the precision of its network, 32 or 64)

@@ -1872,11 +1872,18 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (rc) return rc;
   }
   if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
-    rc = dalloc(ctx, &ctx->adj_seg, size_t(kck + 1) * 3 * N);
-    if (!rc) rc = dalloc(ctx, &ctx->adj_mask, size_t(kck) * N);
-    if (!rc) rc = dalloc(ctx, &ctx->adj_totalA, size_t(kck) * std::max<int32_t>(ctx->n_inlet, 1));
+    for (int b = 0; b < 2 && !rc; ++b) {  // two segment buffers: one recomputed while the other is reversed
+      rc = dalloc(ctx, &ctx->adj_seg[b], size_t(kck + 1) * 3 * N);
+      if (!rc) rc = dalloc(ctx, &ctx->adj_mask[b], size_t(kck) * N);
+      if (!rc) rc = dalloc(ctx, &ctx->adj_totalA[b], size_t(kck) * std::max<int32_t>(ctx->n_inlet, 1));
+    }
     if (rc) return rc;
     ctx->adj_seg_cap = size_t(kck);
+  }
+  if (!ctx->stream2) {  // the recompute stream (default priority) and its events
+    HG_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&ctx->ev_seg[0], &ctx->ev_seg[1], &ctx->ev_rev[0], &ctx->ev_rev[1], &ctx->ev_fwd})
+      HG_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if (size_t(nseg) > ctx->adj_ckpt_cap) {
     rc = dalloc(ctx, &ctx->adj_ckpt, size_t(nseg) * 3 * N);
@@ -2051,11 +2058,11 @@ int cell_out(hg_ctx* ctx, const double* dsrc, double* hdst) {
   return HG_OK;
 }
 
-double* seg_state(hg_ctx* ctx, int64_t t, int c) { return ctx->adj_seg + (t * 3 + c) * ctx->N; }
+double* seg_state(hg_ctx* ctx, int64_t t, int c, int b = 0) { return ctx->adj_seg[b] + (t * 3 + c) * ctx->N; }
 
 // Inlet conveyances of step t of the segment being reversed (nullptr without inlets).
-double* seg_totalA(hg_ctx* ctx, int64_t t) {
-  return ctx->n_inlet > 0 ? ctx->adj_totalA + t * ctx->n_inlet : nullptr;
+double* seg_totalA(hg_ctx* ctx, int64_t t, int b = 0) {
+  return ctx->n_inlet > 0 ? ctx->adj_totalA[b] + t * ctx->n_inlet : nullptr;
 }
 
 // Forward nsteps from the device state, storing a checkpoint every kck steps.
@@ -2072,31 +2079,70 @@ int forward_checkpointed(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck) {
   return HG_OK;
 }
 
+// Recompute segment sgi from its checkpoint into segment buffer b, recording
+// the clamp and the inlet conveyances of every step; enqueued on `stream`.
+int recompute_segment(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int64_t sgi, int b, hipStream_t stream) {
+  const int64_t N = ctx->N;
+  const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
+  const hipStream_t main = ctx->stream;
+  ctx->stream = stream;  // every enqueue below goes to `stream`
+  int rc = HG_OK;
+  for (int c = 0; c < 3 && !rc; ++c)
+    if (hipMemcpyAsync(seg_state(ctx, 0, c, b), ctx->adj_ckpt + (sgi * 3 + c) * N, N * sizeof(double),
+                       hipMemcpyDeviceToDevice, stream) != hipSuccess)
+      rc = HG_ERR_HIP;
+  for (int64_t t = 0; t < ns && !rc; ++t) {
+    const double* in[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
+    double* out[3] = {seg_state(ctx, t + 1, 0, b), seg_state(ctx, t + 1, 1, b), seg_state(ctx, t + 1, 2, b)};
+    rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask[b] + t * N, seg_totalA(ctx, t, b));
+  }
+  ctx->stream = main;
+  return rc;
+}
+
 // Reverse sweep over the checkpointed trajectory; lam (device, 3 x N) holds
 // the cotangent of the final state on entry and of the initial state on exit.
+// Segment s - 1 is recomputed on the second stream into the other segment
+// buffer while segment s is reversed on the context stream (events order
+// the two); with a state-dependent n (closure, law or network), whose
+// kernels rewrite the shared n every step, the two run in turn on one stream.
 int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_cur) {
   const int64_t N = ctx->N;
   const int64_t nseg = (nsteps + kck - 1) / kck;
+  const bool overlap = !ctx->ude_prec && ctx->nlaw.kind == hg::NLAW_OFF && ctx->lux.choice == hg::LUX_OFF;
+  const hipStream_t rs = overlap ? ctx->stream2 : ctx->stream;
+  if (overlap) {  // the recompute stream starts after everything already on the context stream
+    HG_TRY(hipEventRecord(ctx->ev_fwd, ctx->stream));
+    HG_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fwd, 0));
+  }
+  int rc = recompute_segment(ctx, dt, nsteps, kck, nseg - 1, 0, rs);
+  if (rc) return rc;
+  if (overlap) HG_TRY(hipEventRecord(ctx->ev_seg[0], rs));
   for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
-    const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
-    for (int c = 0; c < 3; ++c)
-      HG_TRY(hipMemcpyAsync(seg_state(ctx, 0, c), ctx->adj_ckpt + (sgi * 3 + c) * N, N * sizeof(double),
-                            hipMemcpyDeviceToDevice, ctx->stream));
-    for (int64_t t = 0; t < ns; ++t) {  // recompute the segment, recording the clamp of every step
-      const double* in[3] = {seg_state(ctx, t, 0), seg_state(ctx, t, 1), seg_state(ctx, t, 2)};
-      double* out[3] = {seg_state(ctx, t + 1, 0), seg_state(ctx, t + 1, 1), seg_state(ctx, t + 1, 2)};
-      int rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask + t * N, seg_totalA(ctx, t));
+    const int b = int((nseg - 1 - sgi) & 1);
+    if (overlap && sgi > 0) {  // the previous segment, into the other buffer once its last reversal is done
+      if (sgi < nseg - 1) HG_TRY(hipStreamWaitEvent(rs, ctx->ev_rev[1 - b], 0));
+      rc = recompute_segment(ctx, dt, nsteps, kck, sgi - 1, 1 - b, rs);
       if (rc) return rc;
+      HG_TRY(hipEventRecord(ctx->ev_seg[1 - b], rs));
     }
+    if (overlap) HG_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_seg[b], 0));
+    const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
     for (int64_t t = ns - 1; t >= 0; --t) {
-      const double* q[3] = {seg_state(ctx, t, 0), seg_state(ctx, t, 1), seg_state(ctx, t, 2)};
+      const double* q[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
       double* L = ctx->adj_lam[lam_cur];
       double* O = ctx->adj_lam[1 - lam_cur];
       const double* lam[3] = {L, L + N, L + 2 * N};
       double* out[3] = {O, O + N, O + 2 * N};
-      int rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask + t * N, out, dt, true, seg_totalA(ctx, t));
+      rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask[b] + t * N, out, dt, true, seg_totalA(ctx, t, b));
       if (rc) return rc;
       lam_cur = 1 - lam_cur;
+    }
+    if (overlap) {
+      HG_TRY(hipEventRecord(ctx->ev_rev[b], ctx->stream));
+    } else if (sgi > 0) {
+      rc = recompute_segment(ctx, dt, nsteps, kck, sgi - 1, 1 - b, rs);
+      if (rc) return rc;
     }
   }
   return HG_OK;
@@ -2773,6 +2819,12 @@ int hg_destroy(hg_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream2) {
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipStreamDestroy(ctx->stream2);
+  }
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

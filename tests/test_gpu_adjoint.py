@@ -40,6 +40,21 @@ def wet_dry_case():
     return c.copy_fields(Q0=Q0, dt=cases.cfl_dt(c, 0.4, Q0), name="wet_dry")
 
 
+def corner_inlets_case():
+    """Two inlet-Q boundaries (bottom and left walls of a quad channel) that
+    share the corner cell, an exit on the right: the inlet adjoint then
+    takes the inlets in order in one work-group (a cell on two inlets)."""
+    from hgswe import mesh as hm
+
+    xy, cells, sides = cases.rect_quads(200.0, 20.0, 40, 6)
+    ns = {1: sides["bottom"], 2: sides["right"], 3: sides["top"], 4: sides["left"]}
+    mesh = hm.build_mesh(xy, cells, ns, {1: "INLET-Q", 2: "EXIT-H", 3: "WALL", 4: "INLET-Q"})
+    N = mesh.n_cells
+    z = np.zeros(N)
+    return cases._finish("corner_inlets", mesh, z, np.full(N, 0.03), np.full(N, 1.0), z, z, z, [4.0, 2.0], [1.0],
+                         0.02, 50)
+
+
 def golden(name, tmp_path):
     from conftest import load_golden_case
     from test_oracle_golden import savannah_ic_path
@@ -89,6 +104,15 @@ def test_rhs_vjp_wet_dry():
     say(f"  dry cells: {int(np.sum(h <= c.h_small))}")
     assert np.sum(h <= c.h_small) > 50
     vjp_check(c, Q, seed=3)
+
+
+def test_rhs_vjp_inlets_sharing_a_cell():
+    c = corner_inlets_case()
+    gc = c.mesh.ghost_cell[c.mesh.inlet_ghost]
+    o = c.mesh.inlet_off
+    assert set(gc[o[0]:o[1]].tolist()) & set(gc[o[1]:o[2]].tolist())  # the corner cell
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 40)
+    vjp_check(c, Q, seed=6)
 
 
 @pytest.mark.parametrize("name", ["bump", "savannah"])
