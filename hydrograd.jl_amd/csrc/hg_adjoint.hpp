@@ -65,10 +65,10 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   int mode = 0;  // 0 general, 1 B mirrored from A, 2 A mirrored from B
   double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
   double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
-  if ((A.h + A.zb) < (B.zb + hmin) && (B.h <= hmin)) {
+  if ((B.h <= hmin) && (A.h + A.zb) < (B.zb + hmin)) {
     mode = 1;
     hB = A.h; huB = -A.hu; hvB = -A.hv; uB = -A.u; vB = -A.v; sB = A.s;
-  } else if ((B.h + B.zb) < (A.zb + hmin) && (A.h <= hmin)) {
+  } else if ((A.h <= hmin) && (B.h + B.zb) < (A.zb + hmin)) {
     mode = 2;
     hA = B.h; huA = -B.hu; hvA = -B.hv; uA = -B.u; vA = -B.v; sA = B.s;
   } else if (A.h <= hmin || B.h <= hmin) {  // physical flux of the wet side, h^2 pressure

@@ -29,9 +29,9 @@ __host__ __device__ __forceinline__ void roe_flux(SideT<T> L, SideT<T> R, double
   if (L.h <= hmin && R.h <= hmin) {  // :16-22
     f0 = 0.0; f1 = 0.0; f2 = 0.0;
     return;
-  } else if ((L.h + L.zb) < (R.zb + hmin) && (R.h <= hmin)) {  // :23-36
+  } else if ((R.h <= hmin) && (L.h + L.zb) < (R.zb + hmin)) {  // :23-36
     R.h = L.h; R.hu = -L.hu; R.hv = -L.hv;
-  } else if ((R.h + R.zb) < (L.zb + hmin) && (L.h <= hmin)) {  // :39-51
+  } else if ((L.h <= hmin) && (R.h + R.zb) < (L.zb + hmin)) {  // :39-51
     L.h = R.h; L.hu = -R.hu; L.hv = -R.hv;
   } else if (L.h <= hmin) {  // :54-64
     const T u = R.hu / R.h, v = R.hv / R.h;

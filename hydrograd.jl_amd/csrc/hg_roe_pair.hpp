@@ -61,9 +61,9 @@ __host__ __device__ __forceinline__ void roe_pair(const CellQ& A, const CellQ& B
   }
   double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
   double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
-  if ((A.h + A.zb) < (B.zb + hmin) && (B.h <= hmin)) {  // :23-36, B := mirror of A (xi_B, h_still_B kept)
+  if ((B.h <= hmin) && (A.h + A.zb) < (B.zb + hmin)) {  // :23-36, B := mirror of A (xi_B, h_still_B kept)
     hB = A.h; huB = -A.hu; hvB = -A.hv; uB = -A.u; vB = -A.v; sB = A.s;
-  } else if ((B.h + B.zb) < (A.zb + hmin) && (A.h <= hmin)) {  // :39-51
+  } else if ((A.h <= hmin) && (B.h + B.zb) < (A.zb + hmin)) {  // :39-51
     hA = B.h; huA = -B.hu; hvA = -B.hv; uA = -B.u; vA = -B.v; sA = B.s;
   } else if (A.h <= hmin || B.h <= hmin) {  // :54-75, physical flux of the wet side, h^2 pressure
     const bool left_dry = A.h <= hmin;

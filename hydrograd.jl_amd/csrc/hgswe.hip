@@ -2950,6 +2950,28 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps) {
   return rc;
 }
 
+int hg_advance_region(hg_ctx* ctx, double dt, int64_t nsteps, double* gpu_ms_per_step) {
+  if (!ctx || nsteps <= 0) return HG_ERR_ARG;
+  if (!ctx->fields_set) {
+    ctx->err = "hg_set_fields has not been called";
+    return HG_ERR_STATE;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = clear_flags(ctx);
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = HG_ERR_HIP;
+  if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc) rc = advance_async(ctx, dt, nsteps);
+  if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc) rc = enqueue_finite_check(ctx);
+  if (!rc) rc = check_flags(ctx);  // synchronizes the stream
+  float ms = 0.f;
+  if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc && gpu_ms_per_step) *gpu_ms_per_step = double(ms) / double(nsteps);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
 int hg_advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
   if (!ctx || nsteps < 0) return HG_ERR_ARG;
   if (!ctx->fields_set) {

@@ -166,6 +166,14 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps);
  * custom_ODE_solvers.jl:39). */
 int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
 
+/* Measurement helper with no counterpart in the reference: hg_advance
+ * (hipGraph replays) with HIP events recorded on the context stream right
+ * before the first and after the last step; *gpu_ms_per_step is the elapsed
+ * GPU time over the steps.  When nothing but the step kernel runs per step
+ * (no separate conveyance kernel, no closure), it is the step kernel's
+ * average launch duration over these launches, back to back. */
+int hg_advance_region(hg_ctx* ctx, double dt, int64_t nsteps, double* gpu_ms_per_step);
+
 /* Measurement helper with no counterpart in the reference: hg_advance with
  * every step launched singly and start / stop events attached to each step
  * kernel's dispatch (hipExtLaunchKernelGGL); *kernel_ms is the mean duration
