@@ -304,8 +304,9 @@ struct hg_ctx {
   // events attached to timed step dispatches (hg_advance_timed), reused
   std::vector<hipEvent_t> ev_pool;
 
-  // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
+  // cached graphs of kGraphSteps and kGraphBig steps (buffer 0 -> ... -> buffer 0) for dt gdt
   hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec_big = nullptr;
   double gdt = 0.0;
 
   std::vector<void*> allocs;

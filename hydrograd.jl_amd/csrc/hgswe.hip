@@ -54,6 +54,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kGraphSteps = 16;  // steps per captured graph (even: the graph returns to buffer 0)
+constexpr int kGraphBig = 128;   // steps per graph for long runs (fewer graph boundaries)
 constexpr int kXCD = 8;         // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kTileInlets = 64;  // inlet boundaries one tile may touch when the conveyance is fused
 
@@ -2282,10 +2283,12 @@ int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
 }
 
 int drop_graph(hg_ctx* ctx) {
-  if (ctx->gexec) {
-    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of it may still be running
-    HG_TRY(hipGraphExecDestroy(ctx->gexec));
+  if (ctx->gexec || ctx->gexec_big) {
+    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of them may still be running
+    if (ctx->gexec) HG_TRY(hipGraphExecDestroy(ctx->gexec));
+    if (ctx->gexec_big) HG_TRY(hipGraphExecDestroy(ctx->gexec_big));
     ctx->gexec = nullptr;
+    ctx->gexec_big = nullptr;
   }
   return HG_OK;
 }
@@ -2362,20 +2365,22 @@ int setup_tiles(hg_ctx* ctx, int T) {
   return HG_OK;
 }
 
-int ensure_graph(hg_ctx* ctx, double dt) {
-  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
-  if (drop_graph(ctx)) return HG_ERR_HIP;
+// The cached graph of `steps` steps (kGraphSteps or kGraphBig) for dt.
+int ensure_graph(hg_ctx* ctx, double dt, int steps = kGraphSteps) {
+  hipGraphExec_t& ge = steps == kGraphBig ? ctx->gexec_big : ctx->gexec;
+  if (ge && ctx->gdt == dt) return HG_OK;
+  if (ctx->gdt != dt && drop_graph(ctx)) return HG_ERR_HIP;
   hipGraph_t graph;
   HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(ctx, k & 1, (k + 1) & 1, dt, true);
+  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(ctx, k & 1, (k + 1) & 1, dt, true);
   hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
   if (rc) return rc;
   if (e != hipSuccess) {
     ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
     return HG_ERR_HIP;
   }
-  HG_TRY(hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0));
+  HG_TRY(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
   HG_TRY(hipGraphDestroy(graph));
   ctx->gdt = dt;
   return HG_OK;
@@ -2389,6 +2394,11 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
     if (rc) return rc;
     ctx->cur = 0;
     --nsteps;
+  }
+  if (nsteps >= kGraphBig) {
+    int rc = ensure_graph(ctx, dt, kGraphBig);
+    if (rc) return rc;
+    for (; nsteps >= kGraphBig; nsteps -= kGraphBig) HG_TRY(hipGraphLaunch(ctx->gexec_big, ctx->stream));
   }
   if (nsteps >= kGraphSteps) {
     int rc = ensure_graph(ctx, dt);
@@ -2818,6 +2828,7 @@ int hg_destroy(hg_ctx* ctx) {
   if (!ctx) return HG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->gexec_big) (void)hipGraphExecDestroy(ctx->gexec_big);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd})
     if (e) (void)hipEventDestroy(e);
@@ -3823,10 +3834,28 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
       HG_TRY(hipMemcpyAsync(ctx->scratch + k * N, ctx->st[ctx->cur][k], N * sizeof(double), hipMemcpyDeviceToDevice,
                             ctx->stream));
     int rc = HG_OK;
-    for (int T : {128, 256, 512}) {
-      if (tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) continue;
-      for (int x : {0, 1, 2, 4, 8, 16, 64}) {
-        rc = hg_set_tile_config(ctx, T, x);
+    // The device clocks settle over several hundred steps: warm up first, so
+    // that candidates timed early are not slowed by a cold device; then two
+    // passes over the candidates, each keeping its faster pass.
+    {
+      double warm = 0.0;
+      ctx->cur = 0;
+      for (int k = 0; k < 3 && !rc; ++k)
+        if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess)
+          rc = HG_ERR_HIP;
+      if (!rc) rc = clear_flags(ctx);
+      if (!rc) rc = advance_timed(ctx, dt, 500, &warm);
+      if (!rc) rc = check_flags(ctx);
+    }
+    std::vector<std::pair<int, int>> shapes;
+    for (int T : {128, 256, 512})
+      if (tile_lds_bytes(T, ctx->tileMD) <= ctx->lds_max)
+        for (int x : {0, 1, 2, 4, 8, 16, 64}) shapes.emplace_back(T, x);
+    std::vector<double> t_best(shapes.size(), INFINITY);
+    for (int pass = 0; pass < 2 && !rc; ++pass)
+      for (size_t c = 0; c < shapes.size() && !rc; ++c) {
+        rc = hg_set_tile_config(ctx, shapes[c].first, shapes[c].second);
         for (int k = 0; k < 3 && !rc; ++k)
           if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
                              ctx->stream) != hipSuccess)
@@ -3837,15 +3866,14 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
         if (!rc) rc = advance_timed(ctx, dt, 16, &warm);
         if (!rc) rc = advance_timed(ctx, dt, nsteps, &ms);
         if (!rc) rc = check_flags(ctx);
-        if (rc) break;
-        if (ms < best) {
-          best = ms;
-          bestT = T;
-          bestX = x;
-        }
+        if (!rc) t_best[c] = std::min(t_best[c], ms);
       }
-      if (rc) break;
-    }
+    for (size_t c = 0; c < shapes.size() && !rc; ++c)
+      if (t_best[c] < best) {
+        best = t_best[c];
+        bestT = shapes[c].first;
+        bestX = shapes[c].second;
+      }
     if (!rc) rc = hg_set_tile_config(ctx, bestT, bestX);
     for (int k = 0; k < 3 && !rc; ++k)
       if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,

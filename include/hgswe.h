@@ -198,11 +198,12 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
 int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk);
 int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk);
 
-/* Autotuner: times the step kernel over nsteps steps of every launch shape
- * (events on each dispatch, after 16 warm-up steps) from the current device
- * state (which is restored afterwards), keeps the fastest and reports its
- * mean kernel time in *ms_per_step.  Needs hg_set_fields and a device state
- * (hg_state_upload). */
+/* Autotuner: after 500 warm-up steps (the device clocks settle), times the
+ * step kernel over nsteps steps of every launch shape, twice each (events on
+ * each dispatch, after 16 warm-up steps), from the current device state
+ * (which is restored afterwards); keeps the shape with the lowest time and
+ * reports that mean kernel time in *ms_per_step.  Needs hg_set_fields and a
+ * device state (hg_state_upload). */
 int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_chunk,
                 double* ms_per_step);
 
