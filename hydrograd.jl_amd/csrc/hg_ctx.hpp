@@ -304,9 +304,8 @@ struct hg_ctx {
   // events attached to timed step dispatches (hg_advance_timed), reused
   std::vector<hipEvent_t> ev_pool;
 
-  // cached graphs of kGraphSteps and kGraphBig steps (buffer 0 -> ... -> buffer 0) for dt gdt
+  // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;
-  hipGraphExec_t gexec_big = nullptr;
   double gdt = 0.0;
 
   std::vector<void*> allocs;

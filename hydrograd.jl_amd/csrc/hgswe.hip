@@ -54,7 +54,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kGraphSteps = 16;  // steps per captured graph (even: the graph returns to buffer 0)
-constexpr int kGraphBig = 128;   // steps per graph for long runs (fewer graph boundaries)
 constexpr int kXCD = 8;         // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kTileInlets = 64;  // inlet boundaries one tile may touch when the conveyance is fused
 
@@ -2283,12 +2282,10 @@ int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
 }
 
 int drop_graph(hg_ctx* ctx) {
-  if (ctx->gexec || ctx->gexec_big) {
-    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of them may still be running
-    if (ctx->gexec) HG_TRY(hipGraphExecDestroy(ctx->gexec));
-    if (ctx->gexec_big) HG_TRY(hipGraphExecDestroy(ctx->gexec_big));
+  if (ctx->gexec) {
+    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of it may still be running
+    HG_TRY(hipGraphExecDestroy(ctx->gexec));
     ctx->gexec = nullptr;
-    ctx->gexec_big = nullptr;
   }
   return HG_OK;
 }
@@ -2365,22 +2362,24 @@ int setup_tiles(hg_ctx* ctx, int T) {
   return HG_OK;
 }
 
-// The cached graph of `steps` steps (kGraphSteps or kGraphBig) for dt.
-int ensure_graph(hg_ctx* ctx, double dt, int steps = kGraphSteps) {
-  hipGraphExec_t& ge = steps == kGraphBig ? ctx->gexec_big : ctx->gexec;
-  if (ge && ctx->gdt == dt) return HG_OK;
-  if (ctx->gdt != dt && drop_graph(ctx)) return HG_ERR_HIP;
+// The cached graph of kGraphSteps steps for dt.  (A longer graph saves
+// about 0.1 us per step of graph boundaries, but rocprofv3's kernel tracing
+// replays a 128-step graph at half speed, so profiles would not describe
+// the unprofiled run.)
+int ensure_graph(hg_ctx* ctx, double dt) {
+  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
+  if (drop_graph(ctx)) return HG_ERR_HIP;
   hipGraph_t graph;
   HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int k = 0; k < steps && !rc; ++k) rc = enqueue_step(ctx, k & 1, (k + 1) & 1, dt, true);
+  for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(ctx, k & 1, (k + 1) & 1, dt, true);
   hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
   if (rc) return rc;
   if (e != hipSuccess) {
     ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
     return HG_ERR_HIP;
   }
-  HG_TRY(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+  HG_TRY(hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0));
   HG_TRY(hipGraphDestroy(graph));
   ctx->gdt = dt;
   return HG_OK;
@@ -2394,11 +2393,6 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
     if (rc) return rc;
     ctx->cur = 0;
     --nsteps;
-  }
-  if (nsteps >= kGraphBig) {
-    int rc = ensure_graph(ctx, dt, kGraphBig);
-    if (rc) return rc;
-    for (; nsteps >= kGraphBig; nsteps -= kGraphBig) HG_TRY(hipGraphLaunch(ctx->gexec_big, ctx->stream));
   }
   if (nsteps >= kGraphSteps) {
     int rc = ensure_graph(ctx, dt);
@@ -2828,7 +2822,6 @@ int hg_destroy(hg_ctx* ctx) {
   if (!ctx) return HG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-  if (ctx->gexec_big) (void)hipGraphExecDestroy(ctx->gexec_big);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd})
     if (e) (void)hipEventDestroy(e);
