@@ -2409,6 +2409,14 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
 
 // nsteps single-launch steps, each step kernel with its own start / stop
 // events on its dispatch; *kernel_ms = their mean duration.  Synchronous.
+// Entry points that launch kernels over the fields refuse to run before
+// hg_set_fields has uploaded them (the kernels would read null pointers).
+int require_fields(hg_ctx* ctx, const char* who) {
+  if (ctx->fields_set) return HG_OK;
+  ctx->err = std::string(who) + ": hg_set_fields has not been called";
+  return HG_ERR_STATE;
+}
+
 int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
   std::vector<hipEvent_t>& ev = ctx->ev_pool;  // grown once (at least 2 x 2048), reused by later calls
   int rc = HG_OK;
@@ -3107,6 +3115,7 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
 
 int hg_ude_n(hg_ctx* ctx, const double* Q, double* n) {
   if (!ctx || !Q || !n) return ctx ? (ctx->err = "hg_ude_n: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (require_fields(ctx, "hg_ude_n")) return HG_ERR_STATE;
   if (!ctx->ude_prec) {
     ctx->err = "hg_ude_n: the UDE closure is off";
     return HG_ERR_STATE;
@@ -3236,6 +3245,7 @@ int hg_n_law_eval(hg_ctx* ctx, const double* h, const double* Umag, double* n, d
 
 int hg_cell_n(hg_ctx* ctx, const double* Q, double* n) {
   if (!ctx || !Q || !n) return ctx ? (ctx->err = "hg_cell_n: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (require_fields(ctx, "hg_cell_n")) return HG_ERR_STATE;
   int rc = state_in(ctx, Q, 0);
   if (rc) return rc;
   ctx->cur = 0;
@@ -3369,6 +3379,7 @@ int hg_set_lux_ude(hg_ctx* ctx, const hg_lux_net* net, const double* params, con
 
 int hg_lux_eval(hg_ctx* ctx, const double* Q, double* out) {
   if (!ctx || !Q || !out) return ctx ? (ctx->err = "hg_lux_eval: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (require_fields(ctx, "hg_lux_eval")) return HG_ERR_STATE;
   if (!ctx->lux.choice) {
     ctx->err = "hg_lux_eval: the UDE network is off";
     return HG_ERR_STATE;
@@ -3733,25 +3744,25 @@ int hg_probe_copy(hg_ctx* ctx, int64_t nbytes, int32_t reps, double* gbps) {
 int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double* ms_per_step,
                     double* ms_step_kernel) {
   if (!ctx || nsteps <= 0 || reps <= 0) return HG_ERR_ARG;
-  hipEvent_t e0, e1;
-  HG_TRY(hipEventCreate(&e0));
-  HG_TRY(hipEventCreate(&e1));
+  if (require_fields(ctx, "hg_time_advance")) return HG_ERR_STATE;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = clear_flags(ctx);
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = HG_ERR_HIP;
   if (!rc) rc = advance_async(ctx, dt, kGraphSteps);  // warm-up (graph instantiation)
-  HG_TRY(hipStreamSynchronize(ctx->stream));
-  HG_TRY(hipEventRecord(e0, ctx->stream));
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
   for (int r = 0; r < reps && !rc; ++r) rc = advance_async(ctx, dt, nsteps);
-  HG_TRY(hipEventRecord(e1, ctx->stream));
-  HG_TRY(hipEventSynchronize(e1));
+  if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = HG_ERR_HIP;
   float ms = 0.f;
-  HG_TRY(hipEventElapsedTime(&ms, e0, e1));
-  if (ms_per_step) *ms_per_step = double(ms) / double(reps * nsteps);
-  if (ms_step_kernel && !rc) {  // the step kernel alone: events attached to each of K dispatches
-    ctx->cur = 0;
-    rc = advance_timed(ctx, dt, 200, ms_step_kernel);
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = HG_ERR_HIP;
+  if (!rc && ms_per_step) *ms_per_step = double(ms) / double(reps * nsteps);
+  // the step kernel alone, continuing from the current state: events
+  // attached to each of 200 single dispatches
+  if (!rc && ms_step_kernel) rc = advance_timed(ctx, dt, 200, ms_step_kernel);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
 }

@@ -159,9 +159,10 @@ class Context:
 
     A batch (``cases.concat``) of instances is created with
     hg_batch_create from the instances' own meshes; every other call then
-    takes the batch's concatenated arrays."""
+    takes the batch's concatenated arrays.  With fields=False the case's
+    fields are not uploaded (set_fields must come before any stepping)."""
 
-    def __init__(self, case, reorder=True, cell_kernel=False):
+    def __init__(self, case, reorder=True, cell_kernel=False, fields=True):
         L = lib()
         self.case = case
         self.N = case.mesh.n_cells
@@ -198,7 +199,8 @@ class Context:
             raise HGError(rc, L.hg_last_error(None).decode())
         self.h = h
         self._keep = []
-        self.set_fields(case)
+        if fields:
+            self.set_fields(case)
 
     def _chk(self, rc):
         if rc:

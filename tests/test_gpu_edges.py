@@ -172,3 +172,28 @@ def test_last_tile_holds_one_cell(T):
     check_rhs(c, c.Q0, ctx, orc)
     qg, _ = check_steps(c, c.Q0, c.dt, c.nsteps, ctx, orc)
     assert np.array_equal(qg, capi.Context(c, cell_kernel=True).step(c.Q0, c.dt, c.nsteps))
+
+
+def test_calls_before_fields_are_refused():
+    """Every call that launches kernels over the fields returns
+    HG_ERR_STATE until hg_set_fields has uploaded them; the context then
+    works as usual."""
+    c = tiny(2)
+    ctx = capi.Context(c, fields=False)
+    calls = {
+        "advance": lambda: ctx.advance(c.dt, 2),
+        "advance_timed": lambda: ctx.advance_timed(c.dt, 2),
+        "advance_region": lambda: ctx.advance_region(c.dt, 2),
+        "time_advance": lambda: ctx.time_advance(c.dt, 2, 1),
+        "rhs": lambda: ctx.rhs(c.Q0),
+        "cell_n": lambda: ctx.cell_n(c.Q0),
+        "autotune": lambda: ctx.autotune(c.dt, 2),
+    }
+    for name, call in calls.items():
+        with pytest.raises(capi.HGError) as e:
+            call()
+        say(f"  {name}: {e.value}")
+        assert e.value.code == capi.HG_ERR_STATE
+    ctx.set_fields(c)
+    ref = capi.Context(c).step(c.Q0, c.dt, 5)
+    assert np.array_equal(ctx.step(c.Q0, c.dt, 5), ref)
