@@ -42,24 +42,44 @@ def traffic_from_profiles():
     return None
 
 
+def host_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def timed_sample(run, seconds, cap):
+    """Times run(k) for a k sized to about `seconds` from a one-step probe."""
+    t0 = time.perf_counter()
+    run(1)
+    per = time.perf_counter() - t0
+    n = max(1, min(cap, int(seconds / max(per, 1e-3))))
+    t0 = time.perf_counter()
+    run(n)
+    return n, time.perf_counter() - t0
+
+
 def cpu_baseline(case, seconds):
     """The oracle (C++ restatement of the reference's loop structure) on the
     host, one thread like the single-threaded reference, on a bounded sample
-    of the same workload: a few Euler steps of the same mesh."""
+    of the same workload: a few Euler steps of the same mesh.  The same
+    sample on all host threads (OpenMP over cells) is reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     orc = pyoracle.Oracle(case)
     pyoracle.set_threads(1)
-    t0 = time.perf_counter()
-    orc.run(case.Q0, case.dt, 1)
-    t1 = time.perf_counter()
-    n = max(1, min(50, int(seconds / max(t1 - t0, 1e-3))))
-    t0 = time.perf_counter()
-    orc.run(case.Q0, case.dt, n)
-    el = time.perf_counter() - t0
-    return {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} Euler steps of the same {case.N}-cell mesh, C++ oracle -O3, 1 thread"}
+    n, el = timed_sample(lambda k: orc.run(case.Q0, case.dt, k), seconds, 50)
+    out = {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{n} Euler steps of the same {case.N}-cell mesh, C++ oracle -O3, 1 thread"}
+    nt = host_threads()
+    pyoracle.set_threads(nt)
+    n, el = timed_sample(lambda k: orc.run(case.Q0, case.dt, k), seconds / 4, 50)
+    pyoracle.set_threads(1)
+    out["all_cores"] = {"value": case.N * n / el, "cores": nt, "sample": f"{n} Euler steps, {nt} threads"}
+    return out
 
 
 def gradient_case(cells, steps, seed=2, n_probe=200):
@@ -85,20 +105,23 @@ def cpu_gradient_baseline(case, probe, target, seconds):
     import pyoracle
 
     orc = pyoracle.Oracle(case)
-    pyoracle.set_threads(1)
     dn = np.ones(case.N) * 1e-3
-    t0 = time.perf_counter()
-    orc.run(case.Q0, case.dt, 1)
-    orc.jvp_run(case.Q0, case.dt, 1, dn=dn, probe=probe, target=target)
-    per = time.perf_counter() - t0
-    n = max(1, min(200, int(seconds / max(per, 1e-3))))
-    t0 = time.perf_counter()
-    orc.run(case.Q0, case.dt, n)
-    orc.jvp_run(case.Q0, case.dt, n, dn=dn, probe=probe, target=target)
-    el = time.perf_counter() - t0
-    return {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle forward + Dual tangent sweep "
-                      "(lower bound on a CPU reverse-mode gradient), 1 thread"}
+
+    def run(k):
+        orc.run(case.Q0, case.dt, k)
+        orc.jvp_run(case.Q0, case.dt, k, dn=dn, probe=probe, target=target)
+
+    pyoracle.set_threads(1)
+    n, el = timed_sample(run, seconds, 200)
+    out = {"value": case.N * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle forward + Dual tangent sweep "
+                     "(lower bound on a CPU reverse-mode gradient), 1 thread"}
+    nt = host_threads()
+    pyoracle.set_threads(nt)
+    n, el = timed_sample(run, seconds / 4, 200)
+    pyoracle.set_threads(1)
+    out["all_cores"] = {"value": case.N * n / el, "cores": nt, "sample": f"{n} steps, {nt} threads"}
+    return out
 
 
 def measure_gradient(args, capi):
@@ -125,6 +148,7 @@ def measure_gradient(args, capi):
         cb = cpu_gradient_baseline(c, probe, target, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
+        out["speedup_vs_cpu_all_cores"] = out["value"] / cb["all_cores"]["value"]
     return out
 
 
