@@ -245,6 +245,14 @@ struct hg_ctx {
   std::vector<int32_t> h_deg, h_nbr, h_gkind, h_gbc;
   std::vector<double> h_nx, h_ny, h_len;
   double* scratch = nullptr;  // 3N doubles for host transfers / params
+  // host <-> device transfers in caller order: a 3N staging buffer and the
+  // permutation (library cell i is caller cell perm[i]) on the device
+  double* stage = nullptr;
+  int32_t* d_perm = nullptr;
+  // probe-loss buffers of hg_grad_probes (capacity probe_cap probes)
+  int32_t* probe_i = nullptr;
+  double* probe_d = nullptr;
+  int64_t probe_cap = 0;
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials
 

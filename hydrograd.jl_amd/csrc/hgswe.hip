@@ -505,6 +505,52 @@ __global__ __launch_bounds__(kBlock) void k_cfl(int N, const double* xi, const d
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Caller order -> library order: d_c[i] = src[c N + perm[i]] for the
+// ncomp (1 or 3) components.
+__global__ __launch_bounds__(kBlock) void k_perm_in(int64_t N, int ncomp, const int32_t* __restrict__ perm,
+                                                   const double* __restrict__ src, double* d0, double* d1,
+                                                   double* d2) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t p = perm[i];
+  d0[i] = src[p];
+  if (ncomp > 1) {
+    d1[i] = src[N + p];
+    d2[i] = src[2 * N + p];
+  }
+}
+
+// Library order -> caller order: dst[c N + perm[i]] = s_c[i].
+__global__ __launch_bounds__(kBlock) void k_perm_out(int64_t N, int ncomp, const int32_t* __restrict__ perm,
+                                                    const double* s0, const double* s1, const double* s2,
+                                                    double* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t p = perm[i];
+  dst[p] = s0[i];
+  if (ncomp > 1) {
+    dst[N + p] = s1[i];
+    dst[2 * N + p] = s2[i];
+  }
+}
+
+// Water-surface residuals at the probe cells (library order):
+// r_k = (xi + h_still) + zb - target_k (swe_2D_inversion.jl:387-399).
+__global__ __launch_bounds__(kBlock) void k_probe_resid(int n, const int32_t* cell, const double* target,
+                                                       const double* xi, const double* hstill, const double* zb,
+                                                       double* r) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int c = cell[k];
+  r[k] = xi[c] + hstill[c] + zb[c] - target[k];
+}
+
+// lam[cell[k]] = val[k] over distinct cells.
+__global__ __launch_bounds__(kBlock) void k_seed(int n, const int32_t* cell, const double* val, double* lam) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) lam[cell[k]] = val[k];
+}
+
 // Plain streaming copy (16 B per lane per iteration): the reference kernel
 // of known traffic used to calibrate rocprof's FETCH_SIZE / WRITE_SIZE and to
 // measure the achievable HBM bandwidth.
@@ -2058,13 +2104,12 @@ int zero_adj_accumulators(hg_ctx* ctx) {
   return HG_OK;
 }
 
+int perm_out(hg_ctx* ctx, const double* const s[3], int ncomp, double* dst);
+
 // Per-cell device array (library order) -> host array (caller order).
 int cell_out(hg_ctx* ctx, const double* dsrc, double* hdst) {
-  std::vector<double> tmp(ctx->N);
-  HG_TRY(hipMemcpyAsync(tmp.data(), dsrc, ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HG_TRY(hipStreamSynchronize(ctx->stream));
-  for (int64_t i = 0; i < ctx->N; ++i) hdst[ctx->perm[i]] = tmp[i];
-  return HG_OK;
+  const double* const s[3] = {dsrc, dsrc, dsrc};
+  return perm_out(ctx, s, 1, hdst);
 }
 
 double* seg_state(hg_ctx* ctx, int64_t t, int c, int b = 0) { return ctx->adj_seg[b] + (t * 3 + c) * ctx->N; }
@@ -2188,33 +2233,56 @@ int enqueue_finite_check(hg_ctx* ctx) {
 }
 
 // Host 3N state (caller order) -> device state `slot` (library order).
-int state_in(hg_ctx* ctx, const double* Q, int slot) {
+// The staging buffer and device permutation of the transfers below
+// (allocated on first use).
+int ensure_xfer(hg_ctx* ctx) {
+  if (ctx->stage) return HG_OK;
+  int rc = dalloc(ctx, &ctx->d_perm, size_t(ctx->N));
+  if (!rc) rc = upload(ctx, ctx->d_perm, ctx->perm.data(), size_t(ctx->N));
+  if (!rc) rc = dalloc(ctx, &ctx->stage, size_t(3) * ctx->N);
+  return rc;
+}
+
+// Host array of ncomp x N values in caller order -> device arrays d[c] in
+// library order: one contiguous copy, then the permutation on the device.
+int perm_in(hg_ctx* ctx, const double* src, int ncomp, double* const d[3]) {
   const int64_t N = ctx->N;
-  std::vector<double> tmp(3 * N);
-  for (int c = 0; c < 3; ++c)
-    for (int64_t i = 0; i < N; ++i) tmp[c * N + i] = Q[c * N + ctx->perm[i]];
-  for (int c = 0; c < 3; ++c)
-    HG_TRY(hipMemcpyAsync(ctx->st[slot][c], tmp.data() + c * N, N * sizeof(double), hipMemcpyHostToDevice,
-                          ctx->stream));
+  int rc = ensure_xfer(ctx);
+  if (rc) return rc;
+  HG_TRY(hipMemcpyAsync(ctx->stage, src, size_t(ncomp) * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_perm_in, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, ncomp, ctx->d_perm, ctx->stage,
+                     d[0], d[ncomp > 1 ? 1 : 0], d[ncomp > 1 ? 2 : 0]);
+  HG_TRY(hipGetLastError());
   HG_TRY(hipStreamSynchronize(ctx->stream));
   return HG_OK;
+}
+
+// Device arrays s[c] in library order -> host array in caller order.
+int perm_out(hg_ctx* ctx, const double* const s[3], int ncomp, double* dst) {
+  const int64_t N = ctx->N;
+  int rc = ensure_xfer(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_perm_out, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, ncomp, ctx->d_perm, s[0],
+                     s[ncomp > 1 ? 1 : 0], s[ncomp > 1 ? 2 : 0], ctx->stage);
+  HG_TRY(hipGetLastError());
+  HG_TRY(hipMemcpyAsync(dst, ctx->stage, size_t(ncomp) * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
+int state_in(hg_ctx* ctx, const double* Q, int slot) {
+  double* const d[3] = {ctx->st[slot][0], ctx->st[slot][1], ctx->st[slot][2]};
+  return perm_in(ctx, Q, 3, d);
 }
 
 int state_out(hg_ctx* ctx, double* const src[3], double* Q) {
-  const int64_t N = ctx->N;
-  std::vector<double> tmp(3 * N);
-  for (int c = 0; c < 3; ++c)
-    HG_TRY(hipMemcpyAsync(tmp.data() + c * N, src[c], N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HG_TRY(hipStreamSynchronize(ctx->stream));
-  for (int c = 0; c < 3; ++c)
-    for (int64_t i = 0; i < N; ++i) Q[c * N + ctx->perm[i]] = tmp[c * N + i];
-  return HG_OK;
+  const double* const s[3] = {src[0], src[1], src[2]};
+  return perm_out(ctx, s, 3, Q);
 }
 
 int cell_in(hg_ctx* ctx, double* dst, const double* src) {  // per-cell host array -> device (lib order)
-  std::vector<double> tmp(ctx->N);
-  for (int64_t i = 0; i < ctx->N; ++i) tmp[i] = src[ctx->perm[i]];
-  return upload(ctx, dst, tmp.data(), ctx->N);
+  double* const d[3] = {dst, dst, dst};
+  return perm_in(ctx, src, 1, d);
 }
 
 // Working copies of zb / n / S0 / zb_ghost / inletQ <- the values given to
@@ -3908,12 +3976,8 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
 // ------------------------------------------------------------ adjoint ABI --
 static int adj_upload_lam(hg_ctx* ctx, const double* hbar, double* dst) {
   const int64_t N = ctx->N;
-  std::vector<double> tmp(3 * N);
-  for (int c = 0; c < 3; ++c)
-    for (int64_t i = 0; i < N; ++i) tmp[c * N + i] = hbar[c * N + ctx->perm[i]];
-  HG_TRY(hipMemcpyAsync(dst, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  HG_TRY(hipStreamSynchronize(ctx->stream));
-  return HG_OK;
+  double* const d[3] = {dst, dst + N, dst + 2 * N};
+  return perm_in(ctx, hbar, 3, d);
 }
 
 static int adj_download(hg_ctx* ctx, const double* lam_dev, double* Q_bar, double* n_bar, double* zb_bar,
@@ -3999,23 +4063,55 @@ int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
     }
   int rc = adj_run(ctx, Q0, params, n_params, dt, nsteps, kck);
   if (rc) return rc;
-  // WSE = (xi + h_still) + zb at the probes (swe_2D_inversion.jl:387-399)
-  std::vector<double> QN(3 * N), hs(N), zb(N);
-  rc = hg_state_download(ctx, QN.data());
-  if (!rc) rc = cell_out(ctx, ctx->hstill, hs.data());
-  if (!rc) rc = cell_out(ctx, ctx->zb, zb.data());
-  if (rc) return rc;
-  std::vector<double> lamN(3 * N, 0.0), r(n_probe);
+  // WSE = (xi + h_still) + zb at the probes (swe_2D_inversion.jl:387-399),
+  // formed on the device: only the probe cells travel
+  if (n_probe > ctx->probe_cap) {
+    rc = dalloc(ctx, &ctx->probe_i, size_t(n_probe));
+    if (!rc) rc = dalloc(ctx, &ctx->probe_d, size_t(3) * n_probe);
+    if (rc) return rc;
+    ctx->probe_cap = n_probe;
+  }
+  const int64_t cap = ctx->probe_cap;
+  std::vector<int32_t> pc(n_probe);
+  for (int64_t k = 0; k < n_probe; ++k) pc[k] = ctx->iperm[probe[k]];
+  std::vector<double> r(n_probe);
+  if (n_probe > 0) {
+    rc = upload(ctx, ctx->probe_i, pc.data(), size_t(n_probe));
+    if (!rc) rc = upload(ctx, ctx->probe_d, target, size_t(n_probe));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_probe_resid, dim3(grid_of(n_probe)), dim3(kBlock), 0, ctx->stream, int(n_probe),
+                       ctx->probe_i, ctx->probe_d, ctx->st[ctx->cur][0], ctx->hstill, ctx->zb, ctx->probe_d + cap);
+    HG_TRY(hipGetLastError());
+    HG_TRY(hipMemcpyAsync(r.data(), ctx->probe_d + cap, n_probe * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  // the loss, and its cotangent of the final state: 2 r at each probe cell
+  // (summed over repeated probes, in probe order), zero elsewhere
   double L = 0.0;
+  std::vector<int32_t> sc;
+  std::vector<double> sv;
+  std::unordered_map<int32_t, size_t> at;
   for (int64_t k = 0; k < n_probe; ++k) {
-    const int32_t p = probe[k];
-    r[k] = QN[p] + hs[p] + zb[p] - target[k];
     L = L + r[k] * r[k];
-    lamN[p] += 2.0 * r[k];
+    auto ins = at.emplace(pc[k], sc.size());
+    if (ins.second) {
+      sc.push_back(pc[k]);
+      sv.push_back(0.0);
+    }
+    sv[ins.first->second] += 2.0 * r[k];
   }
   *loss = L;
-  rc = adj_upload_lam(ctx, lamN.data(), ctx->adj_lam[0]);
-  if (!rc) rc = zero_adj_accumulators(ctx);
+  HG_TRY(hipMemsetAsync(ctx->adj_lam[0], 0, size_t(3) * N * sizeof(double), ctx->stream));
+  if (!sc.empty()) {
+    rc = upload(ctx, ctx->probe_i, sc.data(), sc.size());
+    if (!rc) rc = upload(ctx, ctx->probe_d + 2 * cap, sv.data(), sv.size());
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_seed, dim3(grid_of(int64_t(sc.size()))), dim3(kBlock), 0, ctx->stream, int(sc.size()),
+                       ctx->probe_i, ctx->probe_d + 2 * cap, ctx->adj_lam[0]);
+    HG_TRY(hipGetLastError());
+  }
+  rc = zero_adj_accumulators(ctx);
   int cur = 0;
   if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
   if (!rc) rc = finish_adjoint(ctx);
