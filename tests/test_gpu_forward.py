@@ -218,12 +218,40 @@ def test_conveyance_error_is_reported():
 
 def test_full_size_channel_parity():
     """The 1M-triangle channel workload: one RHS and 10 steps against the
-    oracle, plus mass-balance bookkeeping of a longer device run."""
+    oracle."""
     c = cases.channel(1_000_000, seed=0, nsteps=10)
     say(f"  full-size channel: N={c.N} G={c.G} dt={c.dt}")
     ctx, orc = capi.Context(c), pyoracle.Oracle(c)
     check_rhs(c, c.Q0, ctx, orc)
     check_steps(c, c.Q0, c.dt, 10, ctx, orc)
+
+
+def test_full_size_thousand_steps_properties():
+    """The benchmark workload at full size, 1,000 steps on the device, checked
+    through properties that do not need the oracle's run time: the per-cell
+    kernel (the reference's cell-by-cell loop shape) and the tiled kernel
+    with two launch shapes give bitwise equal states; a repeat is bitwise
+    equal; every value is finite; the dry clamp holds (xi >= h_small, and a
+    clamped cell carries no discharge)."""
+    c = cases.channel(1_000_000, seed=0, nsteps=1000)
+    n = 1000
+    ctx = capi.Context(c)
+    ctx.set_tile_config(256, 16)
+    q = ctx.step(c.Q0, c.dt, n)
+    assert np.array_equal(ctx.step(c.Q0, c.dt, n), q)
+    ctx.set_tile_config(128, 0)
+    assert np.array_equal(ctx.step(c.Q0, c.dt, n), q)
+    assert np.array_equal(capi.Context(c, cell_kernel=True).step(c.Q0, c.dt, n), q)
+    N = c.N
+    xi, qx, qy = q[:N], q[N:2 * N], q[2 * N:]
+    assert np.all(np.isfinite(q))
+    assert np.all(xi >= c.h_small)
+    clamped = xi == c.h_small
+    assert np.all(qx[clamped] == 0.0) and np.all(qy[clamped] == 0.0)
+    moved = float(np.max(np.abs(q - c.Q0)))
+    say(f"  1M channel, {n} steps: shapes and per-cell kernel bitwise equal; clamped cells {int(clamped.sum())}; "
+        f"max change {moved:.3e}")
+    assert moved > 0.0
 
 
 def decagon_fan():
