@@ -348,6 +348,9 @@ inline void friction_terms(const Consts& c, const T& h, const T& q_x, const T& q
 
 // FlowResistance branch (:521-531): the UDE gives the friction magnitude m,
 //   mag_q = smooth_sqrt(q_x^2 + q_y^2), friction = m * q / mag_q.
+// Pinned by closed forms (tests/test_lux_oracle.py): opposite to q, magnitude
+// m for |q| >> sqrt(eps) and m |q| / sqrt(eps) for |q| << sqrt(eps), zero at
+// q = 0 and on dry cells, linear in m.
 template <class T>
 inline void friction_terms_mag(const T& m, const T& q_x, const T& q_y, T& fx, T& fy) {
   const T mag_q = smooth_sqrt(q_x * q_x + q_y * q_y);

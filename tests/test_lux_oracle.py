@@ -92,3 +92,59 @@ def test_input_normalisation():
     fr = lx.Net(lx.FLOW_RESISTANCE, [2], [lx.TANH], (0, 1), (0.5, 2.0), (0.0, 1.0))  # |q| bounds 0 .. 2
     F = lx.features(fr, Q, hstill, 1e-3)
     assert abs(F[1][0] - 0.0) < 1e-15 and abs(F[1][1] - 3.0) < 1e-15 and abs(F[1][2] + 1.0) < 1e-7
+
+
+def test_flow_resistance_friction_closed_forms():
+    """Pins of the oracle's FlowResistance friction (semi_discretize_swe_2D.jl:
+    521-531: f = m q / sqrt(q_x^2 + q_y^2 + eps), masked on dry cells by the
+    source term) through what the mathematics fixes, from D = RHS(m) - RHS(0),
+    the friction-only change of the RHS:
+    * the mass row does not change (exactly);
+    * D opposes q (antiparallel, D . q < 0);
+    * |D| = m for |q| >> sqrt(eps), and |D| = m |q| / sqrt(eps) for |q| << sqrt(eps)
+      (the eps sits inside the square root);
+    * q = 0 or a dry cell gives D = 0 exactly; D is linear in m.
+    A dropped or misplaced eps, a wrong sign, swapped components or a missing
+    dry mask each fails one of these."""
+    import pyoracle
+    from hgswe import cases
+
+    c = cases.dam_break(nx=40, ny=4)
+    N = c.N
+    rng = np.random.default_rng(21)
+    h = np.full(N, 0.8)
+    h[:4] = c.h_small * 0.5  # dry cells (h <= h_small)
+    mag = np.empty(N)
+    mag[4:14] = 0.0  # zero discharge
+    mag[14:24] = 1e-11  # |q| << sqrt(eps) ~ 1.5e-8
+    mag[24:] = rng.uniform(0.05, 2.0, N - 24)
+    mag[:4] = 0.3
+    ang = rng.uniform(0.0, 2.0 * np.pi, N)
+    qx, qy = mag * np.cos(ang), mag * np.sin(ang)
+    Q = np.concatenate([h - c.hstill, qx, qy])
+    m = rng.uniform(0.01, 0.1, N)
+    orc = pyoracle.Oracle(c)
+
+    def rhs(fm):
+        orc.set_fields(fmag=fm)
+        return orc.rhs(Q)
+
+    r0, r1, r2 = rhs(np.zeros(N)), rhs(m), rhs(2.0 * m)
+    D, D2 = r1 - r0, r2 - r0
+    Dx, Dy = D[N:2 * N], D[2 * N:]
+    assert np.array_equal(D[:N], np.zeros(N))
+    dry, zero, tiny, big = slice(0, 4), slice(4, 14), slice(14, 24), slice(24, N)
+    assert np.all(Dx[dry] == 0.0) and np.all(Dy[dry] == 0.0)
+    assert np.all(Dx[zero] == 0.0) and np.all(Dy[zero] == 0.0)
+    # large |q|: magnitude m, antiparallel to q
+    Dm = np.hypot(Dx[big], Dy[big])
+    assert np.max(np.abs(Dm / m[big] - 1.0)) < 1e-9
+    assert np.all(Dx[big] * qx[big] + Dy[big] * qy[big] < 0.0)
+    cross = np.abs(Dx[big] * qy[big] - Dy[big] * qx[big]) / (Dm * mag[big])
+    assert np.max(cross) < 1e-9
+    # small |q|: the linear regime of the regularised magnitude
+    Dt = np.hypot(Dx[tiny], Dy[tiny])
+    ratio = Dt / (m[tiny] * mag[tiny] / np.sqrt(lx.EPS))
+    assert np.max(np.abs(ratio - 1.0)) < 1e-5
+    # linear in m
+    assert np.allclose(D2[N:], 2.0 * D[N:], rtol=1e-9, atol=1e-15)
