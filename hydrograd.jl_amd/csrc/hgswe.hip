@@ -2524,8 +2524,7 @@ int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
 extern "C" {
 
 const char* hg_info(void) {
-  static std::string s;
-  if (s.empty()) {
+  static const std::string s = [] {  // formed once (thread-safe static initialisation)
     int dev = 0;
     hipDeviceProp_t p{};
     char buf[512];
@@ -2534,8 +2533,8 @@ const char* hg_info(void) {
                p.gcnArchName, p.multiProcessorCount, double(p.totalGlobalMem) / 1e9);
     else
       snprintf(buf, sizeof(buf), "libhgswe: no HIP device");
-    s = buf;
-  }
+    return std::string(buf);
+  }();
   return s.c_str();
 }
 
