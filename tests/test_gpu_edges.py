@@ -197,3 +197,54 @@ def test_calls_before_fields_are_refused():
     ctx.set_fields(c)
     ref = capi.Context(c).step(c.Q0, c.dt, 5)
     assert np.array_equal(ctx.step(c.Q0, c.dt, 5), ref)
+
+
+def dry_exit_partial_inlet():
+    """A channel whose exit water surface (-1 m) lies below the bed of every
+    exit cell, so the exit-H ghost is clamped to h_small (the wet side's
+    outflow branches of the Roe flux), and whose inlet reach (x < 100 m) is
+    dry over its upper half, so the inlet conveyance and ghost velocities
+    carry w = 0 terms.  The deep front against the dry cells makes the
+    explicit scheme unstable from the first step on: thin layers with h just
+    above h_small take large velocities, and the reference's arithmetic does
+    the same.  So the checks take the RHS, one step and its gradient."""
+    c = cases.channel(6_000, seed=5, Lx=1500.0, Ly=150.0, Q_in=60.0, nsteps=1)
+    N = c.N
+    x, y = c.mesh.centroid[:, 0], c.mesh.centroid[:, 1]
+    Q0 = c.Q0.copy()
+    Q0[:N] = np.where((x < 100.0) & (y > 75.0), c.h_small - c.hstill, Q0[:N])
+    c = c.copy_fields(Q0=Q0, exitWSE=np.array([-1.0]), name="dry_exit_partial_inlet")
+    return c.copy_fields(dt=cases.cfl_dt(c, 0.4, Q0))
+
+
+def test_dry_exit_and_partly_dry_inlet():
+    c = dry_exit_partial_inlet()
+    m = c.mesh
+    N = c.N
+    exit_cells = m.ghost_cell[m.ghost_kind == hm.BC_EXITH]
+    inlet_cells = m.ghost_cell[m.inlet_ghost]
+    h0 = c.Q0[:N] + c.hstill
+    assert np.all(c.exitWSE[0] - c.zb[exit_cells] < c.h_small)
+    assert np.any(h0[inlet_cells] <= c.h_small) and np.any(h0[inlet_cells] > c.h_small)
+    ctx, orc = capi.Context(c), pyoracle.Oracle(c)
+    check_rhs(c, c.Q0, ctx, orc)
+    qg, _ = check_steps(c, c.Q0, c.dt, c.nsteps, ctx, orc)
+    assert np.array_equal(qg, capi.Context(c, cell_kernel=True).step(c.Q0, c.dt, c.nsteps))
+    vjp_check(c, c.Q0, seed=17)
+    # probe-loss gradient against the oracle's Dual run
+    nsteps, kck = c.nsteps, 1
+    rng = np.random.default_rng(18)
+    # after one step n and the inlet discharge reach the water levels only
+    # through the inlet cells, so the probes include them (wet and dry)
+    probe = np.unique(np.concatenate([rng.choice(N, size=30, replace=False), inlet_cells])).astype(np.int32)
+    QT = orc.run(c.Q0, c.dt, nsteps)
+    target = QT[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.01, probe.size)
+    loss, gn, gzb, gq = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    for name, tg, g in (("n", dict(dn=rng.normal(size=N) * 1e-3), gn), ("zb", dict(dzb=rng.normal(size=N) * 1e-2), gzb),
+                        ("Qin", dict(dQin=rng.normal(size=m.n_inlet)), gq)):
+        _, _, L, dL = orc.jvp_run(c.Q0, c.dt, nsteps, probe=probe, target=target, **tg)
+        gv = float(np.dot(g, list(tg.values())[0]))
+        say(f"  {c.name} grad [{name}]: g.v {gv:.12e} oracle {dL:.12e}")
+        assert abs(loss - L) <= 1e-10 * abs(L)
+        assert dL != 0.0
+        assert abs(gv - dL) <= 1e-8 * abs(dL)
