@@ -278,8 +278,8 @@ def main():
                          "(calibrates rocprof FETCH_SIZE / WRITE_SIZE against a kernel of known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU domain decomposition "
-              "is design-only in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a run over several GPUs "
+              "(domain decomposition with halo exchange) is not provided", file=sys.stderr)
         sys.exit(2)
 
     from hgswe import capi, cases

@@ -3,7 +3,7 @@
 * The matrix-core lane layout (16x16x4, f32 and f64) against A @ B.
 * The closure's n against a NumPy statement of the same network (f64:
   1e-13 relative; f32: 1e-5).
-* The per-cell-n solve is the judge of the step: a step with the closure
+* The per-cell-n solve is the reference for the step: a step with the closure
   on is bitwise the step of the per-cell-n path given the closure's n, and
   N steps match the oracle driven by the NumPy closure (1e-10).
 * Reverse mode through the closure (f64): weight and initial-state
