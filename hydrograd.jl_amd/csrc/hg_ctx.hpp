@@ -161,7 +161,9 @@ struct AdjArgs {
   double g, half_g, kn2, hsmall, dt;
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
   const double* fmag; // FlowResistance closure: friction magnitude per cell (else NULL) ...
-  double* fbar;       // ... and += its cotangent
+  double* fbar;       // ... and its cotangent
+  int32_t nbar_set;   // 1: nbar holds this evaluation's n-bar alone (set, not added to)
+  int32_t fbar_set;   // 1: fbar likewise
 };
 
 // A variable Manning-n law (kind HG_NLAW_*; 0: off) and its parameters.

@@ -746,11 +746,17 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
                                                 double hb, double qxb, double qyb, double nb, double zbb) {
   const hg::CellQ& me = ca.me;
   double S0xb = 0.0, S0yb = 0.0;
-  if (ca.wet && a.fmag) {
+  if (a.fmag) {
     double mb = 0.0;
-    hg::source_vjp_mag(a.g, ca.xi, me.hu, me.hv, a.fmag[i], a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, qxb, qyb, mb,
-                       S0xb, S0yb);
-    a.fbar[i] += mb;
+    if (ca.wet)
+      hg::source_vjp_mag(a.g, ca.xi, me.hu, me.hv, a.fmag[i], a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, qxb, qyb, mb,
+                         S0xb, S0yb);
+    // this step's cotangent of the friction magnitude: set (the sum with 0.0
+    // gives the bits of an add into a cleared array), or summed over the steps
+    if (a.fbar_set)
+      a.fbar[i] = mb + 0.0;
+    else if (ca.wet)
+      a.fbar[i] += mb;
   } else if (ca.wet) {
     hg::source_vjp(a.g, a.kn2, a.hsmall, ca.xi, me.h, me.hu, me.hv, ca.n, a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, hb,
                    qxb, qyb, nb, S0xb, S0yb);
@@ -770,7 +776,7 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
   a.ox[i] = ox;
   a.oqx[i] = oqx;
   a.oqy[i] = oqy;
-  if (a.nbar) a.nbar[i] += nb;
+  if (a.nbar) a.nbar[i] = a.nbar_set ? nb + 0.0 : a.nbar[i] + nb;
   if (a.zbbar) a.zbbar[i] += zbb;
   if (a.S0bar) {  // summed over the steps; the bed-slope transpose is applied once at the end
     a.S0bar[i] += S0xb;
@@ -1135,7 +1141,7 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
     Wb[dst] = sacc;
   }
   __syncthreads();
-  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] = Wb[k];
+  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] += Wb[k];
 }
 
 // wbar[k] += sum over blocks b of wpart[k][b]; one work-group per weight,
@@ -1270,7 +1276,7 @@ __global__ __launch_bounds__(64) void k_lux_vjp(hg::LuxArgs a) {
     }
   }
   __syncthreads();
-  for (int k = lane; k < P; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] = acc[k];
+  for (int k = lane; k < P; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] += acc[k];
 }
 
 // Layout check of the 16x16x4 matrix-core products used above: D = A B for
@@ -2010,20 +2016,19 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     ctx->err = "the UDE closure's reverse mode takes one weight set";
     return HG_ERR_STATE;
   }
+  // the closure's, law's or network's n-bar (or friction-magnitude bar) of
+  // this step is set by the adjoint kernel, not cleared and added to
   if (ude) {
     int rc = enqueue_ude_n(ctx, q);
     if (rc) return rc;
-    HG_TRY(hipMemsetAsync(ctx->ude_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   }
   if (law) {
     int rc = enqueue_nlaw_n(ctx, q);
     if (rc) return rc;
-    HG_TRY(hipMemsetAsync(ctx->nlaw_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   }
   if (lux) {
     int rc = enqueue_lux_fwd(ctx, q);
     if (rc) return rc;
-    HG_TRY(hipMemsetAsync(ctx->lux_obar, 0, ctx->N * sizeof(double), ctx->stream));
   }
   if (ctx->n_inlet > 0 && !totalA) {
     hipLaunchKernelGGL(k_inlet_conveyance, dim3(ctx->n_inlet), dim3(kBlock), 0, ctx->stream, inlet_args_p(ctx, q[0]));
@@ -2031,6 +2036,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   }
   hg::AdjArgs a = make_adj_args(ctx, q, lam, mask, out, dt);
   a.nbar = nbar;
+  a.nbar_set = (ude || law || lux_n) ? 1 : 0;
+  a.fbar_set = (lux && !lux_n) ? 1 : 0;
   if (totalA) a.totalA = totalA;
   if (step)
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
@@ -2054,8 +2061,6 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
       hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
     else
       hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
-    hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW), dim3(kBlock), 0, ctx->stream, ude_blocks(ctx), ctx->ude_wpart,
-                       ctx->ude_wbar);
   }
   if (law) {  // through the law: n(Q) adds to the state cotangent
     hg::NLawArgs u = nlaw_args(ctx, q);
@@ -2077,25 +2082,39 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
       hipLaunchKernelGGL(k_lux_vjp<4>, dim3(nb), dim3(64), 0, ctx->stream, u);
     else
       hipLaunchKernelGGL(k_lux_vjp<hg::kLuxMaxW>, dim3(nb), dim3(64), 0, ctx->stream, u);
-    hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, nb, ctx->lux_wpart,
-                       ctx->lux_wbar);
   }
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
 
 // After the last reverse step: the bed-slope transpose of the summed S0
-// cotangent (the map zb -> S0 does not change over the steps).
+// cotangent (the map zb -> S0 does not change over the steps), and the
+// weight cotangents of the closure or network: each reverse-step block
+// summed its share over the steps into its own slot; the slots are summed
+// here by a fixed tree.
 int finish_adjoint(hg_ctx* ctx) {
   hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
                      ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+  if (ctx->ude_prec)
+    hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW), dim3(kBlock), 0, ctx->stream, ude_blocks(ctx), ctx->ude_wpart,
+                       ctx->ude_wbar);
+  if (ctx->lux.choice)
+    hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, lux_vjp_blocks(ctx),
+                       ctx->lux_wpart, ctx->lux_wbar);
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
 
 int zero_adj_accumulators(hg_ctx* ctx) {
-  if (ctx->ude_prec) HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
-  if (ctx->lux.choice) HG_TRY(hipMemsetAsync(ctx->lux_wbar, 0, ctx->lux.P * sizeof(double), ctx->stream));
+  if (ctx->ude_prec) {
+    HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
+    HG_TRY(hipMemsetAsync(ctx->ude_wpart, 0, size_t(hg::kUdeW) * ude_blocks(ctx) * sizeof(double), ctx->stream));
+  }
+  if (ctx->lux.choice) {
+    HG_TRY(hipMemsetAsync(ctx->lux_wbar, 0, ctx->lux.P * sizeof(double), ctx->stream));
+    HG_TRY(hipMemsetAsync(ctx->lux_wpart, 0, size_t(ctx->lux.P) * lux_vjp_blocks(ctx) * sizeof(double),
+                          ctx->stream));
+  }
   HG_TRY(hipMemsetAsync(ctx->adj_S0bar, 0, 2 * ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
