@@ -204,8 +204,9 @@ def measure_batch(args, capi, single_value):
 def measure_ude(args, capi):
     """UDE closure workload: ~50k-triangle channel, n = 0.01 + softplus(MLP(h,
     |u|, zb)) in f32 on the matrix cores every step; 32 perturbed weight sets
-    advanced together as a batch; gradient of the probe loss w.r.t. the 353
-    weights."""
+    advanced together as a batch, forward and in one reverse sweep that gives
+    every set's weight gradient; gradient of the probe loss w.r.t. the 353
+    weights of one set."""
     from hgswe import cases, ude
 
     c = cases.channel(args.ude_cells, seed=4, nsteps=args.ude_steps)
@@ -232,9 +233,26 @@ def measure_ude(args, capi):
     t0 = time.perf_counter()
     bctx.advance(B.dt, args.ude_steps)
     el = time.perf_counter() - t0
-    del bctx
     out["batched_sets"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "sets": nset,
                            "cells": B.N, "ms_per_step": el * 1e3 / args.ude_steps}
+    # one reverse sweep over the batch: the gradient of each set's own probe
+    # loss (200 probes per instance) w.r.t. its 353 weights, all 32 at once
+    rng = np.random.default_rng(6)
+    bprobe = np.concatenate([r * c.N + rng.choice(c.N, size=200, replace=False) for r in range(nset)])
+    bprobe = bprobe.astype(np.int32)
+    QB = bctx.step(B.Q0, B.dt, args.ude_steps)
+    btarget = QB[bprobe] + B.hstill[bprobe] + B.zb[bprobe] + rng.normal(0.0, 0.02, bprobe.size)
+    bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget)  # warm-up
+    t0 = time.perf_counter()
+    bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget)
+    gws = bctx.ude_grad()
+    el = time.perf_counter() - t0
+    del bctx
+    out["batched_sets"]["gradient"] = {
+        "value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+        "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "probes": int(bprobe.size),
+        "wrt": f"{nset} x 353 closure weights (each set's own instance loss)",
+        "grad_norm_min": float(np.min(np.linalg.norm(gws.reshape(nset, -1), axis=1)))}
     # gradient w.r.t. the weights
     rng = np.random.default_rng(5)
     probe = rng.choice(c.N, size=200, replace=False).astype(np.int32)

@@ -1032,10 +1032,11 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
   }
 }
 
-// Reverse mode of k_ude_n for one weight set: given the cotangent of n,
-// adds the cotangents of the features to the state cotangent (through the
-// clamp and u = q / h, as the forward forms them) and writes this block's
-// 353 weight cotangents to wpart[blockIdx.x].
+// Reverse mode of k_ude_n: given the cotangent of n, adds the cotangents of
+// the features to the state cotangent (through the clamp and u = q / h, as
+// the forward forms them) and adds this block's weight cotangents into its
+// slot for each weight set its 64 cells use (one pass per distinct set, in
+// order of first appearance, as k_ude_n).
 template <class T>
 __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
   using M = hg::Mfma<T>;
@@ -1052,113 +1053,128 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
   const bool valid = i < a.N;
   double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
   bool wet = false;
+  int set = 0;
   if (valid) {
     hg::ude_features(a, i, h, u, v, um, wet);
     zb = a.zb[i];
+    if (a.set_of_cell) set = a.set_of_cell[i];
   }
   F[l * 4 + 0] = T(h);
   F[l * 4 + 1] = T(um);
   F[l * 4 + 2] = T(zb);
   F[l * 4 + 3] = T(0);
   hg::wave_sync();
-  const T* W = static_cast<const T*>(a.w);
-  ude_hidden<T>(W, F, A1, A2, l);
-  // output layer and softplus, lane = cell
-  const T z3 = ude_out<T>(W, A2, l);
-  const T z3b = valid ? T(a.nbar[i]) * hg::sigmoid<T>(z3) : T(0);
-  Z3[l] = z3b;
-  for (int j = 0; j < 16; ++j) {
-    const T a2 = A2[l * 16 + j];
-    Z2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
-  }
-  hg::wave_sync();
-  // A1bar = Z2 W2 (rows = cells), Z1 = A1bar (1 - A1^2)
-  for (int rb = 0; rb < 4; ++rb) {
-    typename M::V acc = {T(0), T(0), T(0), T(0)};
-    for (int kc = 0; kc < 4; ++kc) {
-      const T av = Z2[(16 * rb + col) * 16 + 4 * kc + kq];           // Z2[16 rb + i][4 kc + k]
-      const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];         // W2[4 kc + k][j]
-      acc = M::op(av, bv, acc);
+  uint64_t pending = __ballot(valid);
+  for (int pass = 0; pending; ++pass) {
+    const int lead = __ffsll((unsigned long long)pending) - 1;
+    const int s = __shfl(set, lead, 64);
+    const bool mine = valid && set == s;
+    pending &= ~uint64_t(__ballot(mine));
+    const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
+    ude_hidden<T>(W, F, A1, A2, l);
+    // output layer and softplus, lane = cell; cells of other sets carry a zero cotangent
+    const T z3 = ude_out<T>(W, A2, l);
+    const T z3b = mine ? T(a.nbar[i]) * hg::sigmoid<T>(z3) : T(0);
+    Z3[l] = z3b;
+    for (int j = 0; j < 16; ++j) {
+      const T a2 = A2[l * 16 + j];
+      Z2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
     }
-    for (int r = 0; r < 4; ++r) {
-      const int row = 16 * rb + M::row(kq, r);
-      const T a1 = A1[row * 16 + col];
-      Z1[row * 16 + col] = acc[r] * (T(1) - a1 * a1);
+    hg::wave_sync();
+    // A1bar = Z2 W2 (rows = cells), Z1 = A1bar (1 - A1^2)
+    for (int rb = 0; rb < 4; ++rb) {
+      typename M::V acc = {T(0), T(0), T(0), T(0)};
+      for (int kc = 0; kc < 4; ++kc) {
+        const T av = Z2[(16 * rb + col) * 16 + 4 * kc + kq];           // Z2[16 rb + i][4 kc + k]
+        const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];         // W2[4 kc + k][j]
+        acc = M::op(av, bv, acc);
+      }
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * rb + M::row(kq, r);
+        const T a1 = A1[row * 16 + col];
+        Z1[row * 16 + col] = acc[r] * (T(1) - a1 * a1);
+      }
     }
-  }
-  hg::wave_sync();
-  // feature cotangents -> state cotangent (lane = cell)
-  T fb0 = T(0), fb1 = T(0), fb2 = T(0);
-  for (int j = 0; j < 16; ++j) {
-    const T z = Z1[l * 16 + j];
-    fb0 = fb0 + z * W[hg::kUdeW1 + j * 3 + 0];
-    fb1 = fb1 + z * W[hg::kUdeW1 + j * 3 + 1];
-    fb2 = fb2 + z * W[hg::kUdeW1 + j * 3 + 2];
-  }
-  if (valid) {
-    double xib = 0.0, qxb = 0.0, qyb = 0.0;
-    if (wet) {  // h = xi + h_still, |u| = sqrt(u^2 + v^2 + eps), u = q_x / h, v = q_y / h
-      const double ub = double(fb1) * u / um, vb = double(fb1) * v / um;
-      xib = double(fb0) - (ub * u + vb * v) / h;
-      qxb = ub / h;
-      qyb = vb / h;
+    hg::wave_sync();
+    // feature cotangents -> state cotangent (lane = cell)
+    T fb0 = T(0), fb1 = T(0), fb2 = T(0);
+    for (int j = 0; j < 16; ++j) {
+      const T z = Z1[l * 16 + j];
+      fb0 = fb0 + z * W[hg::kUdeW1 + j * 3 + 0];
+      fb1 = fb1 + z * W[hg::kUdeW1 + j * 3 + 1];
+      fb2 = fb2 + z * W[hg::kUdeW1 + j * 3 + 2];
     }
-    a.ox[i] += xib;
-    a.oqx[i] += qxb;
-    a.oqy[i] += qyb;
-    if (a.zbbar) a.zbbar[i] += double(fb2);
-  }
-  // weight cotangents of these 64 cells: W2bar = Z2^T A1 on the matrix cores
-  {
-    typename M::V acc = {T(0), T(0), T(0), T(0)};
-    for (int kc = 0; kc < 16; ++kc) {
-      const T av = Z2[(4 * kc + kq) * 16 + col];  // Z2[cell = 4 kc + k][unit i]
-      const T bv = A1[(4 * kc + kq) * 16 + col];  // A1[cell = 4 kc + k][unit j]
-      acc = M::op(av, bv, acc);
+    if (mine) {
+      double xib = 0.0, qxb = 0.0, qyb = 0.0;
+      if (wet) {  // h = xi + h_still, |u| = sqrt(u^2 + v^2 + eps), u = q_x / h, v = q_y / h
+        const double ub = double(fb1) * u / um, vb = double(fb1) * v / um;
+        xib = double(fb0) - (ub * u + vb * v) / h;
+        qxb = ub / h;
+        qyb = vb / h;
+      }
+      a.ox[i] += xib;
+      a.oqx[i] += qxb;
+      a.oqy[i] += qyb;
+      if (a.zbbar) a.zbbar[i] += double(fb2);
     }
-    for (int r = 0; r < 4; ++r) Wb[hg::kUdeW2 + M::row(kq, r) * 16 + col] = double(acc[r]);
-  }
-  for (int e = l; e < 97; e += 64) {  // W1 (48), b1, b2, W3 (16 each), b3: sums over the cells in order
-    double sacc = 0.0;
-    int dst;
-    if (e < 48) {
-      const int j = e / 3, c = e % 3;
-      for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + j]) * double(F[q * 4 + c]);
-      dst = hg::kUdeW1 + e;
-    } else if (e < 64) {
-      for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + (e - 48)]);
-      dst = hg::kUdeB1 + (e - 48);
-    } else if (e < 80) {
-      for (int q = 0; q < 64; ++q) sacc += double(Z2[q * 16 + (e - 64)]);
-      dst = hg::kUdeB2 + (e - 64);
-    } else if (e < 96) {
-      for (int q = 0; q < 64; ++q) sacc += double(Z3[q]) * double(A2[q * 16 + (e - 80)]);
-      dst = hg::kUdeW3 + (e - 80);
-    } else {
-      for (int q = 0; q < 64; ++q) sacc += double(Z3[q]);
-      dst = hg::kUdeB3;
+    // weight cotangents of these cells: W2bar = Z2^T A1 on the matrix cores
+    {
+      typename M::V acc = {T(0), T(0), T(0), T(0)};
+      for (int kc = 0; kc < 16; ++kc) {
+        const T av = Z2[(4 * kc + kq) * 16 + col];  // Z2[cell = 4 kc + k][unit i]
+        const T bv = A1[(4 * kc + kq) * 16 + col];  // A1[cell = 4 kc + k][unit j]
+        acc = M::op(av, bv, acc);
+      }
+      for (int r = 0; r < 4; ++r) Wb[hg::kUdeW2 + M::row(kq, r) * 16 + col] = double(acc[r]);
     }
-    Wb[dst] = sacc;
+    for (int e = l; e < 97; e += 64) {  // W1 (48), b1, b2, W3 (16 each), b3: sums over the cells in order
+      double sacc = 0.0;
+      int dst;
+      if (e < 48) {
+        const int j = e / 3, c = e % 3;
+        for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + j]) * double(F[q * 4 + c]);
+        dst = hg::kUdeW1 + e;
+      } else if (e < 64) {
+        for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + (e - 48)]);
+        dst = hg::kUdeB1 + (e - 48);
+      } else if (e < 80) {
+        for (int q = 0; q < 64; ++q) sacc += double(Z2[q * 16 + (e - 64)]);
+        dst = hg::kUdeB2 + (e - 64);
+      } else if (e < 96) {
+        for (int q = 0; q < 64; ++q) sacc += double(Z3[q]) * double(A2[q * 16 + (e - 80)]);
+        dst = hg::kUdeW3 + (e - 80);
+      } else {
+        for (int q = 0; q < 64; ++q) sacc += double(Z3[q]);
+        dst = hg::kUdeB3;
+      }
+      Wb[dst] = sacc;
+    }
+    __syncthreads();
+    const int slot = a.blk_slot ? a.blk_slot[a.blk_off[blockIdx.x] + pass] : int(blockIdx.x);
+    for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * a.nslots + slot] += Wb[k];
+    __syncthreads();  // the next pass rewrites the shared arrays
   }
-  __syncthreads();
-  for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * gridDim.x + blockIdx.x] += Wb[k];
 }
 
-// wbar[k] += sum over blocks b of wpart[k][b]; one work-group per weight,
-// strided partial sums then a fixed tree (deterministic).
-__global__ __launch_bounds__(kBlock) void k_ude_wsum(int nblocks, const double* wpart, double* wbar) {
-  __shared__ double s[kBlock];
-  const int k = blockIdx.x, tid = threadIdx.x;
-  const double* p = wpart + size_t(k) * nblocks;
+// wbar[s][k] += the sum of wpart[k][slot] over the slots of weight set s
+// (set_start[s] .. set_start[s + 1], or all nslots when set_start is NULL);
+// one work-group per (weight k = blockIdx.x, set s = blockIdx.y), strided
+// partial sums then a fixed tree (deterministic).
+__global__ __launch_bounds__(kBlock) void k_ude_wsum(int nslots, const int32_t* set_start, const double* wpart,
+                                                      double* wbar) {
+  __shared__ double part[kBlock];
+  const int k = blockIdx.x, set = blockIdx.y, tid = threadIdx.x;
+  const int lo = set_start ? set_start[set] : 0, hi = set_start ? set_start[set + 1] : nslots;
+  const double* p = wpart + size_t(k) * nslots;
   double acc = 0.0;
-  for (int b = tid; b < nblocks; b += kBlock) acc += p[b];
-  s[tid] = acc;
+  for (int b = lo + tid; b < hi; b += kBlock) acc += p[b];
+  part[tid] = acc;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (tid < w) s[tid] += s[tid + w];
+    if (tid < w) part[tid] += part[tid + w];
     __syncthreads();
   }
-  if (tid == 0) wbar[k] += s[0];
+  if (tid == 0) wbar[size_t(set) * gridDim.x + k] += part[0];
 }
 
 // ---- variable Manning n: one law evaluation per cell (hg_nlaw.hpp) ----
@@ -2012,10 +2028,6 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   const bool lux_n = lux && ctx->lux.choice != hg::LUX_FLOW_RESISTANCE;  // the network gives n
   // with a state-dependent n, n-bar of this step only
   double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : lux_n ? ctx->lux_obar : ctx->adj_nbar;
-  if (ude && ctx->ude_nsets != 1) {
-    ctx->err = "the UDE closure's reverse mode takes one weight set";
-    return HG_ERR_STATE;
-  }
   // the closure's, law's or network's n-bar (or friction-magnitude bar) of
   // this step is set by the adjoint kernel, not cleared and added to
   if (ude) {
@@ -2057,6 +2069,9 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     u.oqy = out[2];
     u.zbbar = ctx->adj_zbbar;
     u.wpart = ctx->ude_wpart;
+    u.blk_off = ctx->ude_blk_off;
+    u.blk_slot = ctx->ude_blk_slot;
+    u.nslots = ctx->ude_nslots;
     if (ctx->ude_prec == 32)
       hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
     else
@@ -2096,10 +2111,10 @@ int finish_adjoint(hg_ctx* ctx) {
   hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
                      ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
   if (ctx->ude_prec)
-    hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW), dim3(kBlock), 0, ctx->stream, ude_blocks(ctx), ctx->ude_wpart,
-                       ctx->ude_wbar);
+    hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW, ctx->ude_nsets), dim3(kBlock), 0, ctx->stream, ctx->ude_nslots,
+                       ctx->ude_set_start, ctx->ude_wpart, ctx->ude_wbar);
   if (ctx->lux.choice)
-    hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, lux_vjp_blocks(ctx),
+    hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, lux_vjp_blocks(ctx), nullptr,
                        ctx->lux_wpart, ctx->lux_wbar);
   HG_TRY(hipGetLastError());
   return HG_OK;
@@ -2107,8 +2122,8 @@ int finish_adjoint(hg_ctx* ctx) {
 
 int zero_adj_accumulators(hg_ctx* ctx) {
   if (ctx->ude_prec) {
-    HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
-    HG_TRY(hipMemsetAsync(ctx->ude_wpart, 0, size_t(hg::kUdeW) * ude_blocks(ctx) * sizeof(double), ctx->stream));
+    HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, size_t(ctx->ude_nsets) * hg::kUdeW * sizeof(double), ctx->stream));
+    HG_TRY(hipMemsetAsync(ctx->ude_wpart, 0, size_t(hg::kUdeW) * ctx->ude_nslots * sizeof(double), ctx->stream));
   }
   if (ctx->lux.choice) {
     HG_TRY(hipMemsetAsync(ctx->lux_wbar, 0, ctx->lux.P * sizeof(double), ctx->stream));
@@ -3137,6 +3152,9 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
   ctx->ude_nsets = 0;
   ctx->ude_w = nullptr;
   ctx->ude_set = nullptr;
+  ctx->ude_nbar = ctx->ude_wpart = ctx->ude_wbar = nullptr;
+  ctx->ude_blk_off = ctx->ude_blk_slot = ctx->ude_set_start = nullptr;
+  ctx->ude_nslots = 0;
   if (n_sets == 0) {  // closure off: back to the per-cell n of hg_set_fields
     HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HG_TRY(hipStreamSynchronize(ctx->stream));
@@ -3197,12 +3215,36 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
     if (!rc) rc = upload(ctx, ctx->ude_set, set.data(), N);
   }
   if (!rc) rc = dalloc(ctx, &ctx->ude_nbar, N);
-  if (!rc) rc = dalloc(ctx, &ctx->ude_wpart, size_t(ude_blocks(ctx)) * hg::kUdeW);
-  if (!rc) rc = dalloc(ctx, &ctx->ude_wbar, hg::kUdeW);
+  // Reverse-mode weight slots: one per (64-cell block, set used in it), ordered
+  // by set and then block, so that each set's cotangent sums a contiguous run.
+  const int nb = ude_blocks(ctx);
+  std::vector<int32_t> blk_off(size_t(nb) + 1, 0), blk_set, set_start(size_t(n_sets) + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int64_t i1 = std::min<int64_t>(N, int64_t(b + 1) * 64);
+    for (int64_t i = int64_t(b) * 64; i < i1; ++i) {
+      const int32_t v = set_of_cell ? set[i] : 0;
+      if (std::find(blk_set.begin() + blk_off[b], blk_set.end(), v) == blk_set.end()) blk_set.push_back(v);
+    }
+    blk_off[b + 1] = int32_t(blk_set.size());
+  }
+  for (int32_t v : blk_set) ++set_start[size_t(v) + 1];
+  for (int32_t v = 0; v < n_sets; ++v) set_start[size_t(v) + 1] += set_start[v];
+  std::vector<int32_t> blk_slot(blk_set.size()), next(set_start.begin(), set_start.end() - 1);
+  for (size_t e = 0; e < blk_set.size(); ++e) blk_slot[e] = next[blk_set[e]]++;  // blocks in order within a set
+  const int32_t nslots = int32_t(blk_set.size());
+  if (!rc) rc = dalloc(ctx, &ctx->ude_wpart, size_t(nslots) * hg::kUdeW);
+  if (!rc) rc = dalloc(ctx, &ctx->ude_wbar, nw);
+  if (!rc) rc = dalloc(ctx, &ctx->ude_blk_off, blk_off.size());
+  if (!rc) rc = upload(ctx, ctx->ude_blk_off, blk_off.data(), blk_off.size());
+  if (!rc) rc = dalloc(ctx, &ctx->ude_blk_slot, blk_slot.size());
+  if (!rc) rc = upload(ctx, ctx->ude_blk_slot, blk_slot.data(), blk_slot.size());
+  if (!rc) rc = dalloc(ctx, &ctx->ude_set_start, set_start.size());
+  if (!rc) rc = upload(ctx, ctx->ude_set_start, set_start.data(), set_start.size());
   ctx->ude_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
   if (rc) return rc;
-  HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, hg::kUdeW * sizeof(double), ctx->stream));
+  HG_TRY(hipMemsetAsync(ctx->ude_wbar, 0, nw * sizeof(double), ctx->stream));
   HG_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->ude_nslots = nslots;
   ctx->ude_nsets = n_sets;
   ctx->ude_prec = precision;
   return HG_OK;
@@ -3230,7 +3272,8 @@ int hg_ude_grad(hg_ctx* ctx, double* w_bar) {
     ctx->err = "hg_ude_grad: the UDE closure is off";
     return HG_ERR_STATE;
   }
-  HG_TRY(hipMemcpyAsync(w_bar, ctx->ude_wbar, hg::kUdeW * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipMemcpyAsync(w_bar, ctx->ude_wbar, size_t(ctx->ude_nsets) * hg::kUdeW * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
   HG_TRY(hipStreamSynchronize(ctx->stream));
   return HG_OK;
 }

@@ -359,10 +359,12 @@ class Context:
         weights=None turns it off."""
         if weights is None:
             self._chk(lib().hg_set_ude(self.h, 0, C.c_void_p(0), C.c_void_p(0), 32))
+            self._ude_sets = 0
             return
         w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1, UDE_WEIGHTS)
         s = None if set_of_cell is None else np.ascontiguousarray(set_of_cell, dtype=np.int32)
         self._chk(lib().hg_set_ude(self.h, w.shape[0], _ptr(w), _ptr(s), int(precision)))
+        self._ude_sets = w.shape[0]
 
     def ude_n(self, Q):
         n = np.empty(self.N)
@@ -370,9 +372,14 @@ class Context:
         return n
 
     def ude_grad(self):
-        g = np.empty(UDE_WEIGHTS)
+        """Weight cotangents of the last reverse pass: (353,) for one weight
+        set, (n_sets, 353) for several."""
+        sets = getattr(self, "_ude_sets", 0)
+        if sets < 1:
+            raise RuntimeError("hg_ude_grad: the UDE closure is off")
+        g = np.empty((sets, UDE_WEIGHTS))
         self._chk(lib().hg_ude_grad(self.h, _ptr(g)))
-        return g
+        return g[0] if sets == 1 else g
 
     def set_n_law(self, kind, n_lower=0.0, n_upper=0.0, k=1.0, h_mid=1.0, ks=None):
         """Variable Manning n (the forward simulation's "variable" option):

@@ -62,6 +62,7 @@ mutable struct Context
     N::Int
     active::Int32
     lux_hidden::Int  # hidden layers of the UDE network when its weights are params_vector
+    ude_sets::Int    # weight sets of the MLP closure (0: off)
 end
 
 # hg_lux_net: the reference's UDE network (create_NN_model, process_UDE.jl:2-65)
@@ -139,7 +140,7 @@ function context(p)
                        out, mesh, bc, k, Int32(1))
             check(C_NULL, rc)
         end
-        ctx = Context(out[], d.N, HG_PARAM_NONE, 0)
+        ctx = Context(out[], d.N, HG_PARAM_NONE, 0, 0)
         finalizer(x -> ccall((:hg_destroy, libhgswe), Cint, (Ptr{Cvoid},), x.ptr), ctx)
         S0 = p.S0_cells
         rc = ccall((:hg_set_fields, libhgswe), Cint,
@@ -338,25 +339,31 @@ end
 # UDE closure for Manning's n on the device (3 -> 16 -> 16 -> 1 tanh network,
 # 353 weights; the layout is documented in include/hgswe.h).  weights = nothing
 # turns it off.
-function set_ude!(p_extra, weights; precision = 32)
+function set_ude!(p_extra, weights; precision = 32, set_of_cell = nothing)
     ctx = context(p_extra)
     if weights === nothing
         check(ctx.ptr, ccall((:hg_set_ude, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Int32}, Int32),
                              ctx.ptr, 0, C_NULL, C_NULL, 32))
+        ctx.ude_sets = 0
     else
-        w = Vector{Float64}(weights)
+        # one set per column of a 353 x n_sets matrix; set_of_cell (0-based, per cell) picks each cell's set
+        w = Vector{Float64}(vec(weights))
+        s = set_of_cell === nothing ? C_NULL : Vector{Int32}(set_of_cell)
         check(ctx.ptr, ccall((:hg_set_ude, libhgswe), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Int32}, Int32),
-                             ctx.ptr, Int32(length(w) ÷ 353), w, C_NULL, Int32(precision)))
+                             ctx.ptr, Int32(length(w) ÷ 353), w, s, Int32(precision)))
+        ctx.ude_sets = length(w) ÷ 353
     end
     return nothing
 end
 
-# Cotangent of the closure weights from the last reverse pass (hg_ude_grad).
+# Cotangents of the closure weights from the last reverse pass (hg_ude_grad):
+# a 353-vector for one weight set, a 353 x n_sets matrix for several.
 function ude_grad(p_extra)
     ctx = context(p_extra)
-    g = zeros(353)
+    ctx.ude_sets > 0 || error("libhgswe: the UDE closure is off")
+    g = zeros(353, ctx.ude_sets)
     check(ctx.ptr, ccall((:hg_ude_grad, libhgswe), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.ptr, g))
-    return g
+    return ctx.ude_sets == 1 ? vec(g) : g
 end
 
 # ---------------------------------------------------------------- reverse mode

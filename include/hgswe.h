@@ -282,8 +282,11 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
 /* The closure's n (N, caller order) for state Q (3N). */
 int hg_ude_n(hg_ctx* ctx, const double* Q, double* n);
 
-/* Cotangent of the weights (353) accumulated by the last hg_rhs_vjp,
- * hg_adjoint or hg_grad_probes call with the closure on (one weight set). */
+/* Cotangents of the weights (n_sets x 353, set-major) accumulated by the
+ * last hg_rhs_vjp, hg_adjoint or hg_grad_probes call with the closure on.
+ * Each set's cotangent collects the cells that use it, so one reverse sweep
+ * over a batch with one weight set per instance gives every set's gradient
+ * of the summed loss. */
 int hg_ude_grad(hg_ctx* ctx, double* w_bar);
 
 /* ---- Variable Manning n: the forward simulation's "variable" option ---- */

@@ -9,7 +9,10 @@
 * Reverse mode through the closure (f64): weight and initial-state
   gradients of the probe loss against central differences; f32 gradients
   against f64 ones.
-* Batched instances with one weight set each equal separate runs.
+* Batched instances with one weight set each equal separate runs, forward
+  and in the weight gradient of each set; weight sets interleaved within
+  blocks: equal sets sum to the one-set gradient, distinct sets against
+  central differences.
 """
 import sys
 
@@ -184,6 +187,96 @@ def test_batched_weight_sets_equal_separate_runs():
         k = capi.Context(c)
         k.set_ude(Ws[r], precision=32)
         assert np.array_equal(q, k.step(c.Q0, B.dt, 30)), r
+
+
+def test_batched_weight_set_gradients_equal_separate_runs():
+    """One reverse sweep over a batch with one weight set per instance gives
+    each set the gradient of its own instance's probe loss.  The per-block
+    partial sums group the cells differently from a run alone, so the match
+    is to round-off, not bitwise."""
+    inst = [small_case(seed=s, n=n) for s, n in ((1, 2500), (2, 1900), (3, 3100))]
+    B = cases.concat(inst)
+    Ws = ude.perturbed_sets(ude.xavier_weights(8), 3, scale=0.1)
+    set_of_cell = np.repeat(np.arange(3), [c.N for c in inst]).astype(np.int32)
+    nsteps, kck = 20, 6
+    probes, targets = [], []
+    for r, c in enumerate(inst):
+        p, t = _probe(c, 0, seed=20 + r, k=15)
+        probes.append(p)
+        targets.append(t)
+    ctx = capi.Context(B)
+    ctx.set_ude(Ws, set_of_cell=set_of_cell, precision=64)
+    first = np.cumsum([0] + [c.N for c in inst[:-1]])
+    bprobe = np.concatenate([p + o for p, o in zip(probes, first)]).astype(np.int32)
+    ctx.grad_probes(B.Q0, B.dt, nsteps, kck, bprobe, np.concatenate(targets))
+    gb = ctx.ude_grad()
+    assert gb.shape == (3, capi.UDE_WEIGHTS)
+    for r, c in enumerate(inst):
+        k = capi.Context(c)
+        k.set_ude(Ws[r], precision=64)
+        k.grad_probes(c.Q0, B.dt, nsteps, kck, probes[r], targets[r])
+        g1 = k.ude_grad()
+        rel = float(np.max(np.abs(gb[r] - g1)) / np.max(np.abs(g1)))
+        say(f"  set {r}: |g| {np.linalg.norm(g1):.4e}, batch vs alone max rel {rel:.2e}")
+        assert np.linalg.norm(g1) > 0.0
+        assert rel <= 1e-12, r
+
+
+def _interleaved(c, nsets):
+    """Each cell's set is its caller index mod nsets, so every 64-cell block
+    holds several sets whatever the cell order."""
+    return (np.arange(c.N) % nsets).astype(np.int32)
+
+
+def test_interleaved_weight_sets_sum_to_single_set_gradient():
+    c = small_case(n=3000)
+    nsteps, kck = 20, 5
+    probe, target = _probe(c, nsteps)
+    W = ude.xavier_weights(11)
+    one = capi.Context(c)
+    one.set_ude(W, precision=64)
+    one.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g1 = one.ude_grad()
+    many = capi.Context(c)
+    many.set_ude(np.stack([W] * 4), set_of_cell=_interleaved(c, 4), precision=64)
+    qa = many.step(c.Q0, c.dt, nsteps)
+    assert np.array_equal(qa, one.step(c.Q0, c.dt, nsteps))  # equal weights: the same trajectory
+    many.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g4 = many.ude_grad()
+    assert g4.shape[0] == 4 and all(np.linalg.norm(g) > 0.0 for g in g4)
+    rel = float(np.max(np.abs(g4.sum(axis=0) - g1)) / np.max(np.abs(g1)))
+    say(f"  4 interleaved equal sets: sum vs one set max rel {rel:.2e}")
+    assert rel <= 1e-12
+
+
+def test_interleaved_weight_set_gradient_fd():
+    """Distinct interleaved sets: the gradient of one set's weights against
+    central differences that move only that set."""
+    c = small_case(n=3000)
+    nsteps, kck = 20, 5
+    probe, target = _probe(c, nsteps)
+    Ws = ude.perturbed_sets(ude.xavier_weights(12), 3, scale=0.2)
+    soc = _interleaved(c, 3)
+    ctx = capi.Context(c)
+    ctx.set_ude(Ws, set_of_cell=soc, precision=64)
+    ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g = ctx.ude_grad()
+
+    def L(ws):
+        k = capi.Context(c)
+        k.set_ude(ws, set_of_cell=soc, precision=64)
+        q = k.step(c.Q0, c.dt, nsteps)
+        r = q[probe] + c.hstill[probe] + c.zb[probe] - target
+        return float(np.sum(r * r))
+
+    rng = np.random.default_rng(3)
+    for s in (0, 2):
+        v = np.zeros_like(Ws)
+        v[s] = rng.normal(size=Ws.shape[1])
+        fd = fd4(L, Ws, v, 5e-3)
+        gv = float(np.dot(g[s], v[s]))
+        say(f"  interleaved set {s}: fd {fd:.10e} adjoint {gv:.10e} rel {abs(fd - gv) / abs(gv):.2e}")
+        assert abs(fd - gv) <= 1e-5 * abs(gv)
 
 
 def test_closure_rejects_n_as_active_parameter():
