@@ -275,7 +275,7 @@ struct hg_ctx {
   void* ude_w = nullptr;            // [ude_nsets][353] weights in that precision
   int32_t* ude_set = nullptr;       // weight set of each cell (lib order), or NULL
   double* ude_nbar = nullptr;       // per-step cotangent of n (N)
-  double* ude_wpart = nullptr;      // [353][ude_nslots] weight cotangents per (block, set) slot
+  double* ude_wpart = nullptr;      // [ude_nslots][353] weight cotangents per (block, set) slot
   double* ude_wbar = nullptr;       // accumulated weight cotangents [ude_nsets][353]
   int32_t ude_nslots = 0;           // slots, ordered by set then block
   int32_t* ude_blk_off = nullptr;   // [blocks + 1] first entry of each 64-cell block in ude_blk_slot

@@ -91,7 +91,7 @@ struct UdeArgs {
   double* oqx;
   double* oqy;
   double* zbbar;               // optional, += d/d zb
-  double* wpart;               // [kUdeW][nslots] weight cotangents, one slot per (block, set) pair
+  double* wpart;               // [nslots][kUdeW] weight cotangents, one slot per (block, set) pair
   const int32_t* blk_off;      // slots of block b: blk_slot[blk_off[b] ...], its sets in order of
   const int32_t* blk_slot;     //   first appearance; both NULL: slot = block (one set)
   int32_t nslots;

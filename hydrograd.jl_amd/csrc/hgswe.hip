@@ -1117,57 +1117,55 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
       a.oqy[i] += qyb;
       if (a.zbbar) a.zbbar[i] += double(fb2);
     }
-    // weight cotangents of these cells: W2bar = Z2^T A1 on the matrix cores
+    // weight cotangents of these cells, as sums over the 64 cells on the matrix
+    // cores (K = cells): W2bar = Z2^T A1; [W1bar | b1bar] = Z1^T [F | 1];
+    // b2bar = Z2^T 1 and W3bar = A2^T Z3 (one product over 128 K-rows);
+    // b3bar = sum of Z3 by a fixed xor tree.
     {
-      typename M::V acc = {T(0), T(0), T(0), T(0)};
+      typename M::V w2 = {T(0), T(0), T(0), T(0)}, w1 = w2, w3 = w2;
       for (int kc = 0; kc < 16; ++kc) {
-        const T av = Z2[(4 * kc + kq) * 16 + col];  // Z2[cell = 4 kc + k][unit i]
-        const T bv = A1[(4 * kc + kq) * 16 + col];  // A1[cell = 4 kc + k][unit j]
-        acc = M::op(av, bv, acc);
+        const int c = 4 * kc + kq;  // cell of this lane's K-row
+        const T z2 = Z2[c * 16 + col];
+        w2 = M::op(z2, A1[c * 16 + col], w2);                                 // Z2[c][i] A1[c][j]
+        w1 = M::op(Z1[c * 16 + col], col < 3 ? F[c * 4 + col] : T(col == 3), w1);  // Z1[c][i] [F | 1][c][j]
+        w3 = M::op(z2, T(col == 0), w3);                                      // Z2[c][i] 1
       }
-      for (int r = 0; r < 4; ++r) Wb[hg::kUdeW2 + M::row(kq, r) * 16 + col] = double(acc[r]);
-    }
-    for (int e = l; e < 97; e += 64) {  // W1 (48), b1, b2, W3 (16 each), b3: sums over the cells in order
-      double sacc = 0.0;
-      int dst;
-      if (e < 48) {
-        const int j = e / 3, c = e % 3;
-        for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + j]) * double(F[q * 4 + c]);
-        dst = hg::kUdeW1 + e;
-      } else if (e < 64) {
-        for (int q = 0; q < 64; ++q) sacc += double(Z1[q * 16 + (e - 48)]);
-        dst = hg::kUdeB1 + (e - 48);
-      } else if (e < 80) {
-        for (int q = 0; q < 64; ++q) sacc += double(Z2[q * 16 + (e - 64)]);
-        dst = hg::kUdeB2 + (e - 64);
-      } else if (e < 96) {
-        for (int q = 0; q < 64; ++q) sacc += double(Z3[q]) * double(A2[q * 16 + (e - 80)]);
-        dst = hg::kUdeW3 + (e - 80);
-      } else {
-        for (int q = 0; q < 64; ++q) sacc += double(Z3[q]);
-        dst = hg::kUdeB3;
+      for (int kc = 0; kc < 16; ++kc) {
+        const int c = 4 * kc + kq;
+        w3 = M::op(A2[c * 16 + col], col == 1 ? Z3[c] : T(0), w3);            // A2[c][i] Z3[c]
       }
-      Wb[dst] = sacc;
+      for (int r = 0; r < 4; ++r) {
+        const int i = M::row(kq, r);
+        Wb[hg::kUdeW2 + i * 16 + col] = double(w2[r]);
+        if (col < 3) Wb[hg::kUdeW1 + i * 3 + col] = double(w1[r]);
+        if (col == 3) Wb[hg::kUdeB1 + i] = double(w1[r]);
+        if (col == 0) Wb[hg::kUdeB2 + i] = double(w3[r]);
+        if (col == 1) Wb[hg::kUdeW3 + i] = double(w3[r]);
+      }
+      double b3 = double(Z3[l]);
+      for (int m = 32; m > 0; m >>= 1) b3 += __shfl_xor(b3, m, 64);
+      if (l == 0) Wb[hg::kUdeB3] = b3;
     }
     __syncthreads();
     const int slot = a.blk_slot ? a.blk_slot[a.blk_off[blockIdx.x] + pass] : int(blockIdx.x);
-    for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(k) * a.nslots + slot] += Wb[k];
+    for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(slot) * hg::kUdeW + k] += Wb[k];
     __syncthreads();  // the next pass rewrites the shared arrays
   }
 }
 
-// wbar[s][k] += the sum of wpart[k][slot] over the slots of weight set s
-// (set_start[s] .. set_start[s + 1], or all nslots when set_start is NULL);
-// one work-group per (weight k = blockIdx.x, set s = blockIdx.y), strided
-// partial sums then a fixed tree (deterministic).
+// wbar[s][k] += the sum of the slots of weight set s (set_start[s] ..
+// set_start[s + 1], or all nslots when set_start is NULL), where slot b of
+// weight k sits at wpart[k * kstride + b * sstride]; one work-group per
+// (weight k = blockIdx.x, set s = blockIdx.y), strided partial sums then a
+// fixed tree (deterministic).
 __global__ __launch_bounds__(kBlock) void k_ude_wsum(int nslots, const int32_t* set_start, const double* wpart,
-                                                      double* wbar) {
+                                                      int64_t kstride, int64_t sstride, double* wbar) {
   __shared__ double part[kBlock];
   const int k = blockIdx.x, set = blockIdx.y, tid = threadIdx.x;
   const int lo = set_start ? set_start[set] : 0, hi = set_start ? set_start[set + 1] : nslots;
-  const double* p = wpart + size_t(k) * nslots;
+  const double* p = wpart + k * kstride;
   double acc = 0.0;
-  for (int b = lo + tid; b < hi; b += kBlock) acc += p[b];
+  for (int b = lo + tid; b < hi; b += kBlock) acc += p[b * sstride];
   part[tid] = acc;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -2112,10 +2110,10 @@ int finish_adjoint(hg_ctx* ctx) {
                      ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
   if (ctx->ude_prec)
     hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW, ctx->ude_nsets), dim3(kBlock), 0, ctx->stream, ctx->ude_nslots,
-                       ctx->ude_set_start, ctx->ude_wpart, ctx->ude_wbar);
+                       ctx->ude_set_start, ctx->ude_wpart, int64_t(1), int64_t(hg::kUdeW), ctx->ude_wbar);
   if (ctx->lux.choice)
     hipLaunchKernelGGL(k_ude_wsum, dim3(ctx->lux.P), dim3(kBlock), 0, ctx->stream, lux_vjp_blocks(ctx), nullptr,
-                       ctx->lux_wpart, ctx->lux_wbar);
+                       ctx->lux_wpart, int64_t(lux_vjp_blocks(ctx)), int64_t(1), ctx->lux_wbar);
   HG_TRY(hipGetLastError());
   return HG_OK;
 }
