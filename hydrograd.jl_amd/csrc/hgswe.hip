@@ -1014,7 +1014,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
     uint64_t pending = __ballot(valid);
     while (pending) {
       const int lead = __ffsll((unsigned long long)pending) - 1;
-      const int s = __shfl(set, lead, 64);
+      const int s = __builtin_amdgcn_readfirstlane(__shfl(set, lead, 64));
       const bool mine = valid && set == s;
       pending &= ~uint64_t(__ballot(mine));
       if (s != cur) {
@@ -1036,17 +1036,15 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
 // the features to the state cotangent (through the clamp and u = q / h, as
 // the forward forms them) and adds this block's weight cotangents into its
 // slot for each weight set its 64 cells use (one pass per distinct set, in
-// order of first appearance, as k_ude_n).
+// order of first appearance, as k_ude_n).  The backward activations overwrite
+// the forward ones once read: Z2 replaces A2, then Z1 replaces A1.
 template <class T>
-__global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
+__global__ __launch_bounds__(64, sizeof(T) == 4 ? 3 : 1) void k_ude_vjp(hg::UdeArgs a) {
   using M = hg::Mfma<T>;
   __shared__ T F[64 * 4];
-  __shared__ T A1[64 * 16];
-  __shared__ T A2[64 * 16];
-  __shared__ T Z2[64 * 16];
-  __shared__ T Z1[64 * 16];
+  __shared__ T A1[64 * 16];  // A1, then Z1 = A1bar (1 - A1^2)
+  __shared__ T A2[64 * 16];  // A2, then Z2 = z3bar W3 (1 - A2^2)
   __shared__ T Z3[64];
-  __shared__ double Wb[hg::kUdeW];
   const int l = threadIdx.x;
   const int col = l & 15, kq = l >> 4;
   const int i = blockIdx.x * 64 + l;
@@ -1067,7 +1065,7 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
   uint64_t pending = __ballot(valid);
   for (int pass = 0; pending; ++pass) {
     const int lead = __ffsll((unsigned long long)pending) - 1;
-    const int s = __shfl(set, lead, 64);
+    const int s = __builtin_amdgcn_readfirstlane(__shfl(set, lead, 64));  // wave-uniform: scalar weight loads
     const bool mine = valid && set == s;
     pending &= ~uint64_t(__ballot(mine));
     const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
@@ -1076,12 +1074,30 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
     const T z3 = ude_out<T>(W, A2, l);
     const T z3b = mine ? T(a.nbar[i]) * hg::sigmoid<T>(z3) : T(0);
     Z3[l] = z3b;
-    for (int j = 0; j < 16; ++j) {
-      const T a2 = A2[l * 16 + j];
-      Z2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
+    hg::wave_sync();
+    // Weight cotangents as sums over the 64 cells on the matrix cores (K = cells):
+    // w3 column 1: W3bar = A2^T Z3; column 0: b2bar = Z2^T 1; w2: W2bar = Z2^T A1;
+    // w1: [W1bar | b1bar] = Z1^T [F | 1]; b3bar = sum of Z3 by a fixed xor tree.
+    typename M::V w1 = {T(0), T(0), T(0), T(0)}, w2 = w1, w3 = w1;
+    for (int kc = 0; kc < 16; ++kc) {
+      const int c = 4 * kc + kq;  // cell of this lane's K-row
+      w3 = M::op(A2[c * 16 + col], col == 1 ? Z3[c] : T(0), w3);
     }
     hg::wave_sync();
-    // A1bar = Z2 W2 (rows = cells), Z1 = A1bar (1 - A1^2)
+    for (int j = 0; j < 16; ++j) {  // Z2 over A2, own row
+      const T a2 = A2[l * 16 + j];
+      A2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
+    }
+    hg::wave_sync();
+    const T* Z2 = A2;
+    for (int kc = 0; kc < 16; ++kc) {
+      const int c = 4 * kc + kq;
+      const T z2 = Z2[c * 16 + col];
+      w2 = M::op(z2, A1[c * 16 + col], w2);
+      w3 = M::op(z2, T(col == 0), w3);
+    }
+    // A1bar = Z2 W2 (rows = cells)
+    typename M::V ab[4];
     for (int rb = 0; rb < 4; ++rb) {
       typename M::V acc = {T(0), T(0), T(0), T(0)};
       for (int kc = 0; kc < 4; ++kc) {
@@ -1089,13 +1105,17 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
         const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];         // W2[4 kc + k][j]
         acc = M::op(av, bv, acc);
       }
+      ab[rb] = acc;
+    }
+    hg::wave_sync();
+    for (int rb = 0; rb < 4; ++rb)  // Z1 = A1bar (1 - A1^2) over A1
       for (int r = 0; r < 4; ++r) {
         const int row = 16 * rb + M::row(kq, r);
         const T a1 = A1[row * 16 + col];
-        Z1[row * 16 + col] = acc[r] * (T(1) - a1 * a1);
+        A1[row * 16 + col] = ab[rb][r] * (T(1) - a1 * a1);
       }
-    }
     hg::wave_sync();
+    const T* Z1 = A1;
     // feature cotangents -> state cotangent (lane = cell)
     T fb0 = T(0), fb1 = T(0), fb2 = T(0);
     for (int j = 0; j < 16; ++j) {
@@ -1117,39 +1137,25 @@ __global__ __launch_bounds__(64) void k_ude_vjp(hg::UdeArgs a) {
       a.oqy[i] += qyb;
       if (a.zbbar) a.zbbar[i] += double(fb2);
     }
-    // weight cotangents of these cells, as sums over the 64 cells on the matrix
-    // cores (K = cells): W2bar = Z2^T A1; [W1bar | b1bar] = Z1^T [F | 1];
-    // b2bar = Z2^T 1 and W3bar = A2^T Z3 (one product over 128 K-rows);
-    // b3bar = sum of Z3 by a fixed xor tree.
-    {
-      typename M::V w2 = {T(0), T(0), T(0), T(0)}, w1 = w2, w3 = w2;
-      for (int kc = 0; kc < 16; ++kc) {
-        const int c = 4 * kc + kq;  // cell of this lane's K-row
-        const T z2 = Z2[c * 16 + col];
-        w2 = M::op(z2, A1[c * 16 + col], w2);                                 // Z2[c][i] A1[c][j]
-        w1 = M::op(Z1[c * 16 + col], col < 3 ? F[c * 4 + col] : T(col == 3), w1);  // Z1[c][i] [F | 1][c][j]
-        w3 = M::op(z2, T(col == 0), w3);                                      // Z2[c][i] 1
-      }
-      for (int kc = 0; kc < 16; ++kc) {
-        const int c = 4 * kc + kq;
-        w3 = M::op(A2[c * 16 + col], col == 1 ? Z3[c] : T(0), w3);            // A2[c][i] Z3[c]
-      }
-      for (int r = 0; r < 4; ++r) {
-        const int i = M::row(kq, r);
-        Wb[hg::kUdeW2 + i * 16 + col] = double(w2[r]);
-        if (col < 3) Wb[hg::kUdeW1 + i * 3 + col] = double(w1[r]);
-        if (col == 3) Wb[hg::kUdeB1 + i] = double(w1[r]);
-        if (col == 0) Wb[hg::kUdeB2 + i] = double(w3[r]);
-        if (col == 1) Wb[hg::kUdeW3 + i] = double(w3[r]);
-      }
-      double b3 = double(Z3[l]);
-      for (int m = 32; m > 0; m >>= 1) b3 += __shfl_xor(b3, m, 64);
-      if (l == 0) Wb[hg::kUdeB3] = b3;
+    for (int kc = 0; kc < 16; ++kc) {
+      const int c = 4 * kc + kq;
+      w1 = M::op(Z1[c * 16 + col], col < 3 ? F[c * 4 + col] : T(col == 3), w1);
     }
-    __syncthreads();
-    const int slot = a.blk_slot ? a.blk_slot[a.blk_off[blockIdx.x] + pass] : int(blockIdx.x);
-    for (int k = l; k < hg::kUdeW; k += 64) a.wpart[size_t(slot) * hg::kUdeW + k] += Wb[k];
-    __syncthreads();  // the next pass rewrites the shared arrays
+    double b3 = double(Z3[l]);
+    for (int m = 32; m > 0; m >>= 1) b3 += __shfl_xor(b3, m, 64);
+    // this (block, set) slot: each of the 353 weights from exactly one lane
+    double* wp = a.wpart + size_t(a.blk_slot ? a.blk_slot[a.blk_off[blockIdx.x] + pass] : int(blockIdx.x)) *
+                              hg::kUdeW;
+    for (int r = 0; r < 4; ++r) {
+      const int ir = M::row(kq, r);
+      wp[hg::kUdeW2 + ir * 16 + col] += double(w2[r]);
+      if (col < 3) wp[hg::kUdeW1 + ir * 3 + col] += double(w1[r]);
+      if (col == 3) wp[hg::kUdeB1 + ir] += double(w1[r]);
+      if (col == 0) wp[hg::kUdeB2 + ir] += double(w3[r]);
+      if (col == 1) wp[hg::kUdeW3 + ir] += double(w3[r]);
+    }
+    if (l == 0) wp[hg::kUdeB3] += b3;
+    hg::wave_sync();  // the next pass rewrites A1, A2 and Z3
   }
 }
 
