@@ -268,6 +268,8 @@ struct hg_ctx {
   hipStream_t stream2 = nullptr;  // the adjoint's segment recompute, overlapping the reverse sweep
   hipEvent_t ev_seg[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr}, ev_fwd = nullptr;
   size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
+  double* adj_nseg[2] = {nullptr, nullptr};   // kck x N closure n of each recomputed step (two buffers)
+  size_t adj_nseg_cap = 0;
 
   // UDE closure for n (0: off; else the precision of its network, 32 or 64)
   int32_t ude_prec = 0;

@@ -1961,6 +1961,11 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (rc) return rc;
     ctx->adj_seg_cap = size_t(kck);
   }
+  if (ctx->ude_prec && size_t(kck) > ctx->adj_nseg_cap) {  // the closure's n of each step of a segment
+    for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(ctx, &ctx->adj_nseg[b], size_t(kck) * N);
+    if (rc) return rc;
+    ctx->adj_nseg_cap = size_t(kck);
+  }
   if (!ctx->stream2) {  // the recompute stream (default priority) and its events
     HG_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     for (hipEvent_t* e : {&ctx->ev_seg[0], &ctx->ev_seg[1], &ctx->ev_rev[0], &ctx->ev_rev[1], &ctx->ev_fwd})
@@ -2023,9 +2028,11 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
 // at state q, given the output cotangent lam; writes the state cotangent to
 // out and accumulates into adj_nbar / adj_zbbar / adj_Qbar / adj_S0bar.
 // totalA: the inlet conveyances of q when the forward pass recorded them
-// (else they are recomputed here).
+// (else they are recomputed here).  n_ready: ctx->n already holds the
+// closure's n of q (recorded by the recompute), so it is not evaluated again.
 int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
-                    double* const out[3], double dt, bool step, const double* totalA = nullptr) {
+                    double* const out[3], double dt, bool step, const double* totalA = nullptr,
+                    bool n_ready = false) {
   const bool ude = ctx->ude_prec != 0;
   const bool law = ctx->nlaw.kind != hg::NLAW_OFF;
   const bool lux = ctx->lux.choice != hg::LUX_OFF;
@@ -2034,7 +2041,7 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : lux_n ? ctx->lux_obar : ctx->adj_nbar;
   // the closure's, law's or network's n-bar (or friction-magnitude bar) of
   // this step is set by the adjoint kernel, not cleared and added to
-  if (ude) {
+  if (ude && !n_ready) {
     int rc = enqueue_ude_n(ctx, q);
     if (rc) return rc;
   }
@@ -2152,6 +2159,9 @@ int cell_out(hg_ctx* ctx, const double* dsrc, double* hdst) {
 
 double* seg_state(hg_ctx* ctx, int64_t t, int c, int b = 0) { return ctx->adj_seg[b] + (t * 3 + c) * ctx->N; }
 
+// The closure's n of step t of segment buffer b (recorded by the recompute).
+double* seg_n(hg_ctx* ctx, int64_t t, int b) { return ctx->adj_nseg[b] + t * ctx->N; }
+
 // Inlet conveyances of step t of the segment being reversed (nullptr without inlets).
 double* seg_totalA(hg_ctx* ctx, int64_t t, int b = 0) {
   return ctx->n_inlet > 0 ? ctx->adj_totalA[b] + t * ctx->n_inlet : nullptr;
@@ -2183,11 +2193,14 @@ int recompute_segment(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int64
     if (hipMemcpyAsync(seg_state(ctx, 0, c, b), ctx->adj_ckpt + (sgi * 3 + c) * N, N * sizeof(double),
                        hipMemcpyDeviceToDevice, stream) != hipSuccess)
       rc = HG_ERR_HIP;
+  double* const n_field = ctx->n;
   for (int64_t t = 0; t < ns && !rc; ++t) {
     const double* in[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
     double* out[3] = {seg_state(ctx, t + 1, 0, b), seg_state(ctx, t + 1, 1, b), seg_state(ctx, t + 1, 2, b)};
+    if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure writes this step's n where the reversal reads it
     rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask[b] + t * N, seg_totalA(ctx, t, b));
   }
+  ctx->n = n_field;
   ctx->stream = main;
   return rc;
 }
@@ -2226,7 +2239,11 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
       double* O = ctx->adj_lam[1 - lam_cur];
       const double* lam[3] = {L, L + N, L + 2 * N};
       double* out[3] = {O, O + N, O + 2 * N};
-      rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask[b] + t * N, out, dt, true, seg_totalA(ctx, t, b));
+      double* const n_field = ctx->n;
+      if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure's n of this state, from the recompute
+      rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask[b] + t * N, out, dt, true, seg_totalA(ctx, t, b),
+                           ctx->ude_prec != 0);
+      ctx->n = n_field;
       if (rc) return rc;
       lam_cur = 1 - lam_cur;
     }
