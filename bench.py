@@ -152,6 +152,32 @@ def measure_gradient(args, capi):
     return out
 
 
+def measure_gradient_full(args, capi, case, tile):
+    """The probe-loss gradient w.r.t. per-cell n on the forward workload's
+    own 1M-triangle mesh: the forward's step count, checkpoint every
+    --grad-kck steps, 200 probes, targets from a forward run with a 'true'
+    lognormal n (mean 0.035, sigma 0.3), initial guess n = 0.03."""
+    rng = np.random.default_rng(102)
+    sig = 0.3
+    n_true = np.exp(rng.normal(np.log(0.035) - 0.5 * sig * sig, sig, case.N))
+    probe = rng.choice(case.N, size=200, replace=False).astype(np.int32)
+    c = case.copy_fields(n=np.full(case.N, 0.03))
+    ctx = capi.Context(c)
+    ctx.set_active_param(capi.HG_PARAM_N_CELL)
+    ctx.set_tile_config(*tile)
+    QT = ctx.step(c.Q0, c.dt, args.steps, params=n_true)
+    target = QT[probe] + c.hstill[probe] + c.zb[probe]
+    ctx.grad_probes(c.Q0, c.dt, 20, args.grad_kck, probe, target, params=c.n)  # warm-up
+    t0 = time.perf_counter()
+    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, args.steps, args.grad_kck, probe, target, params=c.n)
+    el = time.perf_counter() - t0
+    return {"value": c.N * args.steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+            "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": args.steps,
+                       "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "per-cell Manning n",
+                       "dt": c.dt},
+            "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
+
+
 def batch_instances(n_inst, seed=7, lo=5_000, hi=500_000):
     """64 independent channel instances with N_i log-uniform in [5k, 500k]
     triangles, each with its own bed (seeded bumps), zone n and inlet
@@ -360,9 +386,12 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     if not args.no_grad:
+        tile = ctx.tile_config()
+        del ctx
+        out["gradient_1M"] = measure_gradient_full(args, capi, case, tile)
         out["gradient"] = measure_gradient(args, capi)
     if args.batch > 0:
-        del ctx
+        ctx = None
         out["batch"] = measure_batch(args, capi, value)
     if not args.no_ude:
         out["ude"] = measure_ude(args, capi)
