@@ -959,18 +959,19 @@ struct UdeRegs {
   }
 };
 
-// ude_hidden with the weights in registers.
-template <class T>
+// ude_hidden with the weights in registers, for the first ROWS cells (16,
+// 32, 48 or 64) of the chunk; each cell's values do not depend on ROWS.
+template <class T, int ROWS = 64>
 __device__ __forceinline__ void ude_hidden_r(const UdeRegs<T>& R, const T* F, T* A1, T* A2, int l) {
   using M = hg::Mfma<T>;
   const int col = l & 15, kq = l >> 4;
-  for (int rb = 0; rb < 4; ++rb) {
+  for (int rb = 0; rb < ROWS / 16; ++rb) {
     typename M::V acc = {T(0), T(0), T(0), T(0)};
     acc = M::op(F[(16 * rb + col) * 4 + kq], R.w1, acc);
     for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b1);
   }
   hg::wave_sync();
-  for (int rb = 0; rb < 4; ++rb) {
+  for (int rb = 0; rb < ROWS / 16; ++rb) {
     typename M::V acc = {T(0), T(0), T(0), T(0)};
     for (int kc = 0; kc < 4; ++kc) acc = M::op(A1[(16 * rb + col) * 16 + 4 * kc + kq], R.w2[kc], acc);
     for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b2);
@@ -979,25 +980,27 @@ __device__ __forceinline__ void ude_hidden_r(const UdeRegs<T>& R, const T* F, T*
 }
 
 // n = 0.01 + softplus(MLP(features)) for every cell.  Each wave walks
-// 64-cell chunks (grid-stride) with its weight set in registers; a chunk
-// whose cells use several sets takes one pass per set (the matrix-core
-// products need one set across the wave).
-template <class T, int WPB>
+// CH-cell chunks (grid-stride; CH = 64, or 32 to spread a small mesh over
+// twice the waves) with its weight set in registers; a chunk whose cells
+// use several sets takes one pass per set (the matrix-core products need one
+// set across the wave).
+template <class T, int WPB, int CH = 64>
 __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
-  __shared__ T Fs[WPB][64 * 4];
-  __shared__ T A1s[WPB][64 * 16];
-  __shared__ T A2s[WPB][64 * 16];
+  static_assert(CH == 32 || CH == 64, "chunk of 32 or 64 cells");
+  __shared__ T Fs[WPB][CH * 4];
+  __shared__ T A1s[WPB][CH * 16];
+  __shared__ T A2s[WPB][CH * 16];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int col = l & 15, kq = l >> 4;
   T* F = Fs[w];
   T* A1 = A1s[w];
   T* A2 = A2s[w];
-  const int nchunks = (a.N + 63) / 64;
+  const int nchunks = (a.N + CH - 1) / CH;
   UdeRegs<T> R;
   int cur = -1;
   for (int ch = blockIdx.x * WPB + w; ch < nchunks; ch += gridDim.x * WPB) {
-    const int i = ch * 64 + l;
-    const bool valid = i < a.N;
+    const int i = ch * CH + l;
+    const bool valid = l < CH && i < a.N;
     double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
     bool wet = false;
     int set = 0;
@@ -1006,10 +1009,12 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
       zb = a.zb[i];
       if (a.set_of_cell) set = a.set_of_cell[i];
     }
-    F[l * 4 + 0] = T(h);
-    F[l * 4 + 1] = T(um);
-    F[l * 4 + 2] = T(zb);
-    F[l * 4 + 3] = T(0);
+    if (l < CH) {  // F has CH rows
+      F[l * 4 + 0] = T(h);
+      F[l * 4 + 1] = T(um);
+      F[l * 4 + 2] = T(zb);
+      F[l * 4 + 3] = T(0);
+    }
     hg::wave_sync();
     uint64_t pending = __ballot(valid);
     while (pending) {
@@ -1021,7 +1026,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
         R.load(static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW, col, kq);
         cur = s;
       }
-      ude_hidden_r<T>(R, F, A1, A2, l);
+      ude_hidden_r<T, CH>(R, F, A1, A2, l);
       if (mine) {
         T z3 = R.b3;
         for (int j = 0; j < 16; ++j) z3 = z3 + A2[l * 16 + j] * R.w3[j];
@@ -1803,7 +1808,12 @@ int enqueue_ude_n(hg_ctx* ctx, const double* const q[3]) {
   if (!ctx->ude_prec) return HG_OK;
   const hg::UdeArgs u = ude_args(ctx, q);
   const int chunks = ude_blocks(ctx);  // 64-cell chunks
-  if (ctx->ude_prec == 32) {
+  if (ctx->ude_prec == 32 && chunks <= 2048) {
+    // a small mesh: 32-cell chunks, twice the waves, each half as long (the
+    // same n per cell); still within the 4,096 wave slots the kernel can hold
+    const int chunks32 = int((ctx->N + 31) / 32);
+    hipLaunchKernelGGL((k_ude_n<float, 4, 32>), dim3((chunks32 + 3) / 4), dim3(256), 0, ctx->stream, u);
+  } else if (ctx->ude_prec == 32) {
     const int grid = std::min((chunks + 3) / 4, 4096);
     hipLaunchKernelGGL((k_ude_n<float, 4>), dim3(grid), dim3(256), 0, ctx->stream, u);
   } else {

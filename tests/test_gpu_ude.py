@@ -95,6 +95,27 @@ def test_closure_steps_match_oracle():
     assert rel <= 1e-10
 
 
+def test_closure_n_independent_of_chunk_size():
+    """A small mesh runs the f32 closure in 32-cell chunks, a large one in
+    64-cell chunks; every cell's matrix-core products are the same either
+    way, so the n of the same cells is bitwise equal."""
+    c = small_case()
+    Q = pyoracle.Oracle(c).run(c.Q0, c.dt, 20)
+    W = ude.xavier_weights(14)
+    one = capi.Context(c)
+    one.set_ude(W, precision=32)
+    n1 = one.ude_n(Q)
+    k = -(-140_000 // c.N)
+    B = cases.concat([c] * k)
+    assert (B.N + 63) // 64 > 2048 >= (c.N + 63) // 64  # the two chunk sizes
+    QB = np.concatenate([Q.reshape(3, c.N)] * k, axis=1).reshape(-1)
+    big = capi.Context(B)
+    big.set_ude(W, precision=32)
+    nb = big.ude_n(QB)
+    for r in (0, k // 2, k - 1):
+        assert np.array_equal(nb[r * c.N:(r + 1) * c.N], n1), r
+
+
 def fd4(f, x, v, e):
     """Fourth-order central difference of f at x along v."""
     return (8.0 * (f(x + e * v) - f(x - e * v)) - (f(x + 2 * e * v) - f(x - 2 * e * v))) / (12.0 * e)
@@ -157,6 +178,29 @@ def test_closure_weight_gradient_fd():
     gv = float(np.dot(Q0b, v))
     say(f"  closure Q0 dir: fd {fd:.10e} adjoint {gv:.10e} rel {abs(fd - gv) / abs(gv):.2e}")
     assert abs(fd - gv) <= 1e-5 * abs(gv)
+
+
+def test_closure_gradient_independent_of_checkpoint_interval():
+    """The recompute keeps each step's closure n for the reverse step.  The
+    reverse steps see the same states and n whatever the segmenting, and each
+    block adds its weight cotangents in step order, so the gradient is bitwise
+    the same for any checkpoint interval (a wrongly indexed n would differ)."""
+    c = small_case()
+    nsteps = 23
+    probe, target = _probe(c, nsteps)
+    W = ude.xavier_weights(13)
+    res = []
+    for kck in (1, 4, 23):
+        ctx = capi.Context(c)
+        ctx.set_ude(W, precision=32)
+        loss, _, _, _ = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+        Q0b, _, _, _ = ctx.adjoint(c.Q0, c.dt, nsteps, kck, np.ones(3 * c.N))
+        res.append((loss, ctx.ude_grad(), Q0b))
+    for kck, r in zip((4, 23), res[1:]):
+        assert r[0] == res[0][0], kck
+        assert np.array_equal(r[1], res[0][1]), kck
+        assert np.array_equal(r[2], res[0][2]), kck
+    assert np.linalg.norm(res[0][1]) > 0.0
 
 
 def test_closure_f32_gradient_tracks_f64():
