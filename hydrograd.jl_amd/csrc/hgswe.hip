@@ -3191,8 +3191,8 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
     HG_TRY(hipStreamSynchronize(ctx->stream));
     return HG_OK;
   }
-  if (n_sets < 0 || !weights || (precision != 32 && precision != 64)) {
-    ctx->err = "hg_set_ude: need n_sets >= 0, weights and precision 32 or 64";
+  if (n_sets < 0 || n_sets > 65535 || !weights || (precision != 32 && precision != 64)) {
+    ctx->err = "hg_set_ude: need 0 <= n_sets <= 65535, weights and precision 32 or 64";
     return HG_ERR_ARG;
   }
   if (ctx->active == HG_PARAM_N_ZONE || ctx->active == HG_PARAM_N_CELL) {

@@ -274,9 +274,9 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
  * weights: n_sets x 353 values, per set W1[16][3], b1[16], W2[16][16]
  * (row = output unit), b2[16], W3[16], b3.  set_of_cell (caller order, NULL =
  * set 0 everywhere) picks each cell's set, e.g. one per batched instance.
- * precision: 32 (float, the production setting) or 64 (double).  n_sets = 0
- * turns the closure off and restores the n of hg_set_fields.  While on, n
- * cannot be the active parameter. */
+ * precision: 32 (float, the production setting) or 64 (double).  At most
+ * 65535 sets.  n_sets = 0 turns the closure off and restores the n of
+ * hg_set_fields.  While on, n cannot be the active parameter. */
 int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t* set_of_cell, int32_t precision);
 
 /* The closure's n (N, caller order) for state Q (3N). */
