@@ -323,6 +323,29 @@ def test_interleaved_weight_set_gradient_fd():
         assert abs(fd - gv) <= 1e-5 * abs(gv)
 
 
+def test_unused_weight_set_has_zero_cotangent():
+    """A set that no cell uses has an empty run of slots: its cotangent is
+    exactly zero, the used sets' are not, and a set's weights do not act
+    where it is unused."""
+    c = small_case(n=3000)
+    nsteps, kck = 10, 4
+    probe, target = _probe(c, nsteps)
+    Ws = ude.perturbed_sets(ude.xavier_weights(15), 4, scale=0.2)
+    soc = np.where(np.arange(c.N) % 2 == 0, 0, 3).astype(np.int32)  # sets 1 and 2 unused
+    ctx = capi.Context(c)
+    ctx.set_ude(Ws, set_of_cell=soc, precision=64)
+    ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g = ctx.ude_grad()
+    assert g.shape == (4, capi.UDE_WEIGHTS)
+    assert not np.any(g[1]) and not np.any(g[2])
+    assert np.linalg.norm(g[0]) > 0.0 and np.linalg.norm(g[3]) > 0.0
+    W2 = Ws.copy()
+    W2[1:3] = ude.xavier_weights(16)  # change only the unused sets
+    k = capi.Context(c)
+    k.set_ude(W2, set_of_cell=soc, precision=64)
+    assert np.array_equal(k.step(c.Q0, c.dt, nsteps), ctx.step(c.Q0, c.dt, nsteps))
+
+
 def test_closure_rejects_n_as_active_parameter():
     c = small_case()
     ctx = capi.Context(c)
