@@ -173,6 +173,96 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
   Bb = ext_to_side(eb, B, half_g);
 }
 
+// The part of roe_vjp that falls on A, for an internal face whose other
+// side's cotangent the neighbour forms from its own side: the cotangents of
+// A's raw and derived quantities are ADDED to e (unfolded; ext_to_side folds
+// a cell's sum once).  Same branches and routing as roe_vjp; B's own
+// cotangent is never formed.
+__host__ __device__ __forceinline__ void roe_vjp_a(const CellQ& A, const CellQ& B, double nx, double ny, double g,
+                                                   double half_g, double hmin, const double Fb[3], ExtBar& e) {
+  if (A.h <= hmin && B.h <= hmin) return;
+  int mode = 0;  // 0 general, 1 B mirrored from A, 2 A mirrored from B
+  double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
+  double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
+  if ((B.h <= hmin) && (A.h + A.zb) < (B.zb + hmin)) {
+    mode = 1;
+    hB = A.h; huB = -A.hu; hvB = -A.hv; uB = -A.u; vB = -A.v; sB = A.s;
+  } else if ((A.h <= hmin) && (B.h + B.zb) < (A.zb + hmin)) {
+    mode = 2;
+    hA = B.h; huA = -B.hu; hvA = -B.hv; uA = -B.u; vA = -B.v; sA = B.s;
+  } else if (A.h <= hmin) {  // B's physical flux alone: nothing reaches A
+    return;
+  } else if (B.h <= hmin) {  // A's physical flux with the h^2 pressure
+    const double un = nx * A.u + ny * A.v;
+    e.hu += Fb[0] * nx + Fb[1] * un;
+    e.hv += Fb[0] * ny + Fb[2] * un;
+    const double t = Fb[1] * A.hu + Fb[2] * A.hv;
+    e.u += t * nx;
+    e.v += t * ny;
+    e.h += (Fb[1] * nx + Fb[2] * ny) * half_g * 2.0 * (A.h + EPS);
+    return;
+  }
+  // ---- forward recompute (general branch)
+  const double hRoe = (hA + hB) * 0.5;
+  const double rsAB = 1.0 / (sA + sB);
+  const double uRoe = (sA * uA + sB * uB) * rsAB;
+  const double vRoe = (sA * vA + sB * vB) * rsAB;
+  const double un = uRoe * nx + vRoe * ny;
+  const double c = sqrt(g * hRoe + EPS);
+  const double rc = 1.0 / c;
+  const double o = 0.5 * rc;
+  const double d0 = B.xi - A.xi, d1 = huB - huA, d2 = hvB - hvA;
+  const double k0 = -(uRoe * ny - vRoe * nx);
+  const double w0 = k0 * d0 + ny * d1 + (-nx) * d2;
+  const double w1 = (un * o + 0.5) * d0 + (-nx * o) * d1 + (-ny * o) * d2;
+  const double w2 = (-un * o + 0.5) * d0 + (nx * o) * d1 + (ny * o) * d2;
+  const double um = un - c, up = un + c;
+  const double l0 = sqrt(un * un + EPS), l1 = sqrt(um * um + EPS), l2 = sqrt(up * up + EPS);
+  // ---- reverse: F = ((fA + fB) - r) / 2, r = R (|Lambda| w)
+  const double hf0 = 0.5 * Fb[0], hf1 = 0.5 * Fb[1], hf2 = 0.5 * Fb[2];
+  const double ab0 = hf2 * nx - hf1 * ny;
+  const double ab1 = -(hf0 + hf1 * (uRoe - c * nx) + hf2 * (vRoe - c * ny));
+  const double ab2 = -(hf0 + hf1 * (uRoe + c * nx) + hf2 * (vRoe + c * ny));
+  const double a12 = l1 * w1 + l2 * w2;
+  double cb = (hf1 * nx + hf2 * ny) * (l1 * w1 - l2 * w2);
+  const double q1b = ab1 * w1 * (um / l1), q2b = ab2 * w2 * (up / l2);  // d l_k: cotangent times l_k'
+  const double w0b = ab0 * l0, w1b = ab1 * l1, w2b = ab2 * l2;
+  double unb = ab0 * w0 * (un / l0) + q1b + q2b;
+  cb += q2b - q1b;
+  const double k0b = w0b * d0;
+  const double wd = w2b - w1b;
+  const double d0b = w0b * k0 + w1b * (un * o + 0.5) + w2b * (-un * o + 0.5);
+  const double d1b = w0b * ny + wd * nx * o;
+  const double d2b = -w0b * nx + wd * ny * o;
+  unb -= wd * o * d0;
+  cb += wd * (un * d0 - nx * d1 - ny * d2) * o * rc;
+  const double uRb = -hf1 * a12 - k0b * ny + unb * nx;
+  const double vRb = -hf2 * a12 + k0b * nx + unb * ny;
+  const double hRb = cb * g * (0.5 * rc);
+  const double pb = hf1 * nx + hf2 * ny;
+  e.xi -= d0b;
+  e.p += pb;
+  if (mode == 2) return;  // everything else reaches B, whose state A's mirror copies
+  const double unA = nx * uA + ny * vA;
+  const double tA = hf1 * huA + hf2 * hvA;
+  e.hu += hf0 * nx + hf1 * unA - d1b;
+  e.hv += hf0 * ny + hf2 * unA - d2b;
+  e.u += tA * nx + uRb * sA * rsAB;
+  e.v += tA * ny + vRb * sA * rsAB;
+  e.s += (uRb * (uA - uRoe) + vRb * (vA - vRoe)) * rsAB;
+  e.h += 0.5 * hRb;
+  if (mode == 1) {  // B's copy of A (hB = hA, qB = -qA, uB = -uA, sB = sA) routes back to A
+    const double unB = -unA;
+    const double tB = -tA;
+    e.hu -= hf0 * nx + hf1 * unB + d1b;
+    e.hv -= hf0 * ny + hf2 * unB + d2b;
+    e.u -= tB * nx + uRb * sB * rsAB;
+    e.v -= tB * ny + vRb * sB * rsAB;
+    e.s += (uRb * (uB - uRoe) + vRb * (vB - vRoe)) * rsAB;
+    e.h += 0.5 * hRb;
+  }
+}
+
 // VJP of the momentum source of one wet cell (semi_discretize_swe_2D.jl:
 // 474-475, 544-547): src = g xi S0 - friction.  y1, y2 are the cotangents of
 // the x / y momentum RHS; outputs add to the cotangents of xi, h, hu, hv,

@@ -230,6 +230,10 @@ struct hg_ctx {
   int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
   bool inlets_disjoint = true;  // no cell sits on two inlets: the inlet adjoint runs one work-group per inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
+  // cells with boundary faces (lib order) and their ghost slots in local
+  // face order, for the boundary part of the reverse step (k_adj_bnd)
+  int32_t n_bcell = 0;
+  int32_t *bc_off = nullptr, *bc_cell = nullptr, *bc_slot = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
   int32_t* zone = nullptr;
   // tiled kernel data

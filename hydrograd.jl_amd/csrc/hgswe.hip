@@ -41,6 +41,7 @@ using hg::StepArgs;
 
 static thread_local std::string g_create_err;  // per caller thread, like the contexts
 
+
 #define HG_TRY(expr)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \
@@ -53,6 +54,7 @@ static thread_local std::string g_create_err;  // per caller thread, like the co
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kRewarmSteps = 320;  // graph-replayed steps the autotuner ends with (a multiple of kGraphSteps)
 constexpr int kGraphSteps = 16;  // steps per captured graph (even: the graph returns to buffer 0)
 constexpr int kXCD = 8;         // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kTileInlets = 64;  // inlet boundaries one tile may touch when the conveyance is fused
@@ -603,14 +605,13 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
   return c;
 }
 
-// The VJP of the face in slot s (cell i = s % N, local face k = s / N) seen
-// from cell i: its contribution to the cotangent of the cell's state (U),
-// of n and of the exit-H bed; the neighbour computes the other side.
-// (Writing both sides from one work-item halves the face evaluations but
-// scatters the neighbour's writes; on a 6.4M-cell batch that is 30 % slower.)
+// The VJP of the boundary face in slot s (cell i = s % N, local face
+// k = s / N) seen from cell i: its contribution to the cotangent of the
+// cell's state (U), of n and of the exit-H bed, with the ghost's dependence
+// on the cell routed back through its boundary rule.
 template <bool STEP>
-__device__ __forceinline__ void adj_face(const hg::AdjArgs& a, const CellAdj& ca, int64_t s, hg::SideBar& U,
-                                         double& nb, double& zbb) {
+__device__ __forceinline__ void adj_ghost_face(const hg::AdjArgs& a, const CellAdj& ca, int64_t s, hg::SideBar& U,
+                                               double& nb, double& zbb) {
   const double hsmall = a.hsmall;
   const hg::CellQ& me = ca.me;
   const int j = a.nbr[s];
@@ -619,18 +620,7 @@ __device__ __forceinline__ void adj_face(const hg::AdjArgs& a, const CellAdj& ca
   U = hg::SideBar{0.0, 0.0, 0.0, 0.0};
   nb = 0.0;
   zbb = 0.0;
-  if (j >= 0) {
-    const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
-    double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
-    if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
-    const double dtj = a.dtc ? a.dtc[j] : a.dt;
-    const double yj0 = STEP ? dtj * m0 : m0, yj1 = STEP ? dtj * m1 : m1, yj2 = STEP ? dtj * m2 : m2;
-    const double fj = L / a.area[j];
-    const double c[3] = {fi * ca.y0 + fj * yj0, fi * ca.y1 + fj * yj1, fi * ca.y2 + fj * yj2};
-    hg::SideBar Ab, Bb;
-    hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
-    U = Ab;
-  } else {
+  {
     const int gi = ~j;
     const int kind = a.g_kind[gi];
     double hgh, qxg, qyg, v = 0.0, w = 0.0;
@@ -715,20 +705,57 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
                                                 double hb, double qxb, double qyb, double nb, double zbb);
 
 // Reverse mode of one RHS evaluation or one step, one work-item per cell:
-// the VJPs of the cell's faces (each seen from the cell), summed in local
-// face order, then the source VJP, the clamp and the identity part of the
-// step.  The neighbour's side of a face is its own work-item's.
+// the part of each internal face's VJP that falls on the cell (the flux
+// cotangent combines both cells' output cotangents, since the neighbour's
+// flux through the face is minus this one), accumulated in local face order
+// over the cell's raw and derived quantities and folded once; then the
+// source VJP, the clamp and the identity part of the step.  Boundary faces
+// are added by k_adj_bnd, which runs next: kept out of this kernel, their
+// ghost rules do not set its register budget.
 template <bool STEP>
 __global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   const CellAdj ca = cell_adj<STEP>(a, i);
-  double xib = 0.0, hb = 0.0, qxb = 0.0, qyb = 0.0, nb = 0.0, zbb = 0.0;
+  hg::ExtBar e;
+  hg::ext_zero(e);
   const int d = a.deg[i];
+  const double hsmall = a.hsmall;
+  const double rA = 1.0 / ca.A;
   for (int k = 0; k < d; ++k) {  // local face order
+    const int64_t s = int64_t(k) * a.N + i;
+    const int j = a.nbr[s];
+    if (j < 0) continue;
+    const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
+    const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
+    double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
+    if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
+    const double dtj = a.dtc ? a.dtc[j] : a.dt;
+    const double fj = L / a.area[j], fi = -(L * rA);
+    const double c[3] = {fi * ca.y0 + fj * (STEP ? dtj * m0 : m0), fi * ca.y1 + fj * (STEP ? dtj * m1 : m1),
+                         fi * ca.y2 + fj * (STEP ? dtj * m2 : m2)};
+    hg::roe_vjp_a(ca.me, B, nx, ny, a.g, a.half_g, hsmall, c, e);
+  }
+  const hg::SideBar U = hg::ext_to_side(e, ca.me, a.half_g);
+  adj_cell_finish<STEP>(a, i, ca, U.xi, U.h, U.hu, U.hv, 0.0, 0.0);
+}
+
+// The boundary faces' part of the reverse step, one work-item per cell that
+// has boundary faces (after k_adj_cell has written the cell's cotangents):
+// each ghost face's VJP with its boundary rule, summed in local face order,
+// then added through the same clamp as the cell's own sum.
+template <bool STEP>
+__global__ __launch_bounds__(kBlock) void k_adj_bnd(hg::AdjArgs a, int nbc, const int32_t* bc_off,
+                                                   const int32_t* bc_cell, const int32_t* bc_slot) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbc) return;
+  const int i = bc_cell[b];
+  const CellAdj ca = cell_adj<STEP>(a, i);
+  double xib = 0.0, hb = 0.0, qxb = 0.0, qyb = 0.0, nb = 0.0, zbb = 0.0;
+  for (int p = bc_off[b]; p < bc_off[b + 1]; ++p) {
     hg::SideBar U;
     double fn, fz;
-    adj_face<STEP>(a, ca, int64_t(k) * a.N + i, U, fn, fz);
+    adj_ghost_face<STEP>(a, ca, int64_t(bc_slot[p]), U, fn, fz);
     xib += U.xi;
     hb += U.h;
     qxb += U.hu;
@@ -736,7 +763,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a) {
     nb += fn;
     zbb += fz;
   }
-  adj_cell_finish<STEP>(a, i, ca, xib, hb, qxb, qyb, nb, zbb);
+  a.ox[i] += ca.wet ? xib + hb : xib;
+  if (ca.wet) {
+    a.oqx[i] += qxb;
+    a.oqy[i] += qyb;
+  }
+  if (a.nbar) a.nbar[i] += nb;
+  if (a.zbbar) a.zbbar[i] += zbb;
 }
 
 // The source VJP, the clamp and the identity part of the step, and the
@@ -2076,6 +2109,14 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
+  if (ctx->n_bcell > 0) {
+    if (step)
+      hipLaunchKernelGGL(k_adj_bnd<true>, dim3(grid_of(ctx->n_bcell)), dim3(kBlock), 0, ctx->stream, a,
+                         ctx->n_bcell, ctx->bc_off, ctx->bc_cell, ctx->bc_slot);
+    else
+      hipLaunchKernelGGL(k_adj_bnd<false>, dim3(grid_of(ctx->n_bcell)), dim3(kBlock), 0, ctx->stream, a,
+                         ctx->n_bcell, ctx->bc_off, ctx->bc_cell, ctx->bc_slot);
+  }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_adj, dim3(ctx->inlets_disjoint ? ctx->n_inlet : 1), dim3(kBlock), 0, ctx->stream,
                        ctx->n_inlet, ctx->in_off,
@@ -2822,6 +2863,31 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
         if (it->second != bi) ctx->inlets_disjoint = false;
       }
   }
+  if (!rc) {  // boundary cells and their ghost slots (k_adj_bnd)
+    std::vector<int32_t> off(1, 0), cell, slot;
+    for (int64_t li = 0; li < N; ++li) {
+      const size_t before = slot.size();
+      for (int32_t k = 0; k < h_deg[li]; ++k)
+        if (h_nbr[k * N + li] < 0) {
+          if (k * N + li > int64_t(INT32_MAX)) {
+            g_create_err = "hg_create: too many face slots for 32-bit slot indices";
+            return fail(HG_ERR_ARG);
+          }
+          slot.push_back(int32_t(k * N + li));
+        }
+      if (slot.size() > before) {
+        cell.push_back(int32_t(li));
+        off.push_back(int32_t(slot.size()));
+      }
+    }
+    ctx->n_bcell = int32_t(cell.size());
+    rc = dalloc(ctx, &ctx->bc_off, off.size());
+    if (!rc) rc = dalloc(ctx, &ctx->bc_cell, std::max<size_t>(cell.size(), 1));
+    if (!rc) rc = dalloc(ctx, &ctx->bc_slot, std::max<size_t>(slot.size(), 1));
+    if (!rc) rc = upload(ctx, ctx->bc_off, off.data(), off.size());
+    if (!rc) rc = upload(ctx, ctx->bc_cell, cell.data(), cell.size());
+    if (!rc) rc = upload(ctx, ctx->bc_slot, slot.data(), slot.size());
+  }
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
 
@@ -3122,6 +3188,15 @@ int hg_advance_region(hg_ctx* ctx, double dt, int64_t nsteps, double* gpu_ms_per
   if (!rc && gpu_ms_per_step) *gpu_ms_per_step = double(ms) / double(nsteps);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int hg_prepare_advance(hg_ctx* ctx, double dt) {
+  if (!ctx || !(dt > 0.0)) return HG_ERR_ARG;
+  if (require_fields(ctx, "hg_prepare_advance")) return HG_ERR_STATE;
+  int rc = ensure_graph(ctx, dt);
+  if (!rc) rc = advance_timed(ctx, dt, 0, nullptr);  // the dispatch events
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
   return rc;
 }
 
@@ -4048,6 +4123,19 @@ int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int
         bestX = shapes[c].second;
       }
     if (!rc) rc = hg_set_tile_config(ctx, bestT, bestX);
+    // Re-warm with the kept shape: switching back may rebuild the tiling on
+    // the host while the device idles, and a caller that times right after
+    // the tuner should find the device as warm as the candidates found it
+    // and the stepping graph for dt already instantiated.
+    for (int k = 0; k < 3 && !rc; ++k)
+      if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream) != hipSuccess)
+        rc = HG_ERR_HIP;
+    ctx->cur = 0;
+    if (!rc) rc = clear_flags(ctx);
+    if (!rc) rc = ensure_graph(ctx, dt);
+    if (!rc) rc = advance_async(ctx, dt, kRewarmSteps);
+    if (!rc) rc = check_flags(ctx);
     for (int k = 0; k < 3 && !rc; ++k)
       if (hipMemcpyAsync(ctx->st[0][k], ctx->scratch + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
                          ctx->stream) != hipSuccess)

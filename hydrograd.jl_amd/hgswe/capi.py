@@ -38,7 +38,7 @@ EXPORTS = [
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
     "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
-    "hg_advance_timed", "hg_advance_region",
+    "hg_advance_timed", "hg_advance_region", "hg_prepare_advance",
 ]
 UDE_WEIGHTS = 353
 
@@ -136,6 +136,7 @@ def lib():
         L.hg_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64]
         L.hg_advance_timed.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_double)]
         L.hg_advance_region.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_double)]
+        L.hg_prepare_advance.argtypes = [C.c_void_p, C.c_double]
         L.hg_cfl_dt.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
         L.hg_time_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
@@ -307,6 +308,11 @@ class Context:
                                        C.c_int64(nsteps), C.c_int64(kck), C.c_int64(pr.size), _ptr(pr), _ptr(tg),
                                        C.byref(loss), _ptr(nb), _ptr(zb), _ptr(ib)))
         return loss.value, nb, zb, ib[: self.case.mesh.n_inlet]
+
+    def prepare(self, dt):
+        """Instantiates the stepping graph for dt and the dispatch events
+        (no steps), so that the next timed advance builds nothing."""
+        self._chk(lib().hg_prepare_advance(self.h, C.c_double(dt)))
 
     def advance_region(self, dt, nsteps):
         """advance() bracketed by HIP events on the context stream: returns

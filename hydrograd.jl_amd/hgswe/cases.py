@@ -109,7 +109,8 @@ def dam_break(nx=100, ny=10, Lx=200.0, Ly=20.0, n=0.03, dt=0.01, nsteps=200, h_l
     zb = np.zeros(N)
     wse = np.where(mesh.centroid[:, 0] < Lx / 2, h_left, h_right)
     return _finish("dam_break_1k", mesh, zb, np.full(N, n), wse, np.zeros(N), np.zeros(N), np.zeros(N),
-                   [], [], dt, nsteps, zone=np.zeros(N, np.int32), zone_n=np.array([n]))
+                   [], [], dt, nsteps, zone=np.zeros(N, np.int32), zone_n=np.array([n]),
+                   z_nodes=np.zeros(len(xy)), nodestrings=ns, bc_types=bcs)
 
 
 def jittered_delaunay(nx, ny, Lx, Ly, seed, jitter=0.3):
@@ -195,7 +196,7 @@ def channel(n_target=1_000_000, seed=0, Lx=10_000.0, Ly=1_000.0, Q_in=500.0, wse
     case = _finish(name or f"channel_{N}", mesh, zb, n, wse, wstill, np.zeros(N), np.zeros(N), [Q_in] * n_in,
                    [wse_out] * n_ex,
                    0.0, nsteps, zone=zone, zone_n=np.concatenate([[zvals[0]], zvals]), h_small=h_small,
-                   z_nodes=z_nodes, cfl=cfl)
+                   z_nodes=z_nodes, cfl=cfl, nodestrings=ns, bc_types=bcs)
     case.dt = cfl_dt(case, cfl)
     return case
 

@@ -107,9 +107,9 @@ def write_case(dirpath, name, node_xyz, cells, nodestrings, bcs, iq=None, ews=No
             ids = " ".join(str(int(v) + 1) for v in row if v >= 0)
             fh.write(f"Elem {i + 1} {ids}\n")
         for i, (x, y, z) in enumerate(node_xyz):
-            fh.write(f"Node {i + 1} {x!r} {y!r} {z!r}\n")
+            fh.write(f"Node {i + 1} {float(x)!r} {float(y)!r} {float(z)!r}\n")
         for k, ns in sorted(nodestrings.items()):
-            fh.write(f"NodeString {k} " + " ".join(str(v + 1) for v in ns) + "\n")
+            fh.write(f"NodeString {k} " + " ".join(str(int(v) + 1) for v in ns) + "\n")
     with open(os.path.join(dirpath, name + ".srhhydro"), "w") as fh:
         fh.write(f"SRHHYDRO 30\nCase \"{name}\"\nDescription \"generated\"\nRunType FLOW\nModelTemp OFF\n"
                  "UnsteadyOutput UNSTEADY\nSimTime 0.0 0.1 1\nTurbulenceModel PARABOLIC\n"
@@ -117,13 +117,13 @@ def write_case(dirpath, name, node_xyz, cells, nodestrings, bcs, iq=None, ews=No
                  f"Grid \"{name}.srhgeom\"\nHydroMat \"{name}.srhmat\"\nOutputFormat XMDFC SI\n"
                  "OutputOption 1\nOutputInterval 0.1\n")
         for z, n in sorted((manning or {0: 0.03}).items()):
-            fh.write(f"ManningsN {z} {n!r}\n")
+            fh.write(f"ManningsN {int(z)} {float(n)!r}\n")
         for k, t in sorted(bcs.items()):
             fh.write(f"BC {k} {t}\n")
         for k, q in sorted((iq or {}).items()):
-            fh.write(f"IQParams {k} {q!r} SI CONVEYANCE\n")
+            fh.write(f"IQParams {k} {float(q)!r} SI CONVEYANCE\n")
         for k, w in sorted((ews or {}).items()):
-            fh.write(f"EWSParamsC {k} {w!r} SI C\n")
+            fh.write(f"EWSParamsC {k} {float(w)!r} SI C\n")
     zone = np.zeros(len(cells), dtype=np.int32) if zone is None else zone
     with open(os.path.join(dirpath, name + ".srhmat"), "w") as fh:
         mats = sorted(set(int(z) for z in zone if z > 0))

@@ -174,6 +174,13 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
  * average launch duration over these launches, back to back. */
 int hg_advance_region(hg_ctx* ctx, double dt, int64_t nsteps, double* gpu_ms_per_step);
 
+/* Set-up helper with no counterpart in the reference: instantiates the
+ * hipGraph that hg_advance replays for this dt and the dispatch events of
+ * hg_advance_timed, without advancing the state, so that a timed call that
+ * follows does no graph capture or event creation on the host.  (Changing
+ * the launch shape, hg_batch_set_dt or a different dt drops the graph.) */
+int hg_prepare_advance(hg_ctx* ctx, double dt);
+
 /* Measurement helper with no counterpart in the reference: hg_advance with
  * every step launched singly and start / stop events attached to each step
  * kernel's dispatch (hipExtLaunchKernelGGL); *kernel_ms is the mean duration
@@ -202,7 +209,10 @@ int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk);
  * step kernel over nsteps steps of every launch shape, twice each (events on
  * each dispatch, after 16 warm-up steps), from the current device state
  * (which is restored afterwards); keeps the shape with the lowest time and
- * reports that mean kernel time in *ms_per_step.  Needs hg_set_fields and a
+ * reports that mean kernel time in *ms_per_step.  It ends by instantiating
+ * the stepping graph for dt with the kept shape and replaying it for 320
+ * steps, so the device is warm and no graph is built by the next
+ * hg_advance; the state is then restored.  Needs hg_set_fields and a
  * device state (hg_state_upload). */
 int hg_autotune(hg_ctx* ctx, double dt, int64_t nsteps, int32_t* tile_cells, int32_t* xcd_chunk,
                 double* ms_per_step);

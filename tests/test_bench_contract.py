@@ -56,3 +56,15 @@ def test_cpu_baseline_objects():
     assert set(out) >= {"value", "unit", "cores", "kind", "sample", "all_cores"}
     assert out["cores"] == 1 and out["kind"] == "port" and out["value"] > 0
     assert out["all_cores"]["cores"] == b.host_threads() and out["all_cores"]["value"] > 0
+
+
+def test_gradient_bytes_per_horizon_cell_step():
+    """Forward sweep + recompute (180 B each on triangles), the reverse step
+    (24 + 24 + 24 + 48 + 28 x 3 + 16 = 220 B) and the checkpoint write 24/k."""
+    from hgswe import cases
+
+    b = bench_module()
+    tri = cases.channel(2_000, seed=1, Lx=400.0, Ly=40.0, nsteps=1)
+    assert b.bytes_per_gradient_cell_step(tri.mesh, 50) == 180 + 180 + 220 + 24 / 50
+    r = b.gradient_roofline(tri.mesh, 1_000_000, 1000, 50, 0.1)
+    assert abs(r["achieved"] - 580.48 * 1e9 / 0.1 / 1e9) < 1e-6 and r["frac"] == r["achieved"] / 8000.0

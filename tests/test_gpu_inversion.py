@@ -1,7 +1,8 @@
 """GPU checks of the inversion driver on the Savannah River mesh (the
 reference's ManningN inversion example, 6 Manning zones): the gradient of
 the reference's inversion loss through the device adjoint against central
-differences, and a twin experiment in which Adam recovers the zone values."""
+differences, a twin experiment in which Adam recovers the zone values, and
+the reference's Adam -> LBFGS schedule lowering the loss past Adam's."""
 import sys
 
 import numpy as np
@@ -63,3 +64,24 @@ def test_twin_experiment_recovers_zones(problem):
         f"n = {np.round(res.params, 4)}")
     assert res.loss[-1] < 0.05 * res.loss[0]
     assert err1 < 0.5 * err0
+
+
+def test_adam_then_lbfgs_lowers_the_loss_further(problem):
+    """The examples' ["Adam", "LBFGS"] schedule (run_control.json
+    inversion_optimizers; swe_2D_inversion.jl:664-729): LBFGS started from
+    Adam's parameters drives the loss well below where Adam stopped."""
+    c, ctx, dt, nsteps, n_true, truth = problem
+    p0 = np.full(6, 0.03)
+    res = inv.invert_staged(ctx, c, truth, p0, capi.HG_PARAM_N_ZONE, dt, nsteps,
+                            [("Adam", 2e-3, 30), ("LBFGS", None, 15)], kck=25, zone=c.zone)
+    adam = [L for k, it, L in res.stage_loss if k == 0]
+    lbfgs = [L for k, it, L in res.stage_loss if k == 1]
+    used = np.unique(c.zone)
+    err = np.max(np.abs(res.params[used] - n_true[used]))
+    say(f"  Adam -> LBFGS: loss {res.loss[0]:.3e} -> {adam[-1]:.3e} (Adam, 30 it) -> {lbfgs[-1]:.3e} "
+        f"(LBFGS, {res.stages[-1][2]} it, {res.stages[-1][1]}); {len(res.loss)} evaluations; "
+        f"max zone error {err:.2e}")
+    assert [s[0] for s in res.stages] == ["Adam", "LBFGS"]
+    assert lbfgs and lbfgs[-1] < 0.1 * adam[-1]
+    assert all(b <= a for a, b in zip(lbfgs, lbfgs[1:]))  # accepted LBFGS steps never raise the loss
+
