@@ -261,6 +261,7 @@ struct hg_ctx {
   int64_t probe_cap = 0;
   size_t scratch_n = 0;
   double* red = nullptr;      // reduction partials
+  double* noalloc_scratch = nullptr;  // device address of the static scratch of hg_cfl_dt_noalloc on this device
 
   // adjoint work buffers (allocated on first use)
   double* adj_lam[2] = {nullptr, nullptr};  // 3N each

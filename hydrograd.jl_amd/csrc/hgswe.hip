@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -41,6 +42,15 @@ using hg::StepArgs;
 
 static thread_local std::string g_create_err;  // per caller thread, like the contexts
 
+// Scratch of hg_cfl_dt_noalloc: the per-work-group partial minima of the CFL
+// reduction (one per 256 cells, so meshes of up to 32768 x 256 = 8,388,608
+// cells) followed by its result.  A static device array, so that the entry
+// point allocates nothing; each GPU holds its own copy (256 KiB) until the
+// process ends, and every call on a GPU shares that GPU's copy.  In host code
+// the name is only a stand-in: host code uses the address that
+// hipGetSymbolAddress returned on the device (kept per device below).
+constexpr int kNoAllocParts = 32768;
+static __device__ __align__(256) double scratch[kNoAllocParts + 32];
 
 #define HG_TRY(expr)                                                                   \
   do {                                                                                 \
@@ -52,6 +62,12 @@ static thread_local std::string g_create_err;  // per caller thread, like the co
   } while (0)
 
 namespace {
+
+// Device address of `scratch` on each device, resolved once per device by
+// the first hg_create with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
 
 constexpr int kBlock = 256;
 constexpr int kRewarmSteps = 320;  // graph-replayed steps the autotuner ends with (a multiple of kGraphSteps)
@@ -505,6 +521,21 @@ __global__ __launch_bounds__(kBlock) void k_cfl(int N, const double* xi, const d
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[0] = scale * min(part[0 .. n-1]), one work-group (the last stage of
+// the CFL reduction on the device).
+__global__ __launch_bounds__(kBlock) void k_min_parts(int n, const double* part, double scale, double* out) {
+  __shared__ double red[kBlock];
+  double v = INFINITY;
+  for (int j = threadIdx.x; j < n; j += kBlock) v = fmin(v, part[j]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = scale * red[0];
 }
 
 // Caller order -> library order: d_c[i] = src[c N + perm[i]] for the
@@ -2737,6 +2768,23 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
     ctx->err = "no HIP device / stream";
     return fail(HG_ERR_HIP);
   }
+  {  // the static scratch's address on this device (current now), once per device
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+      ctx->err = "device index beyond the per-device scratch table";
+      return fail(HG_ERR_HIP);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return fail(HG_ERR_HIP);
+      }
+      g_scratch_dev[ctx->device] = static_cast<double*>(p);
+    }
+    ctx->noalloc_scratch = g_scratch_dev[ctx->device];
+  }
   // ---- renumbering
   if (seg.empty()) seg = {0, N};
   ctx->inst_off = seg;
@@ -2822,6 +2870,7 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   ctx->scratch_n = size_t(std::max<int64_t>(3 * N, 4096));
   if (!rc) rc = dalloc(ctx, &ctx->scratch, ctx->scratch_n);
   if (!rc) rc = dalloc(ctx, &ctx->red, size_t(grid_of(N) + 64));
+  if (!rc) rc = ensure_xfer(ctx);  // host transfers never allocate after set-up
   if (rc) return fail(rc);
 
   if (!rc) rc = upload(ctx, ctx->deg, h_deg.data(), N);
@@ -3241,6 +3290,38 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
   double mn = INFINITY;
   for (double v : part) mn = std::min(mn, v);
   *dt = cfl * mn;
+  return HG_OK;
+}
+
+int hg_cfl_dt_noalloc(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
+  if (!ctx || !Q || !dt) return HG_ERR_ARG;
+  if (require_fields(ctx, "hg_cfl_dt_noalloc")) return HG_ERR_STATE;
+  const int64_t N = ctx->N;
+  const int nb = grid_of(N);
+  if (nb > kNoAllocParts) {
+    ctx->err = "hg_cfl_dt_noalloc: " + std::to_string(N) + " cells need " + std::to_string(nb) +
+               " partial minima; the static scratch holds " + std::to_string(kNoAllocParts) +
+               " (meshes of up to " + std::to_string(int64_t(kNoAllocParts) * kBlock) + " cells)";
+    return HG_ERR_ARG;
+  }
+  double* const sc = ctx->noalloc_scratch;
+  if (!sc || !ctx->stage || !ctx->d_perm) {
+    ctx->err = "hg_cfl_dt_noalloc: context set-up incomplete";
+    return HG_ERR_STATE;
+  }
+  // Q -> the context's transfer buffer -> state slot 0 in library order
+  // (as hg_cfl_dt); partial minima in scratch[0, nb), the result in
+  // scratch[kNoAllocParts].  Every slot read is written first by this call.
+  HG_TRY(hipMemcpyAsync(ctx->stage, Q, size_t(3) * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_perm_in, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, 3, ctx->d_perm, ctx->stage,
+                     ctx->st[0][0], ctx->st[0][1], ctx->st[0][2]);
+  ctx->cur = 0;
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(N), ctx->st[0][0], ctx->st[0][1],
+                     ctx->st[0][2], ctx->hstill, ctx->deg, ctx->flen, ctx->area, ctx->c.g, ctx->c.h_small, sc);
+  hipLaunchKernelGGL(k_min_parts, dim3(1), dim3(kBlock), 0, ctx->stream, nb, sc, cfl, sc + kNoAllocParts);
+  HG_TRY(hipGetLastError());
+  HG_TRY(hipMemcpyAsync(dt, sc + kNoAllocParts, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HG_TRY(hipStreamSynchronize(ctx->stream));
   return HG_OK;
 }
 

@@ -38,7 +38,7 @@ EXPORTS = [
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
     "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
-    "hg_advance_timed", "hg_advance_region", "hg_prepare_advance",
+    "hg_advance_timed", "hg_advance_region", "hg_prepare_advance", "hg_cfl_dt_noalloc",
 ]
 UDE_WEIGHTS = 353
 
@@ -138,6 +138,7 @@ def lib():
         L.hg_advance_region.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_double)]
         L.hg_prepare_advance.argtypes = [C.c_void_p, C.c_double]
         L.hg_cfl_dt.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
+        L.hg_cfl_dt_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
         L.hg_time_advance.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]
         L.hg_probe_copy.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_double)]
@@ -266,6 +267,14 @@ class Context:
         Q = np.ascontiguousarray(Q, dtype=np.float64)
         dt = C.c_double()
         self._chk(lib().hg_cfl_dt(self.h, _ptr(Q), C.c_double(cfl), C.byref(dt)))
+        return dt.value
+
+    def cfl_dt_noalloc(self, Q, cfl):
+        """cfl_dt through the entry point that allocates nothing (static
+        device scratch)."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        dt = C.c_double()
+        self._chk(lib().hg_cfl_dt_noalloc(self.h, _ptr(Q), C.c_double(cfl), C.byref(dt)))
         return dt.value
 
     def rhs_vjp(self, Q, ybar, params=None):

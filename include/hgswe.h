@@ -166,6 +166,16 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps);
  * custom_ODE_solvers.jl:39). */
 int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt);
 
+/* hg_cfl_dt for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc during a call).  No counterpart in
+ * the reference.  Same inputs, output and result bits as hg_cfl_dt; it
+ * allocates nothing: the CFL reduction's partial minima and result live in
+ * a static device array of 256 KiB per GPU, so meshes of up to 8,388,608
+ * cells (larger ones: HG_ERR_ARG).  Calls on one GPU share that array:
+ * calls that overlap on one GPU (several host threads, or contexts on
+ * several streams) race.  Like hg_cfl_dt, it leaves Q as the device state. */
+int hg_cfl_dt_noalloc(hg_ctx* ctx, const double* Q, double cfl, double* dt);
+
 /* Measurement helper with no counterpart in the reference: hg_advance
  * (hipGraph replays) with HIP events recorded on the context stream right
  * before the first and after the last step; *gpu_ms_per_step is the elapsed
