@@ -21,12 +21,17 @@ sys.path.insert(0, os.path.join(ROOT, "hydrograd.jl_amd"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def bytes_per_cell_step(mesh):
+def bytes_per_cell_step(mesh, fields=1):
     """Algorithmic HBM bytes of one forward cell-step: read xi, q_x, q_y (24),
     write them (24), per-cell constants h_still, zb, n, S0x, S0y, area (48),
-    per cell-face neighbour index (4) + n_x, n_y (16) + length (8)."""
+    per cell-face neighbour index (4) + n_x, n_y (16) + length (8).  With K
+    fields sharing the mesh, each field reads its own state and n (56 B) and
+    the mesh data (h_still, zb, S0x, S0y, area and the faces) is read once
+    for all K: 40 + 28 x degree over K."""
     avg_deg = float(mesh.face_off[-1]) / mesh.n_cells
-    return 24 + 24 + 48 + 28 * avg_deg
+    if fields == 1:
+        return 24 + 24 + 48 + 28 * avg_deg
+    return 24 + 24 + 8 + (40 + 28 * avg_deg) / fields
 
 
 def bytes_per_gradient_cell_step(mesh, kck):
@@ -269,38 +274,43 @@ def measure_ude(args, capi):
     out = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / args.ude_steps,
            "config": {"workload": "channel_50k_tri_ude_closure_f32", "cells": c.N, "steps": args.ude_steps,
                       "dt": c.dt, "network": "3-16-16-1 tanh, 353 weights, f32 matrix cores"}}
-    # 32 weight perturbations batched: 32 copies of the mesh, one weight set each
+    # 32 weight perturbations batched, one weight set per copy of the mesh:
+    # as a batch of 32 copies (the mesh stored 32 times) and as 32 fields of
+    # one mesh (stored once); both give the same bits
     nset = args.ude_sets
     B = cases.concat([c] * nset)
     Ws = ude.perturbed_sets(W, nset, scale=0.05)
     set_of_cell = np.repeat(np.arange(nset), c.N).astype(np.int32)
-    bctx = capi.Context(B)
-    bctx.set_ude(Ws, set_of_cell=set_of_cell, precision=32)
-    bctx.upload(B.Q0)
-    bctx.advance(B.dt, args.warmup)
-    t0 = time.perf_counter()
-    bctx.advance(B.dt, args.ude_steps)
-    el = time.perf_counter() - t0
-    out["batched_sets"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "sets": nset,
-                           "cells": B.N, "ms_per_step": el * 1e3 / args.ude_steps}
-    # one reverse sweep over the batch: the gradient of each set's own probe
-    # loss (200 probes per instance) w.r.t. its 353 weights, all 32 at once
     rng = np.random.default_rng(6)
     bprobe = np.concatenate([r * c.N + rng.choice(c.N, size=200, replace=False) for r in range(nset)])
     bprobe = bprobe.astype(np.int32)
-    QB = bctx.step(B.Q0, B.dt, args.ude_steps)
-    btarget = QB[bprobe] + B.hstill[bprobe] + B.zb[bprobe] + rng.normal(0.0, 0.02, bprobe.size)
-    bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget)  # warm-up
-    t0 = time.perf_counter()
-    bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget)
-    gws = bctx.ude_grad()
-    el = time.perf_counter() - t0
-    del bctx
-    out["batched_sets"]["gradient"] = {
-        "value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
-        "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "probes": int(bprobe.size),
-        "wrt": f"{nset} x 353 closure weights (each set's own instance loss)",
-        "grad_norm_min": float(np.min(np.linalg.norm(gws.reshape(nset, -1), axis=1)))}
+    noise = rng.normal(0.0, 0.02, bprobe.size)
+    for key, share in (("batched_sets", False), ("batched_sets_shared_mesh", True)):
+        bctx = capi.Context(B, share_mesh=share)
+        bctx.set_ude(Ws, set_of_cell=set_of_cell, precision=32)
+        bctx.upload(B.Q0)
+        bctx.advance(B.dt, max(args.warmup, 100))
+        t0 = time.perf_counter()
+        bctx.advance(B.dt, args.ude_steps)
+        el = time.perf_counter() - t0
+        out[key] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "sets": nset, "cells": B.N,
+                    "ms_per_step": el * 1e3 / args.ude_steps,
+                    "bytes_per_cell_step": bytes_per_cell_step(c.mesh, nset if share else 1)}
+        # one reverse sweep over the batch: the gradient of each set's own
+        # probe loss (200 probes per instance) w.r.t. its 353 weights
+        QB = bctx.step(B.Q0, B.dt, args.ude_steps)
+        btarget = QB[bprobe] + B.hstill[bprobe] + B.zb[bprobe] + noise
+        bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget)  # warm-up
+        t0 = time.perf_counter()
+        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget)
+        gws = bctx.ude_grad()
+        el = time.perf_counter() - t0
+        del bctx
+        out[key]["gradient"] = {
+            "value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+            "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "probes": int(bprobe.size),
+            "wrt": f"{nset} x 353 closure weights (each set's own instance loss)",
+            "grad_norm_min": float(np.min(np.linalg.norm(gws.reshape(nset, -1), axis=1)))}
     # gradient w.r.t. the weights
     rng = np.random.default_rng(5)
     probe = rng.choice(c.N, size=200, replace=False).astype(np.int32)
@@ -314,6 +324,47 @@ def measure_ude(args, capi):
     out["gradient"] = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
                        "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "wrt": "353 closure weights",
                        "grad_norm": float(np.linalg.norm(gw))}
+    return out
+
+
+def measure_fields(args, capi):
+    """K Manning-n fields (lognormal, mean 0.035, sigma 0.3) on the ~50k-
+    triangle channel: the K fields as a batch of K copies of the mesh and as
+    K fields of one mesh (hg_fields_create), forward steps and the probe-loss
+    gradient of every field w.r.t. its own n."""
+    from hgswe import cases
+
+    c = cases.channel(args.ude_cells, seed=4, nsteps=args.ude_steps)
+    K = args.fields
+    rng = np.random.default_rng(31)
+    inst = [c.copy_fields(n=np.exp(rng.normal(np.log(0.035) - 0.045, 0.3, c.N))) for _ in range(K)]
+    B = cases.concat(inst)
+    probe = np.concatenate([f * c.N + rng.choice(c.N, size=200, replace=False) for f in range(K)]).astype(np.int32)
+    noise = rng.normal(0.0, 0.02, probe.size)
+    out = {"config": {"workload": f"channel_50k_tri_{K}_manning_n_fields", "fields": K, "cells_per_field": c.N,
+                      "steps": args.ude_steps, "dt": B.dt}}
+    for key, share in (("batch_of_copies", False), ("shared_mesh", True)):
+        k = capi.Context(B, share_mesh=share)
+        k.upload(B.Q0)
+        k.advance(B.dt, max(args.warmup, 100))
+        t0 = time.perf_counter()
+        k.advance(B.dt, args.ude_steps)
+        el = time.perf_counter() - t0
+        ms_k = k.advance_timed(B.dt, 50)
+        bpc = bytes_per_cell_step(c.mesh, K if share else 1)
+        r = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / args.ude_steps,
+             "kernel_ms": ms_k, "bytes_per_cell_step": bpc,
+             "achieved_GBps": bpc * B.N / (ms_k * 1e-3) / 1e9}
+        QN = k.step(B.Q0, B.dt, args.ude_steps)
+        target = QN[probe] + B.hstill[probe] + B.zb[probe] + noise
+        k.grad_probes(B.Q0, B.dt, 10, 5, probe, target)  # warm-up
+        t0 = time.perf_counter()
+        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, probe, target)
+        el = time.perf_counter() - t0
+        r["gradient"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+                         "wrt": "each field's own per-cell n"}
+        out[key] = r
+        del k
     return out
 
 
@@ -339,6 +390,7 @@ def main():
     ap.add_argument("--ude-steps", type=int, default=1000)
     ap.add_argument("--ude-sets", type=int, default=32)
     ap.add_argument("--no-ude", action="store_true")
+    ap.add_argument("--fields", type=int, default=32, help="Manning-n fields of the shared-mesh leg (0: skip it)")
     ap.add_argument("--tile-config", default="", help="T,G: fixed launch shape (tile cells, XCD chunk); no autotune")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (256-cell tiles, identity tile mapping)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -422,6 +474,8 @@ def main():
         out["batch"] = measure_batch(args, capi, value)
     if not args.no_ude:
         out["ude"] = measure_ude(args, capi)
+    if args.fields > 0:
+        out["fields"] = measure_fields(args, capi)
     print(json.dumps(out), flush=True)
 
 

@@ -16,8 +16,10 @@ namespace hg {
 // library's (possibly renumbered) order; faces in ELL form, slot k of cell i
 // at k * N + i, so every face load of a wavefront is coalesced.
 struct StepArgs {
-  int32_t N;
+  int32_t N;  // cells per field (the mesh); the launch covers K fields of N cells
   int32_t maxdeg;
+  int32_t K;                // fields sharing the mesh (1 unless the context holds K fields)
+  int32_t nin_m, nex_m;     // inlet / exit boundaries per field
   const double* xi;   // state in
   const double* qx;
   const double* qy;
@@ -47,10 +49,13 @@ struct StepArgs {
   double g, half_g, kn2, hsmall, dt;
   const double* dtc;  // per-cell time step (batches with per-instance steps), or NULL: dt
   const double* fmag; // friction magnitude per cell (FlowResistance closure), or NULL: Manning friction
+  int32_t* flag;      // [1] set to 1 when a step writes a NaN or infinite value
 };
 
 struct InletArgs {
-  int32_t n_inlet;
+  int32_t n_inlet;  // inlets over all fields (one work-group each)
+  int32_t nin_m;    // inlets per field (n_inlet / K)
+  int32_t Nm;       // cells per field
   const int32_t* off;
   const int32_t* cell;
   const double* L53;
@@ -80,7 +85,9 @@ struct ItemRec {
 //                     (first side only; ref = global neighbour), 2 ghost
 //                     (ref = ghost id)
 struct TileArgs {
-  int32_t N;
+  int32_t N;  // cells per field (the mesh); the launch covers K fields of N cells
+  int32_t K;
+  int32_t nin_m, nex_m;  // inlet / exit boundaries per field
   int32_t T;
   int32_t maxdeg;
   int32_t n_inlet;  // inlets whose conveyance the kernel forms (0: done by a separate kernel)
@@ -123,8 +130,10 @@ struct TileArgs {
 
 // Arguments of the adjoint (VJP) kernel of one RHS evaluation / one step.
 struct AdjArgs {
-  int32_t N;
+  int32_t N;  // cells per field (the mesh); the launch covers K fields of N cells
   int32_t maxdeg;
+  int32_t K;
+  int32_t nin_m, nex_m, G_m;  // inlet / exit boundaries and ghosts per field
   const double* xi;  // state the RHS was evaluated at
   const double* qx;
   const double* qy;
@@ -166,6 +175,49 @@ struct AdjArgs {
   int32_t fbar_set;   // 1: fbar likewise
 };
 
+// Per-field views of a launch over K fields that share one mesh: field f's
+// state-like arrays start f N cells into the arrays over all fields (its
+// inlet / exit / ghost arrays f nin_m / nex_m / G_m entries in, its S0
+// cotangent f 2N), while the mesh geometry and per-cell constants are
+// indexed by the cell within the mesh.  Field 0 is the arrays themselves.
+template <class T>
+__host__ __device__ __forceinline__ T* shift(T* p, int64_t o) { return p ? p + o : p; }
+
+__host__ __device__ __forceinline__ StepArgs field_view(StepArgs a, int f) {
+  const int64_t o = int64_t(f) * a.N;
+  a.xi = shift(a.xi, o); a.qx = shift(a.qx, o); a.qy = shift(a.qy, o);
+  a.xo = shift(a.xo, o); a.qxo = shift(a.qxo, o); a.qyo = shift(a.qyo, o);
+  a.n = shift(a.n, o); a.clamp = shift(a.clamp, o); a.dtc = shift(a.dtc, o); a.fmag = shift(a.fmag, o);
+  a.inletQ = shift(a.inletQ, int64_t(f) * a.nin_m); a.totalA = shift(a.totalA, int64_t(f) * a.nin_m);
+  a.exitWSE = shift(a.exitWSE, int64_t(f) * a.nex_m);
+  return a;
+}
+
+__host__ __device__ __forceinline__ TileArgs field_view(TileArgs a, int f) {
+  const int64_t o = int64_t(f) * a.N;
+  a.xi = shift(a.xi, o); a.qx = shift(a.qx, o); a.qy = shift(a.qy, o);
+  a.xo = shift(a.xo, o); a.qxo = shift(a.qxo, o); a.qyo = shift(a.qyo, o);
+  a.n = shift(a.n, o); a.clamp = shift(a.clamp, o); a.dtc = shift(a.dtc, o); a.fmag = shift(a.fmag, o);
+  a.inletQ = shift(a.inletQ, int64_t(f) * a.nin_m); a.totalA = shift(a.totalA, int64_t(f) * a.nin_m);
+  a.totalA_out = shift(a.totalA_out, int64_t(f) * a.nin_m);
+  a.exitWSE = shift(a.exitWSE, int64_t(f) * a.nex_m);
+  return a;
+}
+
+__host__ __device__ __forceinline__ AdjArgs field_view(AdjArgs a, int f) {
+  const int64_t o = int64_t(f) * a.N;
+  a.xi = shift(a.xi, o); a.qx = shift(a.qx, o); a.qy = shift(a.qy, o);
+  a.lx = shift(a.lx, o); a.lqx = shift(a.lqx, o); a.lqy = shift(a.lqy, o);
+  a.mask = shift(a.mask, o);
+  a.ox = shift(a.ox, o); a.oqx = shift(a.oqx, o); a.oqy = shift(a.oqy, o);
+  a.nbar = shift(a.nbar, o); a.zbbar = shift(a.zbbar, o); a.S0bar = shift(a.S0bar, 2 * o);
+  a.vbar = shift(a.vbar, int64_t(f) * a.G_m);
+  a.n = shift(a.n, o); a.dtc = shift(a.dtc, o); a.fmag = shift(a.fmag, o); a.fbar = shift(a.fbar, o);
+  a.inletQ = shift(a.inletQ, int64_t(f) * a.nin_m); a.totalA = shift(a.totalA, int64_t(f) * a.nin_m);
+  a.exitWSE = shift(a.exitWSE, int64_t(f) * a.nex_m);
+  return a;
+}
+
 // A variable Manning-n law (kind HG_NLAW_*; 0: off) and its parameters.
 enum { NLAW_OFF = 0, NLAW_POWER = 1, NLAW_SIGMOID = 2, NLAW_INVERSE = 3, NLAW_H_UMAG_KS = 4 };
 struct NLaw {
@@ -198,9 +250,31 @@ struct DBuf {  // owning device buffer
 
 }  // namespace hg
 
+// Key of a cached hipGraph of the adjoint's per-segment work: kind 0 the
+// recompute of ns steps into segment buffer b, kind 1 the reversal of ns
+// steps in buffer b starting from cotangent buffer lam0; for time step dt.
+struct SegGraphKey {
+  int kind, b, lam0;
+  int64_t ns;
+  double dt;
+  bool operator<(const SegGraphKey& o) const {
+    if (kind != o.kind) return kind < o.kind;
+    if (b != o.b) return b < o.b;
+    if (lam0 != o.lam0) return lam0 < o.lam0;
+    if (ns != o.ns) return ns < o.ns;
+    return dt < o.dt;
+  }
+};
+
 struct hg_ctx {
   std::string err;
   int64_t N = 0, G = 0;
+  // K fields on one mesh (hg_fields_create; K = 1 otherwise): N, G, n_inlet,
+  // n_exit and n_inlet_faces count all fields; the geometry and the per-cell
+  // constants (h_still, zb, S0) are stored once for the mesh of Nm cells
+  int32_t K = 1;
+  int64_t Nm = 0, Gm = 0;
+  int32_t nin_m = 0, nex_m = 0;
   int32_t maxdeg = 0;
   int32_t n_inlet = 0, n_exit = 0, n_inlet_faces = 0;
   hg_consts c{};
@@ -230,10 +304,6 @@ struct hg_ctx {
   int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
   bool inlets_disjoint = true;  // no cell sits on two inlets: the inlet adjoint runs one work-group per inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
-  // cells with boundary faces (lib order) and their ghost slots in local
-  // face order, for the boundary part of the reverse step (k_adj_bnd)
-  int32_t n_bcell = 0;
-  int32_t *bc_off = nullptr, *bc_cell = nullptr, *bc_slot = nullptr;
   int32_t* flags = nullptr;  // [0] conveyance error
   int32_t* zone = nullptr;
   // tiled kernel data
@@ -307,6 +377,7 @@ struct hg_ctx {
   double* lux_obar = nullptr;  // N
   double* lux_wpart = nullptr; // P x blocks
   double* lux_wbar = nullptr;  // P
+  double* lux_dw = nullptr;    // weight tangent of hg_jvp (kLuxMaxP)
   std::vector<void*> lux_allocs;
 
   // batches: instance boundaries in caller order, and the per-cell time step
@@ -327,6 +398,9 @@ struct hg_ctx {
 
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;
+  // cached graphs of the adjoint's segment recompute and reversal (dropped
+  // with gexec whenever a launch they captured would change)
+  std::map<SegGraphKey, hipGraphExec_t> seg_graphs;
   double gdt = 0.0;
 
   std::vector<void*> allocs;

@@ -85,84 +85,110 @@ __device__ __forceinline__ void lux_features(const LuxNet& L, const T& xi, const
   }
 }
 
+// Scalars of the forward pass: double, or Dual1 for forward mode (value and
+// one tangent; the tangent of weight k is dP[k], none when dP is NULL).  The
+// value part of a Dual1 pass is computed by the same operations in the same
+// order as the double pass, so both give the same bits.
+template <class T>
+__device__ __forceinline__ T lux_w(const double* __restrict__ P, const double* dP, int k);
+template <>
+__device__ __forceinline__ double lux_w<double>(const double* __restrict__ P, const double*, int k) { return P[k]; }
+template <>
+__device__ __forceinline__ Dual1 lux_w<Dual1>(const double* __restrict__ P, const double* dP, int k) {
+  return Dual1(P[k], dP ? dP[k] : 0.0);
+}
+__device__ __forceinline__ double act_t(int k, double z) { return act_f(k, z); }
+__device__ __forceinline__ Dual1 act_t(int k, const Dual1& z) {
+  const double a = act_f(k, z.v);
+  return Dual1(a, act_d(k, z.v, a) * z.d);
+}
+__device__ __forceinline__ double sigmoid_t(double x) { return nn_sigmoid(x); }
+__device__ __forceinline__ Dual1 sigmoid_t(const Dual1& x) {
+  const double s = nn_sigmoid(x.v);
+  return Dual1(s, s * (1.0 - s) * x.d);
+}
+__device__ __forceinline__ double sqrt_t(double x) { return ::sqrt(x); }
+__device__ __forceinline__ Dual1 sqrt_t(const Dual1& x) { return hg::sqrt(x); }
+
 // Hidden layer l on input x (width win): pre-activation z, activation a,
 // normalised xh = (a - mean) / s.
-template <int MW>
-__device__ __forceinline__ void lux_layer(const LuxNet& L, const double* __restrict__ P, int l, const double (&x)[MW],
-                                          int win, double (&z)[MW], double (&a)[MW], double (&xh)[MW], double& s) {
+template <int MW, class T = double>
+__device__ __forceinline__ void lux_layer(const LuxNet& L, const double* __restrict__ P, int l, const T (&x)[MW],
+                                          int win, T (&z)[MW], T (&a)[MW], T (&xh)[MW], T& s,
+                                          const double* dP = nullptr) {
   const int w = L.width[l];
   const int act = L.act[l];
-  const double* W = P + L.off_W[l];
-  const double* b = P + L.off_b[l];
-  double mu = 0.0;
+  const int oW = L.off_W[l], ob = L.off_b[l];
+  T mu = 0.0;
 #pragma unroll
   for (int j = 0; j < MW; ++j) {
     if (j < w) {
-      double acc = 0.0;
+      T acc = 0.0;
 #pragma unroll
       for (int k = 0; k < MW; ++k)
-        if (k < win) acc = (k == 0) ? W[j] * x[0] : acc + W[j + k * w] * x[k];
-      z[j] = acc + b[j];
-      a[j] = act_f(act, z[j]);
+        if (k < win)
+          acc = (k == 0) ? lux_w<T>(P, dP, oW + j) * x[0] : acc + lux_w<T>(P, dP, oW + j + k * w) * x[k];
+      z[j] = acc + lux_w<T>(P, dP, ob + j);
+      a[j] = act_t(act, z[j]);
       mu = (j == 0) ? a[0] : mu + a[j];
     }
   }
   mu = mu / double(w);
-  double var = 0.0;
+  T var = 0.0;
 #pragma unroll
   for (int j = 0; j < MW; ++j) {
     if (j < w) {
-      const double d = a[j] - mu;
+      const T d = a[j] - mu;
       var = (j == 0) ? d * d : var + d * d;
     }
   }
   var = var / double(w);
-  s = ::sqrt(var + L.eps);
+  s = sqrt_t(var + L.eps);
 #pragma unroll
   for (int j = 0; j < MW; ++j)
     if (j < w) xh[j] = (a[j] - mu) / s;
 }
 
 // Input of hidden layer `upto` (the last hidden output when upto == n_hidden).
-template <int MW>
-__device__ __forceinline__ int lux_forward_to(const LuxNet& L, const double* __restrict__ P, const double (&f)[3],
-                                              int upto, double (&x)[MW]) {
+template <int MW, class T = double>
+__device__ __forceinline__ int lux_forward_to(const LuxNet& L, const double* __restrict__ P, const T (&f)[3],
+                                              int upto, T (&x)[MW], const double* dP = nullptr) {
 #pragma unroll
-  for (int k = 0; k < MW; ++k) x[k] = k < 3 ? f[k < 3 ? k : 0] : 0.0;
+  for (int k = 0; k < MW; ++k) x[k] = k < 3 ? f[k < 3 ? k : 0] : T(0.0);
   int win = L.n_in;
   for (int l = 0; l < upto; ++l) {
-    double z[MW], a[MW], xh[MW], s;
-    lux_layer<MW>(L, P, l, x, win, z, a, xh, s);
+    T z[MW], a[MW], xh[MW], s;
+    lux_layer<MW, T>(L, P, l, x, win, z, a, xh, s, dP);
     const int w = L.width[l];
-    const double* g = P + L.off_g[l];
-    const double* be = P + L.off_be[l];
+    const int og = L.off_g[l], obe = L.off_be[l];
 #pragma unroll
     for (int j = 0; j < MW; ++j)
-      if (j < w) x[j] = xh[j] * g[j] + be[j];
+      if (j < w) x[j] = xh[j] * lux_w<T>(P, dP, og + j) + lux_w<T>(P, dP, obe + j);
     win = w;
   }
   return win;
 }
 
 // Output layer and bounds: returns lo + (hi - lo) sigmoid(Wo x + bo).
-template <int MW>
-__device__ __forceinline__ double lux_output(const LuxNet& L, const double* __restrict__ P, const double (&x)[MW],
-                                             int win, double& sg) {
-  const double* Wo = P + L.off_Wo;
-  double acc = 0.0;
+template <int MW, class T = double>
+__device__ __forceinline__ T lux_output(const LuxNet& L, const double* __restrict__ P, const T (&x)[MW], int win,
+                                        T& sg, const double* dP = nullptr) {
+  T acc = 0.0;
 #pragma unroll
   for (int k = 0; k < MW; ++k)
-    if (k < win) acc = (k == 0) ? Wo[0] * x[0] : acc + Wo[k] * x[k];
-  sg = nn_sigmoid(acc + P[L.off_bo]);
+    if (k < win)
+      acc = (k == 0) ? lux_w<T>(P, dP, L.off_Wo) * x[0] : acc + lux_w<T>(P, dP, L.off_Wo + k) * x[k];
+  sg = sigmoid_t(acc + lux_w<T>(P, dP, L.off_bo));
   return L.lo + (L.hi - L.lo) * sg;
 }
 
-template <int MW>
-__device__ __forceinline__ double lux_eval(const LuxNet& L, const double* __restrict__ P, const double (&f)[3]) {
-  double x[MW];
-  const int win = lux_forward_to<MW>(L, P, f, L.n_hidden, x);
-  double sg;
-  return lux_output<MW>(L, P, x, win, sg);
+template <int MW, class T = double>
+__device__ __forceinline__ T lux_eval(const LuxNet& L, const double* __restrict__ P, const T (&f)[3],
+                                      const double* dP = nullptr) {
+  T x[MW];
+  const int win = lux_forward_to<MW, T>(L, P, f, L.n_hidden, x, dP);
+  T sg;
+  return lux_output<MW, T>(L, P, x, win, sg, dP);
 }
 
 // Sum over the 64 lanes of a wave (fixed xor tree: every lane gets the total).

@@ -75,7 +75,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 struct UdeArgs {
-  int32_t N;
+  int32_t N;   // cells over all fields
+  int32_t Nm;  // cells per field: h_still and zb of cell i are at i % Nm
   int32_t n_sets;
   const double* xi;
   const double* qx;
@@ -103,7 +104,7 @@ struct UdeArgs {
 __device__ __forceinline__ void ude_features(const UdeArgs& a, int i, double& h, double& u, double& v, double& um,
                                              bool& wet) {
   const double xi = a.xi[i];
-  h = xi + a.hstill[i];
+  h = xi + a.hstill[i % a.Nm];
   double hu = a.qx[i], hv = a.qy[i];
   wet = h > a.hsmall;
   if (!wet) {

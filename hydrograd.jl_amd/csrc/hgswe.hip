@@ -70,7 +70,7 @@ std::mutex g_scratch_mu;
 double* g_scratch_dev[kMaxDevices] = {};
 
 constexpr int kBlock = 256;
-constexpr int kRewarmSteps = 320;  // graph-replayed steps the autotuner ends with (a multiple of kGraphSteps)
+constexpr int kRewarmSteps = 1024;  // graph-replayed steps the autotuner ends with (a multiple of kGraphSteps)
 constexpr int kGraphSteps = 16;  // steps per captured graph (even: the graph returns to buffer 0)
 constexpr int kXCD = 8;         // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int kTileInlets = 64;  // inlet boundaries one tile may touch when the conveyance is fused
@@ -97,14 +97,11 @@ int upload(hg_ctx* ctx, T* dst, const T* src, size_t n) {
 
 // --------------------------------------------------------------- kernels ---
 
-// flag[1] = 1 when any of a, b, c (n values each) is NaN or infinite.
-__global__ __launch_bounds__(kBlock) void k_nonfinite(int64_t n, const double* a, const double* b, const double* c,
-                                                      int32_t* flag) {
-  bool bad = false;
-  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
-    bad = bad || !isfinite(a[i]) || !isfinite(b[i]) || !isfinite(c[i]);
-  if (bad) flag[1] = 1;
+// flags[0..3] = 0.
+__global__ void k_clear_flags(int32_t* flags) {
+  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
 }
+
 
 // Inlet conveyance (bc_2D.jl:640-680): total_A_b = sum over the inlet's
 // faces, in NodeString order, of ((L^(5/3) h_c) / n_c) * wet_c.  The terms
@@ -112,17 +109,21 @@ __global__ __launch_bounds__(kBlock) void k_nonfinite(int64_t n, const double* a
 // reference's sequential order).
 __global__ __launch_bounds__(kBlock) void k_inlet_conveyance(hg::InletArgs a) {
   __shared__ double term[kBlock];
-  const int b = blockIdx.x;
-  const int f0 = a.off[b], f1 = a.off[b + 1];
+  const int b = blockIdx.x;              // inlet over all fields
+  const int fld = b / a.nin_m;           // its field
+  const int bl = b - fld * a.nin_m;      // the inlet within the mesh
+  const double* xi = a.xi + int64_t(fld) * a.Nm;
+  const double* nf = a.n + int64_t(fld) * a.Nm;
+  const int f0 = a.off[bl], f1 = a.off[bl + 1];
   double acc = 0.0;
   for (int base = f0; base < f1; base += kBlock) {
     const int f = base + threadIdx.x;
     if (f < f1) {
       const int c = a.cell[f];
-      double h = a.xi[c] + a.hstill[c];
+      double h = xi[c] + a.hstill[c];
       if (h <= a.hsmall) h = a.hsmall;
       const double w = (h > a.hsmall) ? 1.0 : 0.0;
-      term[threadIdx.x] = a.L53[f] * h / a.n[c] * w;
+      term[threadIdx.x] = a.L53[f] * h / nf[c] * w;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -148,9 +149,12 @@ __device__ __forceinline__ void store_state(double* p, double v) {
 
 // Fused RHS (+ Euler step and dry clamp when STEP).  One work-item per cell.
 template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
+__global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a0) {
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;  // over the K fields' cells
+  const int f = a0.K > 1 ? ig / a0.N : 0;
+  const int i = ig - f * a0.N;
+  if (f >= a0.K || i >= a0.N) return;
+  const StepArgs a = hg::field_view(a0, f);
   const double hsmall = a.hsmall;
   const double xi = a.xi[i], hs = a.hstill[i];
   const double qx0 = a.qx[i], qy0 = a.qy[i];
@@ -246,6 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
       qyn = 0.0;
     }
     if (a.clamp) a.clamp[i] = dry ? 1 : 0;
+    if (!(isfinite(xn) && isfinite(qxn) && isfinite(qyn))) a.flag[1] = 1;  // checked after the stepping call
     store_state(a.xo + i, xn);
     store_state(a.qxo + i, qxn);
     store_state(a.qyo + i, qyn);
@@ -270,24 +275,33 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a) {
 // T (cells per tile) and MD (slots per cell) are compile-time constants so
 // every LDS address is a base register plus an immediate offset.
 template <bool STEP, int T, int MD>
-__global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
+__global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a0) {
   extern __shared__ double lds[];
   // Work-groups are dealt round-robin to the 8 XCDs (block b -> XCD b % 8)
-  // in launch order.  With xcd_chunk = G > 0, each window of 8G consecutive
-  // Morton-ordered tiles is split so that XCD x takes G consecutive tiles of
-  // it (cross-tile neighbours then tend to share that XCD's L2) while the
-  // window of tiles in flight stays compact; a final partial window keeps
-  // the identity.  G = 0 is the identity mapping.  Bijection on [0, nt).
-  const int G = a.xcd_chunk;
-  int t = int(blockIdx.x);
-  if (G > 0) {
-    const int W = kXCD * G;
-    const int w0 = (t / W) * W;
-    if (w0 + W <= int(gridDim.x)) {
-      const int r = t - w0;
-      t = w0 + (r % kXCD) * G + r / kXCD;
-    }
+  // in launch order; the launch covers nt tiles of each of K fields.  Each
+  // window of 8 G' consecutive Morton-ordered tiles (G' = max(G, 1), G =
+  // xcd_chunk) and its K fields is split so that XCD x takes G' consecutive
+  // tiles of it for every field: cross-tile neighbours then tend to share
+  // that XCD's L2, and the K fields of a tile read its mesh data from that
+  // L2.  A final partial window takes its tiles in turn for each field.
+  // With K = 1, G = 0 is the identity.  Bijection on [0, K nt).
+  const int nt = int(gridDim.x) / a0.K;
+  const int Gc = a0.xcd_chunk > 0 ? a0.xcd_chunk : 1;
+  const int blk = int(blockIdx.x);
+  const int W = kXCD * Gc * a0.K;
+  const int w = blk / W, rr = blk - w * W;
+  const int tw = w * kXCD * Gc;
+  int t, fld;
+  if (tw + kXCD * Gc <= nt) {
+    const int q = rr / kXCD;
+    t = tw + (rr % kXCD) * Gc + q % Gc;
+    fld = q / Gc;
+  } else {
+    const int rem = nt - tw;
+    t = tw + rr % rem;
+    fld = rr / rem;
   }
+  const hg::TileArgs a = hg::field_view(a0, fld);
   const int li = threadIdx.x;
   const int i = t * T + li;
   const bool valid = i < a.N;
@@ -460,6 +474,7 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a) {
       qyn = 0.0;
     }
     if (a.clamp) a.clamp[i] = dry ? 1 : 0;
+    if (!(isfinite(xn) && isfinite(qxn) && isfinite(qyn))) a.flag[1] = 1;  // checked after the stepping call
     store_state(a.xo + i, xn);
     store_state(a.qxo + i, qxn);
     store_state(a.qyo + i, qyn);
@@ -498,20 +513,21 @@ __global__ __launch_bounds__(kBlock) void k_zone_gather(int N, const int32_t* zo
 }
 
 // CFL: per-block minimum of (2A/P)/(|u| + sqrt(g h)) over wet cells.
-__global__ __launch_bounds__(kBlock) void k_cfl(int N, const double* xi, const double* qx, const double* qy,
+__global__ __launch_bounds__(kBlock) void k_cfl(int N, int Nm, const double* xi, const double* qx, const double* qy,
                                                const double* hstill, const int32_t* deg, const double* flen,
                                                const double* area, double g, double hsmall, double* part) {
   __shared__ double red[kBlock];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over all fields; the mesh cell is i % Nm
   double v = INFINITY;
   if (i < N) {
-    const double h = xi[i] + hstill[i];
+    const int m = i % Nm;
+    const double h = xi[i] + hstill[m];
     if (h > hsmall) {
       double P = 0.0;
-      for (int k = 0; k < deg[i]; ++k) P = P + flen[k * N + i];
+      for (int k = 0; k < deg[m]; ++k) P = P + flen[k * Nm + m];
       const double u = qx[i] / h, w = qy[i] / h;
       const double spd = sqrt(u * u + w * w) + sqrt(g * h);
-      v = ((2.0 * area[i]) / P) / spd;
+      v = ((2.0 * area[m]) / P) / spd;
     }
   }
   red[threadIdx.x] = v;
@@ -569,13 +585,13 @@ __global__ __launch_bounds__(kBlock) void k_perm_out(int64_t N, int ncomp, const
 
 // Water-surface residuals at the probe cells (library order):
 // r_k = (xi + h_still) + zb - target_k (swe_2D_inversion.jl:387-399).
-__global__ __launch_bounds__(kBlock) void k_probe_resid(int n, const int32_t* cell, const double* target,
+__global__ __launch_bounds__(kBlock) void k_probe_resid(int n, int Nm, const int32_t* cell, const double* target,
                                                        const double* xi, const double* hstill, const double* zb,
                                                        double* r) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int c = cell[k];
-  r[k] = xi[c] + hstill[c] + zb[c] - target[k];
+  const int c = cell[k];  // over all fields; the mesh cell is c % Nm
+  r[k] = xi[c] + hstill[c % Nm] + zb[c % Nm] - target[k];
 }
 
 // lam[cell[k]] = val[k] over distinct cells.
@@ -735,28 +751,43 @@ template <bool STEP>
 __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, const CellAdj& ca, double xib,
                                                 double hb, double qxb, double qyb, double nb, double zbb);
 
-// Reverse mode of one RHS evaluation or one step, one work-item per cell:
-// the part of each internal face's VJP that falls on the cell (the flux
-// cotangent combines both cells' output cotangents, since the neighbour's
-// flux through the face is minus this one), accumulated in local face order
-// over the cell's raw and derived quantities and folded once; then the
-// source VJP, the clamp and the identity part of the step.  Boundary faces
-// are added by k_adj_bnd, which runs next: kept out of this kernel, their
-// ghost rules do not set its register budget.
+// Reverse mode of one RHS evaluation or one step, one work-item per cell,
+// over the cell's faces in local face order: of an internal face, the part
+// of its VJP that falls on the cell (the flux cotangent combines both
+// cells' output cotangents, since the neighbour's flux through the face is
+// minus this one), accumulated over the cell's raw and derived quantities
+// and folded once; of a boundary face, the VJP with the ghost's boundary
+// rule routed back to the cell.  Then the source VJP, the clamp and the
+// identity part of the step.
 template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.N) return;
+__global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a0) {
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;  // over the K fields' cells
+  const int fld = a0.K > 1 ? ig / a0.N : 0;
+  const int i = ig - fld * a0.N;
+  if (fld >= a0.K || i >= a0.N) return;
+  const hg::AdjArgs a = hg::field_view(a0, fld);
   const CellAdj ca = cell_adj<STEP>(a, i);
   hg::ExtBar e;
   hg::ext_zero(e);
+  double gxi = 0.0, gh = 0.0, gqx = 0.0, gqy = 0.0, nb = 0.0, zbb = 0.0;  // the boundary faces' sums
   const int d = a.deg[i];
   const double hsmall = a.hsmall;
   const double rA = 1.0 / ca.A;
   for (int k = 0; k < d; ++k) {  // local face order
     const int64_t s = int64_t(k) * a.N + i;
     const int j = a.nbr[s];
-    if (j < 0) continue;
+    if (j < 0) {
+      hg::SideBar U;
+      double fn, fz;
+      adj_ghost_face<STEP>(a, ca, s, U, fn, fz);
+      gxi += U.xi;
+      gh += U.h;
+      gqx += U.hu;
+      gqy += U.hv;
+      nb += fn;
+      zbb += fz;
+      continue;
+    }
     const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
     const hg::CellQ B = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], a.half_g, hsmall);
     double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
@@ -768,39 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a) {
     hg::roe_vjp_a(ca.me, B, nx, ny, a.g, a.half_g, hsmall, c, e);
   }
   const hg::SideBar U = hg::ext_to_side(e, ca.me, a.half_g);
-  adj_cell_finish<STEP>(a, i, ca, U.xi, U.h, U.hu, U.hv, 0.0, 0.0);
-}
-
-// The boundary faces' part of the reverse step, one work-item per cell that
-// has boundary faces (after k_adj_cell has written the cell's cotangents):
-// each ghost face's VJP with its boundary rule, summed in local face order,
-// then added through the same clamp as the cell's own sum.
-template <bool STEP>
-__global__ __launch_bounds__(kBlock) void k_adj_bnd(hg::AdjArgs a, int nbc, const int32_t* bc_off,
-                                                   const int32_t* bc_cell, const int32_t* bc_slot) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbc) return;
-  const int i = bc_cell[b];
-  const CellAdj ca = cell_adj<STEP>(a, i);
-  double xib = 0.0, hb = 0.0, qxb = 0.0, qyb = 0.0, nb = 0.0, zbb = 0.0;
-  for (int p = bc_off[b]; p < bc_off[b + 1]; ++p) {
-    hg::SideBar U;
-    double fn, fz;
-    adj_ghost_face<STEP>(a, ca, int64_t(bc_slot[p]), U, fn, fz);
-    xib += U.xi;
-    hb += U.h;
-    qxb += U.hu;
-    qyb += U.hv;
-    nb += fn;
-    zbb += fz;
-  }
-  a.ox[i] += ca.wet ? xib + hb : xib;
-  if (ca.wet) {
-    a.oqx[i] += qxb;
-    a.oqy[i] += qyb;
-  }
-  if (a.nbar) a.nbar[i] += nb;
-  if (a.zbbar) a.zbbar[i] += zbb;
+  adj_cell_finish<STEP>(a, i, ca, U.xi + gxi, U.h + gh, U.hu + gqx, U.hv + gqy, nb, zbb);
 }
 
 // The source VJP, the clamp and the identity part of the step, and the
@@ -853,13 +852,13 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
 // faces in NodeString order (deterministic; a cell may own several inlet
 // faces), adds d/dh of A_b to the state cotangent and accumulates d/dn and
 // d/dQ_b.
-__global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t* off, const int32_t* ghost,
-                                                     const int32_t* cell, const int32_t* next,
+__global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, int Nm, int Gm, const int32_t* off,
+                                                     const int32_t* ghost, const int32_t* cell, const int32_t* next,
                                                      const int32_t* first, const double* L53,
-                                                     const double* g_L23, const double* xi, const double* hstill,
-                                                     const double* n, const double* inletQ, const double* totalA,
-                                                     const double* vbar, double hsmall, double* ox, double* nbar,
-                                                     double* Qbar) {
+                                                     const double* g_L23, const double* xi_all, const double* hstill,
+                                                     const double* n_all, const double* inletQ, const double* totalA,
+                                                     const double* vbar_all, double hsmall, double* ox_all,
+                                                     double* nbar_all, double* Qbar) {
   // Per inlet: the per-face terms are formed in parallel and summed by a
   // fixed tree; each cell's faces of the inlet are applied, in NodeString
   // order, by the work-item of its first face (first[f] = 1; next[f]: the
@@ -870,10 +869,17 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t
   const int tid = threadIdx.x;
   const int b0 = gridDim.x > 1 ? int(blockIdx.x) : 0;
   const int b1 = gridDim.x > 1 ? int(blockIdx.x) + 1 : n_inlet;
-  for (int b = b0; b < b1; ++b) {
+  for (int b = b0; b < b1; ++b) {  // inlet over all fields: field fl, inlet bl within the mesh
+    const int fl = b / nin_m, bl = b - fl * nin_m;
+    const int64_t o = int64_t(fl) * Nm;
+    const double* xi = xi_all + o;
+    const double* n = n_all + o;
+    const double* vbar = vbar_all + int64_t(fl) * Gm;
+    double* ox = ox_all + o;
+    double* nbar = nbar_all + o;
     const double A = totalA[b];
     double pa = 0.0, pq = 0.0;
-    for (int f = off[b] + tid; f < off[b + 1]; f += kBlock) {
+    for (int f = off[bl] + tid; f < off[bl + 1]; f += kBlock) {
       const int gi = ghost[f], c = cell[f];
       const double vb = vbar[gi];
       const double v = inletQ[b] / A * g_L23[gi] / n[c];
@@ -892,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t
     }
     const double Ab = sA[0];
     if (tid == 0) Qbar[b] += sQ[0];
-    for (int f = off[b] + tid; f < off[b + 1]; f += kBlock) {
+    for (int f = off[bl] + tid; f < off[bl + 1]; f += kBlock) {
       if (!first[f]) continue;
       for (int g = f; g >= 0; g = next[g]) {
         const int c = cell[g];
@@ -910,11 +916,14 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, const int32_t
 // fvm_schemes_2D.jl:89-167): zbbar_m += sum over its faces of
 //   -(w L / A_m) (n . S0bar_m) + [internal] (L / (2 A_j)) (n . S0bar_j),
 // w = 1/2 on internal faces, 1 on boundary faces.
-__global__ __launch_bounds__(kBlock) void k_bed_adj(int N, const int32_t* deg, const int32_t* nbr, const double* fnx,
-                                                   const double* fny, const double* flen, const double* area,
-                                                   const double* S0bar, double* zbbar) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= N) return;
+__global__ __launch_bounds__(kBlock) void k_bed_adj(int Ntot, int N, const int32_t* deg, const int32_t* nbr,
+                                                   const double* fnx, const double* fny, const double* flen,
+                                                   const double* area, const double* S0bar_all, double* zbbar_all) {
+  const int mg = blockIdx.x * blockDim.x + threadIdx.x;  // over the fields' cells; N cells per field
+  if (mg >= Ntot) return;
+  const int fl = mg / N, m = mg - fl * N;
+  const double* S0bar = S0bar_all + 2 * int64_t(fl) * N;  // field fl: [x: N][y: N]
+  double* zbbar = zbbar_all + int64_t(fl) * N;
   const double sx = S0bar[m], sy = S0bar[N + m], Am = area[m];
   double acc = 0.0;
   for (int k = 0; k < deg[m]; ++k) {
@@ -1070,7 +1079,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
     int set = 0;
     if (valid) {
       hg::ude_features(a, i, h, u, v, um, wet);
-      zb = a.zb[i];
+      zb = a.zb[i % a.Nm];
       if (a.set_of_cell) set = a.set_of_cell[i];
     }
     if (l < CH) {  // F has CH rows
@@ -1123,7 +1132,7 @@ __global__ __launch_bounds__(64, sizeof(T) == 4 ? 3 : 1) void k_ude_vjp(hg::UdeA
   int set = 0;
   if (valid) {
     hg::ude_features(a, i, h, u, v, um, wet);
-    zb = a.zb[i];
+    zb = a.zb[i % a.Nm];
     if (a.set_of_cell) set = a.set_of_cell[i];
   }
   F[l * 4 + 0] = T(h);
@@ -1324,6 +1333,23 @@ __global__ __launch_bounds__(kBlock) void k_lux_fwd(hg::LuxArgs a) {
   a.out[i] = hg::lux_eval<MW>(a.net, a.w, f);
 }
 
+// Forward mode: the network's output and its tangent along the state
+// tangent (dxi, dqx, dqy) and the weight tangent dw (NULL: none), one cell
+// per work-item, on dual numbers (the value part is bitwise k_lux_fwd's).
+template <int MW>
+__global__ __launch_bounds__(kBlock) void k_lux_jvp(hg::LuxArgs a, const double* dxi, const double* dqx,
+                                                   const double* dqy, const double* dw, double* dout) {
+  using hg::Dual1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  Dual1 f[3];
+  hg::lux_features<Dual1>(a.net, Dual1(a.xi[i], dxi[i]), Dual1(a.qx[i], dqx[i]), Dual1(a.qy[i], dqy[i]),
+                          a.hstill[i], a.hsmall, hg::lux_ks(a, i), f);
+  const Dual1 o = hg::lux_eval<MW, Dual1>(a.net, a.w, f, dw);
+  a.out[i] = o.v;
+  dout[i] = o.d;
+}
+
 // Reverse mode: one wave per work-group, 64 cells at a time (grid-stride).
 // Weight cotangents: wave sums into LDS, then one slot per work-group in
 // wpart; the state cotangent adds through the features (one dual pass per
@@ -1470,6 +1496,7 @@ struct JvpArgs {
   const double* dg_zb;
   const double* dinletQ;
   const double* dtotalA;  // tangent of the inlet conveyances of this step
+  const double* dfmag;    // tangent of the FlowResistance friction magnitude (with a.fmag)
 };
 
 __device__ __forceinline__ hg::Dual1 ldd(const double* v, const double* d, int i) {
@@ -1577,7 +1604,10 @@ __global__ __launch_bounds__(kBlock) void k_swe_step_jvp(JvpArgs J) {
   Dual1 s1(0.0), s2(0.0);
   if (h > hsmall) {  // :463-475
     Dual1 fx, fy;
-    hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
+    if (a.fmag)
+      hg::friction_mag(ldd(a.fmag, J.dfmag, i), qx, qy, fx, fy);
+    else
+      hg::friction(a.g, a.kn2, hsmall, h, qx, qy, nman, fx, fy);
     s1 = a.g * xi * ldd(a.S0x, J.dS0x, i) - fx;
     s2 = a.g * xi * ldd(a.S0y, J.dS0y, i) - fy;
   }
@@ -1685,7 +1715,10 @@ size_t tile_lds_bytes(int T, int maxdeg) {
 hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt,
                               uint8_t* clamp = nullptr) {
   hg::TileArgs a{};
-  a.N = int32_t(ctx->N);
+  a.N = int32_t(ctx->Nm);
+  a.K = ctx->K;
+  a.nin_m = ctx->nin_m;
+  a.nex_m = ctx->nex_m;
   a.T = ctx->tileT;
   a.maxdeg = ctx->tileMD;
   a.n_inlet = ctx->fuse_inlet ? ctx->n_inlet : 0;
@@ -1739,7 +1772,11 @@ hg::TileArgs make_tile_args(hg_ctx* ctx, int src, int dst, double dt, double* o0
 StepArgs make_args_p(hg_ctx* ctx, const double* const in[3], double* const out[3], double dt,
                      uint8_t* clamp = nullptr) {
   StepArgs a{};
-  a.N = int32_t(ctx->N);
+  a.flag = ctx->flags;
+  a.N = int32_t(ctx->Nm);
+  a.K = ctx->K;
+  a.nin_m = ctx->nin_m;
+  a.nex_m = ctx->nex_m;
   a.maxdeg = ctx->maxdeg;
   a.xi = in[0];
   a.qx = in[1];
@@ -1786,6 +1823,8 @@ StepArgs make_args(hg_ctx* ctx, int src, int dst, double dt, double* out0 = null
 hg::InletArgs inlet_args_p(hg_ctx* ctx, const double* xi) {
   hg::InletArgs a{};
   a.n_inlet = ctx->n_inlet;
+  a.nin_m = ctx->nin_m;
+  a.Nm = int32_t(ctx->Nm);
   a.off = ctx->in_off;
   a.cell = ctx->in_cell;
   a.L53 = ctx->in_L53;
@@ -1801,6 +1840,8 @@ hg::InletArgs inlet_args_p(hg_ctx* ctx, const double* xi) {
 hg::InletArgs inlet_args(hg_ctx* ctx, int src) {
   hg::InletArgs a{};
   a.n_inlet = ctx->n_inlet;
+  a.nin_m = ctx->nin_m;
+  a.Nm = int32_t(ctx->Nm);
   a.off = ctx->in_off;
   a.cell = ctx->in_cell;
   a.L53 = ctx->in_L53;
@@ -1820,12 +1861,13 @@ int grid_of(int64_t n) { return int((n + kBlock - 1) / kBlock); }
 template <int T, int MD>
 void launch_tile_tm(hg_ctx* ctx, const hg::TileArgs& a, bool step, hipEvent_t e0, hipEvent_t e1) {
   const size_t lds = tile_lds_bytes(T, MD);
+  const dim3 grid(unsigned(ctx->ntiles) * unsigned(ctx->K));  // every tile of every field
   if (e0)  // start / stop events attached to the dispatch itself
-    hipExtLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, e0, e1, 0, a);
+    hipExtLaunchKernelGGL((k_swe_tile<true, T, MD>), grid, dim3(T), lds, ctx->stream, e0, e1, 0, a);
   else if (step)
-    hipLaunchKernelGGL((k_swe_tile<true, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
+    hipLaunchKernelGGL((k_swe_tile<true, T, MD>), grid, dim3(T), lds, ctx->stream, a);
   else
-    hipLaunchKernelGGL((k_swe_tile<false, T, MD>), dim3(ctx->ntiles), dim3(T), lds, ctx->stream, a);
+    hipLaunchKernelGGL((k_swe_tile<false, T, MD>), grid, dim3(T), lds, ctx->stream, a);
 }
 
 template <int T>
@@ -1852,6 +1894,7 @@ void launch_tile(hg_ctx* ctx, const hg::TileArgs& a, bool step, hipEvent_t e0 = 
 hg::UdeArgs ude_args(hg_ctx* ctx, const double* const q[3]) {
   hg::UdeArgs u{};
   u.N = int32_t(ctx->N);
+  u.Nm = int32_t(ctx->Nm);
   u.n_sets = ctx->ude_nsets;
   u.xi = q[0];
   u.qx = q[1];
@@ -2012,6 +2055,42 @@ int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3],
 // ---- adjoint host side
 int advance_async(hg_ctx* ctx, double dt, int64_t nsteps);
 int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms);
+int drop_graph(hg_ctx* ctx);
+
+// Launches the cached graph of `key` on `stream`, capturing it first from
+// enqueue() (which enqueues on ctx->stream, pointed at `stream` meanwhile;
+// other streams join through events).
+template <class F>
+int launch_seg_graph(hg_ctx* ctx, hipStream_t stream, const SegGraphKey& key, F&& enqueue) {
+  auto it = ctx->seg_graphs.find(key);
+  if (it == ctx->seg_graphs.end()) {
+    hipGraph_t graph = nullptr;
+    HG_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const hipStream_t saved = ctx->stream;
+    ctx->stream = stream;
+    int rc = enqueue();
+    ctx->stream = saved;
+    const hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (rc) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      ctx->err = std::string("hipStreamEndCapture (adjoint segment): ") + hipGetErrorString(e);
+      return HG_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      ctx->err = std::string("hipGraphInstantiate (adjoint segment): ") + hipGetErrorString(ei);
+      return HG_ERR_HIP;
+    }
+    it = ctx->seg_graphs.emplace(key, exec).first;
+  }
+  HG_TRY(hipGraphLaunch(it->second, stream));
+  return HG_OK;
+}
 
 int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
   const int64_t N = ctx->N;
@@ -2027,6 +2106,8 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (rc) return rc;
   }
   if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
+    rc = drop_graph(ctx);  // the segment graphs captured the old buffers
+    if (rc) return rc;
     for (int b = 0; b < 2 && !rc; ++b) {  // two segment buffers: one recomputed while the other is reversed
       rc = dalloc(ctx, &ctx->adj_seg[b], size_t(kck + 1) * 3 * N);
       if (!rc) rc = dalloc(ctx, &ctx->adj_mask[b], size_t(kck) * N);
@@ -2036,6 +2117,8 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     ctx->adj_seg_cap = size_t(kck);
   }
   if (ctx->ude_prec && size_t(kck) > ctx->adj_nseg_cap) {  // the closure's n of each step of a segment
+    rc = drop_graph(ctx);
+    if (rc) return rc;
     for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(ctx, &ctx->adj_nseg[b], size_t(kck) * N);
     if (rc) return rc;
     ctx->adj_nseg_cap = size_t(kck);
@@ -2056,7 +2139,11 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
 hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* const lam[3], const uint8_t* mask,
                           double* const out[3], double dt) {
   hg::AdjArgs a{};
-  a.N = int32_t(ctx->N);
+  a.N = int32_t(ctx->Nm);
+  a.K = ctx->K;
+  a.nin_m = ctx->nin_m;
+  a.nex_m = ctx->nex_m;
+  a.G_m = int32_t(ctx->Gm);
   a.maxdeg = ctx->maxdeg;
   a.xi = q[0]; a.qx = q[1]; a.qy = q[2];
   a.lx = lam[0]; a.lqx = lam[1]; a.lqy = lam[2];
@@ -2140,20 +2227,11 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
-  if (ctx->n_bcell > 0) {
-    if (step)
-      hipLaunchKernelGGL(k_adj_bnd<true>, dim3(grid_of(ctx->n_bcell)), dim3(kBlock), 0, ctx->stream, a,
-                         ctx->n_bcell, ctx->bc_off, ctx->bc_cell, ctx->bc_slot);
-    else
-      hipLaunchKernelGGL(k_adj_bnd<false>, dim3(grid_of(ctx->n_bcell)), dim3(kBlock), 0, ctx->stream, a,
-                         ctx->n_bcell, ctx->bc_off, ctx->bc_cell, ctx->bc_slot);
-  }
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_adj, dim3(ctx->inlets_disjoint ? ctx->n_inlet : 1), dim3(kBlock), 0, ctx->stream,
-                       ctx->n_inlet, ctx->in_off,
-                       ctx->in_ghost, ctx->in_cell, ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0],
-                       ctx->hstill, ctx->n, ctx->inletQ, a.totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar,
-                       ctx->adj_Qbar);
+                       ctx->n_inlet, ctx->nin_m, int(ctx->Nm), int(ctx->Gm), ctx->in_off, ctx->in_ghost, ctx->in_cell,
+                       ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
+                       a.totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar, ctx->adj_Qbar);
   if (ude) {  // through the closure: n(Q) adds to the state cotangent, and the weight cotangent accumulates
     hg::UdeArgs u = ude_args(ctx, q);
     u.nbar = ctx->ude_nbar;
@@ -2201,8 +2279,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
 // summed its share over the steps into its own slot; the slots are summed
 // here by a fixed tree.
 int finish_adjoint(hg_ctx* ctx) {
-  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->deg, ctx->nbr,
-                     ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
+  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), int(ctx->Nm),
+                     ctx->deg, ctx->nbr, ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
   if (ctx->ude_prec)
     hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW, ctx->ude_nsets), dim3(kBlock), 0, ctx->stream, ctx->ude_nslots,
                        ctx->ude_set_start, ctx->ude_wpart, int64_t(1), int64_t(hg::kUdeW), ctx->ude_wbar);
@@ -2275,16 +2353,21 @@ int recompute_segment(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int64
     if (hipMemcpyAsync(seg_state(ctx, 0, c, b), ctx->adj_ckpt + (sgi * 3 + c) * N, N * sizeof(double),
                        hipMemcpyDeviceToDevice, stream) != hipSuccess)
       rc = HG_ERR_HIP;
-  double* const n_field = ctx->n;
-  for (int64_t t = 0; t < ns && !rc; ++t) {
-    const double* in[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
-    double* out[3] = {seg_state(ctx, t + 1, 0, b), seg_state(ctx, t + 1, 1, b), seg_state(ctx, t + 1, 2, b)};
-    if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure writes this step's n where the reversal reads it
-    rc = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask[b] + t * N, seg_totalA(ctx, t, b));
-  }
-  ctx->n = n_field;
   ctx->stream = main;
-  return rc;
+  if (rc) return rc;
+  // the ns steps from the segment's first state: one cached graph per (b, ns, dt)
+  return launch_seg_graph(ctx, stream, SegGraphKey{0, b, 0, ns, dt}, [&] {
+    double* const n_field = ctx->n;
+    int r = HG_OK;
+    for (int64_t t = 0; t < ns && !r; ++t) {
+      const double* in[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
+      double* out[3] = {seg_state(ctx, t + 1, 0, b), seg_state(ctx, t + 1, 1, b), seg_state(ctx, t + 1, 2, b)};
+      if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure writes this step's n where the reversal reads it
+      r = enqueue_step_p(ctx, in, out, dt, true, ctx->adj_mask[b] + t * N, seg_totalA(ctx, t, b));
+    }
+    ctx->n = n_field;
+    return r;
+  });
 }
 
 // Reverse sweep over the checkpointed trajectory; lam (device, 3 x N) holds
@@ -2315,20 +2398,27 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
     }
     if (overlap) HG_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_seg[b], 0));
     const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
-    for (int64_t t = ns - 1; t >= 0; --t) {
-      const double* q[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
-      double* L = ctx->adj_lam[lam_cur];
-      double* O = ctx->adj_lam[1 - lam_cur];
-      const double* lam[3] = {L, L + N, L + 2 * N};
-      double* out[3] = {O, O + N, O + 2 * N};
-      double* const n_field = ctx->n;
-      if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure's n of this state, from the recompute
-      rc = enqueue_adjoint(ctx, q, lam, ctx->adj_mask[b] + t * N, out, dt, true, seg_totalA(ctx, t, b),
-                           ctx->ude_prec != 0);
-      ctx->n = n_field;
-      if (rc) return rc;
-      lam_cur = 1 - lam_cur;
-    }
+    // the segment's reversal: one cached graph per (b, starting cotangent buffer, ns, dt)
+    const int lam0 = lam_cur;
+    rc = launch_seg_graph(ctx, ctx->stream, SegGraphKey{1, b, lam0, ns, dt}, [&] {
+      int lc = lam0, r = HG_OK;
+      for (int64_t t = ns - 1; t >= 0 && !r; --t) {
+        const double* q[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
+        double* L = ctx->adj_lam[lc];
+        double* O = ctx->adj_lam[1 - lc];
+        const double* lam[3] = {L, L + N, L + 2 * N};
+        double* out[3] = {O, O + N, O + 2 * N};
+        double* const n_field = ctx->n;
+        if (ctx->ude_prec) ctx->n = seg_n(ctx, t, b);  // the closure's n of this state, from the recompute
+        r = enqueue_adjoint(ctx, q, lam, ctx->adj_mask[b] + t * N, out, dt, true, seg_totalA(ctx, t, b),
+                            ctx->ude_prec != 0);
+        ctx->n = n_field;
+        lc = 1 - lc;
+      }
+      return r;
+    });
+    if (rc) return rc;
+    lam_cur = (ns & 1) ? 1 - lam0 : lam0;
     if (overlap) {
       HG_TRY(hipEventRecord(ctx->ev_rev[b], ctx->stream));
     } else if (sgi > 0) {
@@ -2339,9 +2429,9 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
   return HG_OK;
 }
 
-int clear_flags(hg_ctx* ctx) {
-  const int32_t z[4] = {0, 0, 0, 0};
-  HG_TRY(hipMemcpyAsync(ctx->flags, z, sizeof(z), hipMemcpyHostToDevice, ctx->stream));
+int clear_flags(hg_ctx* ctx) {  // a fill kernel: no pageable host copy on the stream
+  hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->flags);
+  HG_TRY(hipGetLastError());
   return HG_OK;
 }
 
@@ -2361,13 +2451,6 @@ int check_flags(hg_ctx* ctx) {
 }
 
 // Finiteness check of the device state (one reduction, at the end of a call).
-int enqueue_finite_check(hg_ctx* ctx) {
-  const int grid = std::min(grid_of(ctx->N), 1024);
-  hipLaunchKernelGGL(k_nonfinite, dim3(grid), dim3(kBlock), 0, ctx->stream, int64_t(ctx->N), ctx->st[ctx->cur][0],
-                     ctx->st[ctx->cur][1], ctx->st[ctx->cur][2], ctx->flags);
-  HG_TRY(hipGetLastError());
-  return HG_OK;
-}
 
 // Host 3N state (caller order) -> device state `slot` (library order).
 // The staging buffer and device permutation of the transfers below
@@ -2422,17 +2505,30 @@ int cell_in(hg_ctx* ctx, double* dst, const double* src) {  // per-cell host arr
   return perm_in(ctx, src, 1, d);
 }
 
+// A per-cell array of the mesh (field 0's part of a caller array over all
+// fields) -> device, library order: the renumbering of field 0 is the mesh's.
+int mesh_cell_in(hg_ctx* ctx, double* dst, const double* src) {
+  if (ctx->K == 1) return cell_in(ctx, dst, src);
+  const int64_t Nm = ctx->Nm;
+  HG_TRY(hipMemcpyAsync(ctx->stage, src, size_t(Nm) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_perm_in, dim3(grid_of(Nm)), dim3(kBlock), 0, ctx->stream, Nm, 1, ctx->d_perm, ctx->stage, dst,
+                     dst, dst);
+  HG_TRY(hipGetLastError());
+  HG_TRY(hipStreamSynchronize(ctx->stream));
+  return HG_OK;
+}
+
 // Working copies of zb / n / S0 / zb_ghost / inletQ <- the values given to
 // hg_set_fields (p_extra is immutable in the reference; params override it
 // per call only).
 int restore_base(hg_ctx* ctx) {
-  const int64_t N = ctx->N;
-  HG_TRY(hipMemcpyAsync(ctx->zb, ctx->zb0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  HG_TRY(hipMemcpyAsync(ctx->S0x, ctx->S0x0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  HG_TRY(hipMemcpyAsync(ctx->S0y, ctx->S0y0, N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  if (ctx->G)
-    HG_TRY(hipMemcpyAsync(ctx->g_zb, ctx->g_zb0, ctx->G * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  const int64_t Nm = ctx->Nm;  // the mesh's bed data (shared by the fields); n per field
+  HG_TRY(hipMemcpyAsync(ctx->zb, ctx->zb0, Nm * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->S0x, ctx->S0x0, Nm * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HG_TRY(hipMemcpyAsync(ctx->S0y, ctx->S0y0, Nm * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->Gm)
+    HG_TRY(hipMemcpyAsync(ctx->g_zb, ctx->g_zb0, ctx->Gm * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   if (ctx->n_inlet)
     HG_TRY(hipMemcpyAsync(ctx->inletQ, ctx->inletQ0, ctx->n_inlet * sizeof(double), hipMemcpyDeviceToDevice,
                           ctx->stream));
@@ -2496,13 +2592,19 @@ int apply_params(hg_ctx* ctx, const double* params, int64_t np) {
 }
 
 int drop_graph(hg_ctx* ctx) {
+  if (ctx->gexec || !ctx->seg_graphs.empty()) {
+    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of them may still be running
+    if (ctx->stream2) HG_TRY(hipStreamSynchronize(ctx->stream2));
+  }
   if (ctx->gexec) {
-    HG_TRY(hipStreamSynchronize(ctx->stream));  // no replay of it may still be running
     HG_TRY(hipGraphExecDestroy(ctx->gexec));
     ctx->gexec = nullptr;
   }
+  for (auto& kv : ctx->seg_graphs) HG_TRY(hipGraphExecDestroy(kv.second));
+  ctx->seg_graphs.clear();
   return HG_OK;
 }
+
 
 // (Re)build the tiled kernel's work items for tiles of T cells from the
 // host copies of the face arrays.  Any T gives bit-identical results: every
@@ -2517,7 +2619,7 @@ int setup_tiles(hg_ctx* ctx, int T) {
                " bytes of LDS; the device allows " + std::to_string(ctx->lds_max);
     return HG_ERR_ARG;
   }
-  const int64_t N = ctx->N;
+  const int64_t N = ctx->Nm;  // tiles of the mesh; every field of it uses them
   const int32_t maxdeg = ctx->maxdeg;
   const std::vector<int32_t>& h_deg = ctx->h_deg;
   TileBuild tb = build_tiles(N, T, h_deg, ctx->h_nbr, ctx->h_nx, ctx->h_ny, ctx->h_len, ctx->h_gkind, ctx->h_gbc);
@@ -2631,6 +2733,13 @@ int require_fields(hg_ctx* ctx, const char* who) {
   return HG_ERR_STATE;
 }
 
+// Entry points a context of several fields on one mesh does not provide.
+int refuse_fields(hg_ctx* ctx, const char* who) {
+  if (ctx->K == 1) return HG_OK;
+  ctx->err = std::string(who) + ": not available for a context of several fields on one mesh";
+  return HG_ERR_STATE;
+}
+
 int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) {
   std::vector<hipEvent_t>& ev = ctx->ev_pool;  // grown once (at least 2 x 2048), reused by later calls
   int rc = HG_OK;
@@ -2683,7 +2792,7 @@ namespace {
 // Context creation for one mesh; seg lists the cell offsets of batched
 // instances ({0, N} for a single mesh), within which cells are renumbered.
 int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags,
-                std::vector<int64_t> seg) {
+                std::vector<int64_t> seg, int32_t K = 1) {
   if (!out || !m || !b || !c) {
     g_create_err = "hg_create: NULL argument";
     return HG_ERR_ARG;
@@ -2752,13 +2861,20 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   }
 
   hg_ctx* ctx = new hg_ctx();
-  ctx->N = N;
-  ctx->G = G;
+  // K fields of this mesh (1 for a plain or batched context): the counts
+  // below cover every field; the geometry is the mesh's (N cells, G ghosts)
+  ctx->K = K;
+  ctx->Nm = N;
+  ctx->Gm = G;
+  ctx->nin_m = b->n_inlet;
+  ctx->nex_m = b->n_exit;
+  ctx->N = int64_t(K) * N;
+  ctx->G = int64_t(K) * G;
   ctx->maxdeg = maxdeg;
   ctx->c = *c;
-  ctx->n_inlet = b->n_inlet;
-  ctx->n_exit = b->n_exit;
-  ctx->n_inlet_faces = b->n_inlet ? b->inlet_off[b->n_inlet] : 0;
+  ctx->n_inlet = K * b->n_inlet;
+  ctx->n_exit = K * b->n_exit;
+  ctx->n_inlet_faces = b->n_inlet ? b->inlet_off[b->n_inlet] : 0;  // the mesh's inlet faces
   auto fail = [&](int rc) {
     g_create_err = ctx->err;
     hg_destroy(ctx);
@@ -2835,14 +2951,16 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
     in_L53[f] = std::pow(m->face_len[b->ghost_slot[gi]], 5.0 / 3.0);
   }
 
-  // ---- device allocation
+  // ---- device allocation (state-like arrays for every field, mesh data once)
+  const int64_t NT = ctx->N;
   int rc = 0;
   for (int s = 0; s < 2 && !rc; ++s)
-    for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(ctx, &ctx->st[s][k], N);
-  double** cellbufs[] = {&ctx->hstill, &ctx->zb, &ctx->n, &ctx->S0x, &ctx->S0y, &ctx->area,
-                         &ctx->zb0, &ctx->n0, &ctx->S0x0, &ctx->S0y0};
+    for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(ctx, &ctx->st[s][k], NT);
+  double** cellbufs[] = {&ctx->hstill, &ctx->zb, &ctx->S0x, &ctx->S0y, &ctx->area, &ctx->zb0, &ctx->S0x0, &ctx->S0y0};
   for (double** p : cellbufs)
     if (!rc) rc = dalloc(ctx, p, N);
+  if (!rc) rc = dalloc(ctx, &ctx->n, NT);
+  if (!rc) rc = dalloc(ctx, &ctx->n0, NT);
   if (!rc) rc = dalloc(ctx, &ctx->deg, N);
   if (!rc) rc = dalloc(ctx, &ctx->nbr, E);
   if (!rc) rc = dalloc(ctx, &ctx->fnx, E);
@@ -2861,15 +2979,27 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = dalloc(ctx, &ctx->in_L53, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_next, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_first, ctx->n_inlet_faces);
-  if (!rc) rc = dalloc(ctx, &ctx->inletQ, b->n_inlet);
-  if (!rc) rc = dalloc(ctx, &ctx->inletQ0, b->n_inlet);
-  if (!rc) rc = dalloc(ctx, &ctx->exitWSE, b->n_exit);
-  if (!rc) rc = dalloc(ctx, &ctx->totalA, b->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->inletQ, ctx->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->inletQ0, ctx->n_inlet);
+  if (!rc) rc = dalloc(ctx, &ctx->exitWSE, ctx->n_exit);
+  if (!rc) rc = dalloc(ctx, &ctx->totalA, ctx->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->flags, 4);
   if (!rc) rc = dalloc(ctx, &ctx->zone, N);
-  ctx->scratch_n = size_t(std::max<int64_t>(3 * N, 4096));
+  ctx->scratch_n = size_t(std::max<int64_t>(3 * NT, 4096));
   if (!rc) rc = dalloc(ctx, &ctx->scratch, ctx->scratch_n);
-  if (!rc) rc = dalloc(ctx, &ctx->red, size_t(grid_of(N) + 64));
+  if (!rc) rc = dalloc(ctx, &ctx->red, size_t(grid_of(NT) + 64));
+  if (!rc && K > 1) {  // caller order <-> library order of every field: the mesh's renumbering, per field
+    std::vector<int32_t> p(NT), ip(NT);
+    for (int64_t f = 0; f < K; ++f)
+      for (int64_t i = 0; i < N; ++i) {
+        p[f * N + i] = int32_t(f * N + ctx->perm[i]);
+        ip[f * N + i] = int32_t(f * N + ctx->iperm[i]);
+      }
+    ctx->perm.swap(p);
+    ctx->iperm.swap(ip);
+    ctx->inst_off.resize(K + 1);
+    for (int64_t f = 0; f <= K; ++f) ctx->inst_off[f] = f * N;
+  }
   if (!rc) rc = ensure_xfer(ctx);  // host transfers never allocate after set-up
   if (rc) return fail(rc);
 
@@ -2912,31 +3042,6 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
         if (it->second != bi) ctx->inlets_disjoint = false;
       }
   }
-  if (!rc) {  // boundary cells and their ghost slots (k_adj_bnd)
-    std::vector<int32_t> off(1, 0), cell, slot;
-    for (int64_t li = 0; li < N; ++li) {
-      const size_t before = slot.size();
-      for (int32_t k = 0; k < h_deg[li]; ++k)
-        if (h_nbr[k * N + li] < 0) {
-          if (k * N + li > int64_t(INT32_MAX)) {
-            g_create_err = "hg_create: too many face slots for 32-bit slot indices";
-            return fail(HG_ERR_ARG);
-          }
-          slot.push_back(int32_t(k * N + li));
-        }
-      if (slot.size() > before) {
-        cell.push_back(int32_t(li));
-        off.push_back(int32_t(slot.size()));
-      }
-    }
-    ctx->n_bcell = int32_t(cell.size());
-    rc = dalloc(ctx, &ctx->bc_off, off.size());
-    if (!rc) rc = dalloc(ctx, &ctx->bc_cell, std::max<size_t>(cell.size(), 1));
-    if (!rc) rc = dalloc(ctx, &ctx->bc_slot, std::max<size_t>(slot.size(), 1));
-    if (!rc) rc = upload(ctx, ctx->bc_off, off.data(), off.size());
-    if (!rc) rc = upload(ctx, ctx->bc_cell, cell.data(), cell.size());
-    if (!rc) rc = upload(ctx, ctx->bc_slot, slot.data(), slot.size());
-  }
   if (!rc) rc = clear_flags(ctx);
   if (rc) return fail(rc);
 
@@ -2977,6 +3082,19 @@ extern "C" {
 
 int hg_create(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts* c, int32_t flags) {
   return create_impl(out, m, b, c, flags, {});
+}
+
+int hg_fields_create(hg_ctx** out, int32_t n_fields, const hg_mesh* m, const hg_bcs* b, const hg_consts* c,
+                     int32_t flags) {
+  if (!out || !m || n_fields < 1) {
+    g_create_err = "hg_fields_create: NULL argument or no fields";
+    return HG_ERR_ARG;
+  }
+  if (int64_t(n_fields) * m->n_cells > (int64_t(1) << 30)) {
+    g_create_err = "hg_fields_create: more than 2^30 cells over all fields";
+    return HG_ERR_ARG;
+  }
+  return create_impl(out, m, b, c, flags, {}, n_fields);
 }
 
 int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const hg_bcs* bcs, const hg_consts* c,
@@ -3085,7 +3203,9 @@ int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const h
 int hg_destroy(hg_ctx* ctx) {
   if (!ctx) return HG_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  for (auto& kv : ctx->seg_graphs) (void)hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd})
     if (e) (void)hipEventDestroy(e);
@@ -3108,14 +3228,30 @@ int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost,
     ctx->err = "hg_set_fields: NULL array";
     return HG_ERR_ARG;
   }
+  if (ctx->K > 1) {  // fields of one mesh: its per-cell constants are stored once
+    const int64_t Nm = ctx->Nm, Gm = ctx->Gm;
+    for (int64_t f = 1; f < ctx->K; ++f) {
+      const bool same = !std::memcmp(hstill + f * Nm, hstill, Nm * sizeof(double)) &&
+                        !std::memcmp(zb + f * Nm, zb, Nm * sizeof(double)) &&
+                        !std::memcmp(S0x + f * Nm, S0x, Nm * sizeof(double)) &&
+                        !std::memcmp(S0y + f * Nm, S0y, Nm * sizeof(double)) &&
+                        (!Gm || (!std::memcmp(hstill_ghost + f * Gm, hstill_ghost, Gm * sizeof(double)) &&
+                                 !std::memcmp(zb_ghost + f * Gm, zb_ghost, Gm * sizeof(double))));
+      if (!same) {
+        ctx->err = "hg_set_fields: the fields of one mesh must share h_still, zb and S0 (field " + std::to_string(f) +
+                   " differs from field 0)";
+        return HG_ERR_ARG;
+      }
+    }
+  }
   int rc = 0;
-  if (!rc) rc = cell_in(ctx, ctx->hstill, hstill);
-  if (!rc) rc = cell_in(ctx, ctx->zb0, zb);
-  if (!rc) rc = cell_in(ctx, ctx->S0x0, S0x);
-  if (!rc) rc = cell_in(ctx, ctx->S0y0, S0y);
+  if (!rc) rc = mesh_cell_in(ctx, ctx->hstill, hstill);
+  if (!rc) rc = mesh_cell_in(ctx, ctx->zb0, zb);
+  if (!rc) rc = mesh_cell_in(ctx, ctx->S0x0, S0x);
+  if (!rc) rc = mesh_cell_in(ctx, ctx->S0y0, S0y);
   if (!rc) rc = cell_in(ctx, ctx->n0, n);
-  if (!rc) rc = upload(ctx, ctx->g_hstill, hstill_ghost, ctx->G);
-  if (!rc) rc = upload(ctx, ctx->g_zb0, zb_ghost, ctx->G);
+  if (!rc) rc = upload(ctx, ctx->g_hstill, hstill_ghost, ctx->Gm);
+  if (!rc) rc = upload(ctx, ctx->g_zb0, zb_ghost, ctx->Gm);
   if (!rc) rc = upload(ctx, ctx->inletQ0, inletQ, ctx->n_inlet);
   if (!rc) rc = upload(ctx, ctx->exitWSE, exitWSE, ctx->n_exit);
   if (rc) return rc;
@@ -3125,6 +3261,9 @@ int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost,
 
 int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, int32_t n_zones) {
   if (!ctx) return HG_ERR_ARG;
+  if ((kind == HG_PARAM_ZB || kind == HG_PARAM_N_ZONE || kind == HG_PARAM_UDE) &&
+      refuse_fields(ctx, "hg_set_active_param (zb, zone n or network weights)"))
+    return HG_ERR_STATE;
   if (kind < HG_PARAM_NONE || kind > HG_PARAM_UDE) {
     ctx->err = "unknown active parameter kind";
     return HG_ERR_ARG;
@@ -3213,7 +3352,6 @@ int hg_advance(hg_ctx* ctx, double dt, int64_t nsteps) {
   }
   int rc = clear_flags(ctx);
   if (!rc) rc = advance_async(ctx, dt, nsteps);
-  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
 }
@@ -3230,7 +3368,6 @@ int hg_advance_region(hg_ctx* ctx, double dt, int64_t nsteps, double* gpu_ms_per
   if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
   if (!rc) rc = advance_async(ctx, dt, nsteps);
   if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = HG_ERR_HIP;
-  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);  // synchronizes the stream
   float ms = 0.f;
   if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = HG_ERR_HIP;
@@ -3257,7 +3394,6 @@ int hg_advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms) 
   }
   int rc = clear_flags(ctx);
   if (!rc) rc = advance_timed(ctx, dt, nsteps, kernel_ms);
-  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
 }
@@ -3281,7 +3417,7 @@ int hg_cfl_dt(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
   if (rc) return rc;
   ctx->cur = 0;
   const int nb = grid_of(ctx->N);
-  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(ctx->N), ctx->st[0][0], ctx->st[0][1],
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(ctx->N), int(ctx->Nm), ctx->st[0][0], ctx->st[0][1],
                      ctx->st[0][2], ctx->hstill, ctx->deg, ctx->flen, ctx->area, ctx->c.g, ctx->c.h_small, ctx->red);
   HG_TRY(hipGetLastError());
   std::vector<double> part(nb);
@@ -3316,7 +3452,7 @@ int hg_cfl_dt_noalloc(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
   hipLaunchKernelGGL(k_perm_in, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, 3, ctx->d_perm, ctx->stage,
                      ctx->st[0][0], ctx->st[0][1], ctx->st[0][2]);
   ctx->cur = 0;
-  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(N), ctx->st[0][0], ctx->st[0][1],
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(kBlock), 0, ctx->stream, int(N), int(ctx->Nm), ctx->st[0][0], ctx->st[0][1],
                      ctx->st[0][2], ctx->hstill, ctx->deg, ctx->flen, ctx->area, ctx->c.g, ctx->c.h_small, sc);
   hipLaunchKernelGGL(k_min_parts, dim3(1), dim3(kBlock), 0, ctx->stream, nb, sc, cfl, sc + kNoAllocParts);
   HG_TRY(hipGetLastError());
@@ -3467,6 +3603,7 @@ int hg_ude_grad(hg_ctx* ctx, double* w_bar) {
 
 int hg_set_n_law(hg_ctx* ctx, int32_t kind, const double* params, const double* ks) {
   if (!ctx) return HG_ERR_ARG;
+  if (kind != HG_NLAW_OFF && refuse_fields(ctx, "hg_set_n_law")) return HG_ERR_STATE;
   if (kind < HG_NLAW_OFF || kind > HG_NLAW_H_UMAG_KS) {
     ctx->err = "hg_set_n_law: unknown law";
     return HG_ERR_ARG;
@@ -3637,6 +3774,7 @@ int64_t hg_lux_n_params(const hg_lux_net* net) {
 
 int hg_set_lux_ude(hg_ctx* ctx, const hg_lux_net* net, const double* params, const double* ks) {
   if (!ctx) return HG_ERR_ARG;
+  if (net && refuse_fields(ctx, "hg_set_lux_ude")) return HG_ERR_STATE;
   HG_TRY(hipStreamSynchronize(ctx->stream));
   if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the network
   for (void* p : ctx->lux_allocs) {
@@ -3774,16 +3912,13 @@ int hg_mfma_probe(int32_t precision, const double* A, const double* B, double* D
 int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* params, int64_t n_params,
            const double* dparams, double dt, int64_t nsteps, double* QN, double* dQN) {
   if (!ctx || !Q0 || !QN || !dQN) return ctx ? (ctx->err = "hg_jvp: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (refuse_fields(ctx, "hg_jvp")) return HG_ERR_STATE;
   if (nsteps < 0) {
     ctx->err = "hg_jvp: nsteps must be >= 0";
     return HG_ERR_ARG;
   }
   if (ctx->ude_prec) {
     ctx->err = "hg_jvp: forward mode through the UDE closure is not provided";
-    return HG_ERR_STATE;
-  }
-  if (ctx->lux.choice) {
-    ctx->err = "hg_jvp: forward mode through the UDE network is not provided";
     return HG_ERR_STATE;
   }
   const int64_t N = ctx->N, G = ctx->G;
@@ -3848,12 +3983,22 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
       rc = HG_ERR_HIP;
     }
   }
+  // the network's weight tangent, when its weights are the active parameter
+  const bool lux = ctx->lux.choice != hg::LUX_OFF;
+  const bool lux_fr = ctx->lux.choice == hg::LUX_FLOW_RESISTANCE;
+  const double* pw = nullptr;
+  if (!rc && lux && dparams && ctx->active == HG_PARAM_UDE) {
+    if (!ctx->lux_dw) rc = dalloc(ctx, &ctx->lux_dw, size_t(hg::kLuxMaxP));
+    if (!rc) rc = upload(ctx, ctx->lux_dw, dparams, size_t(ctx->lux.P));
+    pw = ctx->lux_dw;
+  }
   if (!rc) rc = clear_flags(ctx);
   if (rc) return rc;
   int cur = 0;
   double* Tb[2] = {T0, T1};
   const bool law = ctx->nlaw.kind != hg::NLAW_OFF;
-  if (law) pn = dn;  // the law's tangent of n, rewritten every step
+  if (law || (lux && !lux_fr)) pn = dn;  // the law's or network's tangent of n, rewritten every step
+  double* dfm = lux_fr ? dn : nullptr;     // the network's tangent of the friction magnitude
   for (int64_t t = 0; t < nsteps; ++t) {
     const double* in[3] = {ctx->st[cur][0], ctx->st[cur][1], ctx->st[cur][2]};
     double* out[3] = {ctx->st[1 - cur][0], ctx->st[1 - cur][1], ctx->st[1 - cur][2]};
@@ -3864,6 +4009,15 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
       u.dqy = Tb[cur] + 2 * N;
       u.dn = dn;
       hipLaunchKernelGGL(k_nlaw_jvp, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, u);
+    }
+    if (lux) {  // n (or the friction magnitude) and its tangent from the network on dual numbers
+      const hg::LuxArgs u = lux_args(ctx, in);
+      if (ctx->lux_mw == 4)
+        hipLaunchKernelGGL(k_lux_jvp<4>, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, u, Tb[cur], Tb[cur] + N,
+                           Tb[cur] + 2 * N, pw, dn);
+      else
+        hipLaunchKernelGGL(k_lux_jvp<hg::kLuxMaxW>, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, u, Tb[cur],
+                           Tb[cur] + N, Tb[cur] + 2 * N, pw, dn);
     }
     if (ctx->n_inlet > 0)
       hipLaunchKernelGGL(k_inlet_conveyance_jvp, dim3((ctx->n_inlet + 63) / 64), dim3(64), 0, ctx->stream,
@@ -3883,6 +4037,7 @@ int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* param
     J.dg_zb = pzb ? dgzb : nullptr;
     J.dinletQ = pQ;
     J.dtotalA = dtot;
+    J.dfmag = dfm;
     hipLaunchKernelGGL(k_swe_step_jvp, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, J);
     HG_TRY(hipGetLastError());
     cur = 1 - cur;
@@ -3909,6 +4064,7 @@ int hg_tsit5_tableau(double* c, double* a, double* btilde) {
 int hg_tsit5_solve(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double t0, double t1,
                    int64_t n_save, double abstol, double reltol, double dt0, double* Qsave, int64_t* stats) {
   if (!ctx || !Q0 || !Qsave) return ctx ? (ctx->err = "hg_tsit5_solve: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (refuse_fields(ctx, "hg_tsit5_solve")) return HG_ERR_STATE;
   if (!(t1 > t0) || n_save <= 0 || !(abstol > 0.0) || !(reltol >= 0.0) || !(dt0 > 0.0)) {
     ctx->err = "hg_tsit5_solve: need t1 > t0, n_save > 0, abstol > 0, reltol >= 0, dt0 > 0";
     return HG_ERR_ARG;
@@ -4087,13 +4243,13 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
   if (!rc && ms_step_kernel) rc = advance_timed(ctx, dt, 200, ms_step_kernel);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (!rc) rc = enqueue_finite_check(ctx);
   if (!rc) rc = check_flags(ctx);
   return rc;
 }
 
 int hg_batch_set_dt(hg_ctx* ctx, const double* dt_inst) {
   if (!ctx) return HG_ERR_ARG;
+  if (dt_inst && refuse_fields(ctx, "hg_batch_set_dt")) return HG_ERR_STATE;
   HG_TRY(hipStreamSynchronize(ctx->stream));
   if (drop_graph(ctx)) return HG_ERR_HIP;  // captured launches hold the previous step pointer
   if (!dt_inst) {
@@ -4341,7 +4497,8 @@ int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
     if (!rc) rc = upload(ctx, ctx->probe_d, target, size_t(n_probe));
     if (rc) return rc;
     hipLaunchKernelGGL(k_probe_resid, dim3(grid_of(n_probe)), dim3(kBlock), 0, ctx->stream, int(n_probe),
-                       ctx->probe_i, ctx->probe_d, ctx->st[ctx->cur][0], ctx->hstill, ctx->zb, ctx->probe_d + cap);
+                       int(ctx->Nm), ctx->probe_i, ctx->probe_d, ctx->st[ctx->cur][0], ctx->hstill, ctx->zb,
+                       ctx->probe_d + cap);
     HG_TRY(hipGetLastError());
     HG_TRY(hipMemcpyAsync(r.data(), ctx->probe_d + cap, n_probe * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));

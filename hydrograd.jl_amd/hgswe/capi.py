@@ -38,7 +38,7 @@ EXPORTS = [
     "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
     "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
-    "hg_advance_timed", "hg_advance_region", "hg_prepare_advance", "hg_cfl_dt_noalloc",
+    "hg_advance_timed", "hg_advance_region", "hg_prepare_advance", "hg_cfl_dt_noalloc", "hg_fields_create",
 ]
 UDE_WEIGHTS = 353
 
@@ -100,6 +100,8 @@ def lib():
         L.hg_info.restype = C.c_char_p
         L.hg_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(hg_mesh), C.POINTER(hg_bcs), C.POINTER(hg_consts),
                                 C.c_int32]
+        L.hg_fields_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(hg_mesh), C.POINTER(hg_bcs),
+                                       C.POINTER(hg_consts), C.c_int32]
         L.hg_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(hg_mesh), C.POINTER(hg_bcs),
                                       C.POINTER(hg_consts), C.c_int32]
         L.hg_set_tile_config.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
@@ -161,10 +163,13 @@ class Context:
 
     A batch (``cases.concat``) of instances is created with
     hg_batch_create from the instances' own meshes; every other call then
-    takes the batch's concatenated arrays.  With fields=False the case's
+    takes the batch's concatenated arrays.  With share_mesh=True, a batch
+    whose instances are all the same mesh is created with hg_fields_create
+    instead (the mesh stored once, every field advanced in each launch); the
+    calls and their arrays are the same.  With fields=False the case's
     fields are not uploaded (set_fields must come before any stepping)."""
 
-    def __init__(self, case, reorder=True, cell_kernel=False, fields=True):
+    def __init__(self, case, reorder=True, cell_kernel=False, fields=True, share_mesh=False):
         L = lib()
         self.case = case
         self.N = case.mesh.n_cells
@@ -189,7 +194,13 @@ class Context:
         h = C.c_void_p()
         flags = (HG_REORDER if reorder else 0) | (HG_CELL_KERNEL if cell_kernel else 0)
         inst = case.extra.get("instances") if case.extra else None
-        if inst:
+        if inst and share_mesh:
+            m0 = inst[0].mesh
+            if any(c.mesh is not m0 for c in inst):
+                raise ValueError("share_mesh needs a batch of one mesh (the same Mesh2D object in every instance)")
+            mesh, bcs = structs(m0)
+            rc = L.hg_fields_create(C.byref(h), len(inst), C.byref(mesh), C.byref(bcs), C.byref(consts), flags)
+        elif inst:
             pairs = [structs(c.mesh) for c in inst]
             meshes = (hg_mesh * len(inst))(*[p[0] for p in pairs])
             bcs = (hg_bcs * len(inst))(*[p[1] for p in pairs])

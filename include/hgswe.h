@@ -116,6 +116,26 @@ int hg_destroy(hg_ctx* ctx);
 int hg_batch_create(hg_ctx** out, int32_t n_inst, const hg_mesh* meshes, const hg_bcs* bcs, const hg_consts* consts,
                     int32_t flags);
 
+/* K parameter fields on one mesh, advanced together (no reference
+ * counterpart: the reference runs one parameter field per solve; this is
+ * its "many parameter fields" use, e.g. an ensemble of Manning-n fields or
+ * of UDE weight sets).  The context behaves exactly like hg_batch_create of
+ * n_fields copies of `mesh` -- every other entry point takes and returns the
+ * concatenated arrays of the K instances (states [xi; q_x; q_y] over K N
+ * cells, per-cell fields and params, probes, inlets and exits of field f
+ * after those of field f-1) -- but stores the mesh geometry and its per-cell
+ * constants (h_still, zb, S0 and the ghost tables) once, and each step
+ * launch covers every field with the work-groups of one tile's K fields on
+ * one XCD, so that they read the tile's mesh data from that XCD's L2.  Per
+ * field: the state, Manning n, inlet discharges, exit levels and the UDE
+ * closure's weight set (hg_set_ude's set_of_cell).  hg_set_fields requires
+ * h_still, zb, S0 and the ghost h_still / zb to be the same in every field
+ * (HG_ERR_ARG otherwise).  Not available (HG_ERR_STATE): zb, zone n or the
+ * network weights as the active parameter, hg_set_n_law, hg_set_lux_ude,
+ * hg_jvp, hg_tsit5_solve, hg_batch_set_dt. */
+int hg_fields_create(hg_ctx** out, int32_t n_fields, const hg_mesh* mesh, const hg_bcs* bcs, const hg_consts* consts,
+                     int32_t flags);
+
 /* Per-instance time steps for a batch (n_inst values, or NULL to return to
  * the scalar dt).  While set, every stepping call (hg_step, hg_advance,
  * hg_adjoint, hg_grad_probes, hg_jvp) advances instance r by dt_inst[r] and
@@ -278,7 +298,13 @@ int hg_tsit5_tableau(double* c, double* a, double* btilde);
  * active parameter (dparams, n_params, NULL = 0): QN and its tangent dQN.
  * The device counterpart of ForwardDiff.Dual through the step
  * (swe_2D_sensitivity.jl:33-80, ForwardDiff.jacobian column by column);
- * branch tests follow the primal values, clamps have zero tangent. */
+ * branch tests follow the primal values, clamps have zero tangent.  With
+ * the reference's UDE network on (hg_set_lux_ude), the network runs on dual
+ * numbers every step: its output's tangent follows the state tangent and,
+ * with HG_PARAM_UDE active, the weight tangent dparams (the reference's
+ * AutoForwardDiff / ForwardDiffSensitivity UDE training, swe_2D_UDE.jl:
+ * 156-157).  The MLP closure of hg_set_ude has no forward mode
+ * (HG_ERR_STATE). */
 int hg_jvp(hg_ctx* ctx, const double* Q0, const double* dQ0, const double* params, int64_t n_params,
            const double* dparams, double dt, int64_t nsteps, double* QN, double* dQN);
 

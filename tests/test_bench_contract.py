@@ -68,3 +68,14 @@ def test_gradient_bytes_per_horizon_cell_step():
     assert b.bytes_per_gradient_cell_step(tri.mesh, 50) == 180 + 180 + 220 + 24 / 50
     r = b.gradient_roofline(tri.mesh, 1_000_000, 1000, 50, 0.1)
     assert abs(r["achieved"] - 580.48 * 1e9 / 0.1 / 1e9) < 1e-6 and r["frac"] == r["achieved"] / 8000.0
+
+
+def test_shared_mesh_bytes_per_cell_step():
+    """K fields of one mesh: each field's state (48 B) and n (8 B), the mesh
+    data (h_still, zb, S0x, S0y, area: 40 B; faces 28 B each) once for all."""
+    from hgswe import cases
+
+    b = bench_module()
+    tri = cases.channel(2_000, seed=1, Lx=400.0, Ly=40.0, nsteps=1)
+    assert b.bytes_per_cell_step(tri.mesh, 1) == 180
+    assert b.bytes_per_cell_step(tri.mesh, 32) == 56 + (40 + 84) / 32

@@ -13,6 +13,11 @@ the three UDE choices (ManningN_h, ManningN_h_Umag_ks, FlowResistance).
   probe loss against central differences of device runs.
 * The weights as the active parameter (HG_PARAM_UDE, the reference's
   params_vector) and the exclusions.
+* Forward mode through the network (hg_jvp, the reference's AutoForwardDiff
+  UDE path): the primal run is bitwise the stepping run; along the weights
+  and along the initial state, <ybar, J v> equals <J^T ybar, v> from the
+  reverse sweep (hg_adjoint with the weight cotangent); along the weights,
+  against fourth-order central differences of device runs.
 """
 import sys
 
@@ -215,11 +220,47 @@ def test_network_exclusions():
         ctx.set_active_param(capi.HG_PARAM_N_CELL)
     with pytest.raises(capi.HGError):
         ctx.set_n_law("inverse", n_lower=0.02, n_upper=0.05, k=1.0)
-    with pytest.raises(capi.HGError):
-        ctx.jvp(Q0, c.dt, 2)
     ctx.set_lux_ude(None)
     assert np.array_equal(ctx.step(Q0, c.dt, 2), capi.Context(c).step(Q0, c.dt, 2))
     with pytest.raises(capi.HGError):
         ctx.set_active_param(capi.HG_PARAM_UDE)
     bad = lx.Net(lx.MANNING_H, [17], [lx.TANH], (0.02, 0.06), (1.0, 4.0))
     assert capi.lux_n_params(device_net(bad)) == -1
+
+
+@pytest.mark.parametrize("name", ["h", "h_umag_ks", "flow_resistance"])
+def test_forward_mode_through_the_network(name):
+    c, Q0, ks = flowing_case()
+    net = NETS[name]
+    P = lx.random_params(net, seed=21)
+    ctx = make(c, net, P, ks)
+    ctx.set_active_param(capi.HG_PARAM_UDE)
+    nsteps, kck = 12, 5
+    rng = np.random.default_rng(22)
+    dW = rng.normal(size=P.size)
+    QN, dQN = ctx.jvp(Q0, c.dt, nsteps, params=P, dparams=dW)
+    assert np.array_equal(QN, ctx.step(Q0, c.dt, nsteps, params=P))
+    ybar = rng.normal(size=3 * c.N)
+    Q0b, _, _, _ = ctx.adjoint(Q0, c.dt, nsteps, kck, ybar, params=P)
+    gW = ctx.lux_grad()
+    lhs, rhs = float(np.dot(ybar, dQN)), float(np.dot(gW, dW))
+    scale = float(np.sum(np.abs(ybar * dQN)))
+    say(f"  {name} JVP along the weights: <y, Jv> {lhs:.12e}, <J^T y, v> {rhs:.12e}")
+    assert abs(lhs - rhs) <= 1e-10 * scale
+    # along the initial state, the weights held
+    dQ0 = rng.normal(size=3 * c.N) * 1e-3
+    _, dQN2 = ctx.jvp(Q0, c.dt, nsteps, dQ0=dQ0, params=P)
+    lhs2, rhs2 = float(np.dot(ybar, dQN2)), float(np.dot(Q0b, dQ0))
+    scale2 = float(np.sum(np.abs(ybar * dQN2)))
+    say(f"  {name} JVP along Q0: <y, Jv> {lhs2:.12e}, <J^T y, v> {rhs2:.12e}")
+    assert abs(lhs2 - rhs2) <= 1e-10 * scale2
+    # fourth-order central differences of device runs along the weights
+    e = 1e-4
+
+    def f(w):
+        return float(np.dot(ybar, ctx.step(Q0, c.dt, nsteps, params=w)))
+
+    fd = (8.0 * (f(P + e * dW) - f(P - e * dW)) - (f(P + 2 * e * dW) - f(P - 2 * e * dW))) / (12.0 * e)
+    say(f"  {name} JVP along the weights vs differences: {lhs:.10e} / {fd:.10e}")
+    assert abs(fd - lhs) <= 1e-6 * abs(lhs) + 1e-12
+
