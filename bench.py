@@ -168,6 +168,7 @@ def measure_gradient(args, capi):
                       "dt": c.dt},
            "roofline": gradient_roofline(c.mesh, c.N, args.grad_steps, args.grad_kck, el),
            "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
+    out["stored_trajectory"] = stored_gradient(ctx, c, args.grad_steps, probe, target)
     if not args.no_cpu:
         cb = cpu_gradient_baseline(c, probe, target, args.cpu_seconds)
         out["cpu_baseline"] = cb
@@ -197,12 +198,32 @@ def measure_gradient_full(args, capi, case, tile):
     t0 = time.perf_counter()
     loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, args.grad_kck, probe, target, params=c.n)
     el = time.perf_counter() - t0
+    out = {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
+           "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": steps,
+                      "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "per-cell Manning n",
+                      "dt": c.dt},
+           "roofline": gradient_roofline(c.mesh, c.N, steps, args.grad_kck, el),
+           "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
+    out["stored_trajectory"] = stored_gradient(ctx, c, steps, probe, target)
+    return out
+
+
+def stored_gradient(ctx, c, steps, probe, target):
+    """The same gradient with one segment covering the run (checkpoint
+    interval = steps): the forward sweep keeps every state, nothing is
+    recomputed (bitwise the checkpointed gradient; (steps + 1) 3N doubles of
+    device memory)."""
+    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n)  # warm-up (allocates the trajectory)
+    t0 = time.perf_counter()
+    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n)
+    el = time.perf_counter() - t0
+    avg_deg = float(c.mesh.face_off[-1]) / c.N
+    bpc = bytes_per_cell_step(c.mesh) + 24 + (24 + 24 + 24 + 48 + 28 * avg_deg + 16)  # forward + store + reverse
+    achieved = bpc * c.N * steps / el / 1e9
     return {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
-            "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": steps,
-                       "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "per-cell Manning n",
-                       "dt": c.dt},
-            "roofline": gradient_roofline(c.mesh, c.N, steps, args.grad_kck, el),
-            "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
+            "checkpoint_every": steps, "trajectory_GB": (steps + 1) * 3 * c.N * 8 / 1e9,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_horizon_cell_step": bpc}}
 
 
 def batch_instances(n_inst, seed=7, lo=5_000, hi=500_000):

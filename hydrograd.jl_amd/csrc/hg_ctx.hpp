@@ -304,7 +304,8 @@ struct hg_ctx {
   int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
   bool inlets_disjoint = true;  // no cell sits on two inlets: the inlet adjoint runs one work-group per inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
-  int32_t* flags = nullptr;  // [0] conveyance error
+  int32_t* flags = nullptr;  // device address of flags_host: [0] conveyance error, [1] non-finite state
+  volatile int32_t* flags_host = nullptr;  // pinned, coherent host memory: cleared and read by the host
   int32_t* zone = nullptr;
   // tiled kernel data
   bool tiled = false;
@@ -342,9 +343,9 @@ struct hg_ctx {
   double* adj_totalA[2] = {nullptr, nullptr};  // kck x n_inlet conveyances recorded by each segment's recompute
   hipStream_t stream2 = nullptr;  // the adjoint's segment recompute, overlapping the reverse sweep
   hipEvent_t ev_seg[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr}, ev_fwd = nullptr;
-  size_t adj_seg_cap = 0, adj_ckpt_cap = 0;
+  size_t adj_seg_cap[2] = {0, 0}, adj_ckpt_cap = 0;
   double* adj_nseg[2] = {nullptr, nullptr};   // kck x N closure n of each recomputed step (two buffers)
-  size_t adj_nseg_cap = 0;
+  size_t adj_nseg_cap[2] = {0, 0};
 
   // UDE closure for n (0: off; else the precision of its network, 32 or 64)
   int32_t ude_prec = 0;

@@ -87,6 +87,27 @@ int dalloc(hg_ctx* ctx, T** p, size_t n) {
   return HG_OK;
 }
 
+// Frees a buffer dalloc gave (NULL: nothing), after the context's streams
+// are idle.
+template <class T>
+int dfree(hg_ctx* ctx, T*& p) {
+  if (!p) return HG_OK;
+  for (hipStream_t st : {ctx->stream, ctx->stream2})
+    if (st && hipStreamSynchronize(st) != hipSuccess) {
+      ctx->err = "hipStreamSynchronize before a free failed";
+      return HG_ERR_HIP;
+    }
+  auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), static_cast<void*>(p));
+  if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+  const hipError_t e = hipFree(p);
+  p = nullptr;
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipFree: ") + hipGetErrorString(e);
+    return HG_ERR_HIP;
+  }
+  return HG_OK;
+}
+
 template <class T>
 int upload(hg_ctx* ctx, T* dst, const T* src, size_t n) {
   if (n == 0) return HG_OK;
@@ -96,11 +117,6 @@ int upload(hg_ctx* ctx, T* dst, const T* src, size_t n) {
 }
 
 // --------------------------------------------------------------- kernels ---
-
-// flags[0..3] = 0.
-__global__ void k_clear_flags(int32_t* flags) {
-  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
-}
 
 
 // Inlet conveyance (bc_2D.jl:640-680): total_A_b = sum over the inlet's
@@ -2105,31 +2121,37 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (!rc) rc = dalloc(ctx, &ctx->adj_Qbar, std::max<int32_t>(ctx->n_inlet, 1));
     if (rc) return rc;
   }
-  if (size_t(kck) > ctx->adj_seg_cap) {  // buffers are only ever grown; old ones stay owned by allocs
-    rc = drop_graph(ctx);  // the segment graphs captured the old buffers
-    if (rc) return rc;
-    for (int b = 0; b < 2 && !rc; ++b) {  // two segment buffers: one recomputed while the other is reversed
-      rc = dalloc(ctx, &ctx->adj_seg[b], size_t(kck + 1) * 3 * N);
+  // Segment buffers: two (one recomputed while the other is reversed), or
+  // one when a single segment covers the run.  Grown when kck grows (the
+  // segment graphs that captured the old ones are dropped first).
+  const int nbuf = nseg > 1 ? 2 : 1;
+  for (int b = 0; b < nbuf && !rc; ++b) {
+    if (size_t(kck) > ctx->adj_seg_cap[b]) {
+      rc = drop_graph(ctx);
+      if (!rc) rc = dfree(ctx, ctx->adj_seg[b]);
+      if (!rc) rc = dfree(ctx, ctx->adj_mask[b]);
+      if (!rc) rc = dfree(ctx, ctx->adj_totalA[b]);
+      if (!rc) rc = dalloc(ctx, &ctx->adj_seg[b], size_t(kck + 1) * 3 * N);
       if (!rc) rc = dalloc(ctx, &ctx->adj_mask[b], size_t(kck) * N);
       if (!rc) rc = dalloc(ctx, &ctx->adj_totalA[b], size_t(kck) * std::max<int32_t>(ctx->n_inlet, 1));
+      if (!rc) ctx->adj_seg_cap[b] = size_t(kck);
     }
-    if (rc) return rc;
-    ctx->adj_seg_cap = size_t(kck);
+    if (!rc && ctx->ude_prec && size_t(kck) > ctx->adj_nseg_cap[b]) {  // the closure's n of each step
+      rc = drop_graph(ctx);
+      if (!rc) rc = dfree(ctx, ctx->adj_nseg[b]);
+      if (!rc) rc = dalloc(ctx, &ctx->adj_nseg[b], size_t(kck) * N);
+      if (!rc) ctx->adj_nseg_cap[b] = size_t(kck);
+    }
   }
-  if (ctx->ude_prec && size_t(kck) > ctx->adj_nseg_cap) {  // the closure's n of each step of a segment
-    rc = drop_graph(ctx);
-    if (rc) return rc;
-    for (int b = 0; b < 2 && !rc; ++b) rc = dalloc(ctx, &ctx->adj_nseg[b], size_t(kck) * N);
-    if (rc) return rc;
-    ctx->adj_nseg_cap = size_t(kck);
-  }
+  if (rc) return rc;
   if (!ctx->stream2) {  // the recompute stream (default priority) and its events
     HG_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     for (hipEvent_t* e : {&ctx->ev_seg[0], &ctx->ev_seg[1], &ctx->ev_rev[0], &ctx->ev_rev[1], &ctx->ev_fwd})
       HG_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if (size_t(nseg) > ctx->adj_ckpt_cap) {
-    rc = dalloc(ctx, &ctx->adj_ckpt, size_t(nseg) * 3 * N);
+    rc = dfree(ctx, ctx->adj_ckpt);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_ckpt, size_t(nseg) * 3 * N);
     if (rc) return rc;
     ctx->adj_ckpt_cap = size_t(nseg);
   }
@@ -2328,6 +2350,13 @@ double* seg_totalA(hg_ctx* ctx, int64_t t, int b = 0) {
 }
 
 // Forward nsteps from the device state, storing a checkpoint every kck steps.
+int recompute_segment(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int64_t sgi, int b, hipStream_t stream);
+
+// The forward sweep: a checkpoint every kck steps.  Its last segment is run
+// as the reverse sweep's recompute would run it (recording every state, the
+// clamps and the conveyances into segment buffer 0), so the reversal starts
+// without recomputing it; with kck >= nsteps nothing is recomputed.  The
+// final state ends in state slot 0.
 int forward_checkpointed(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck) {
   const int64_t N = ctx->N;
   const int64_t nseg = (nsteps + kck - 1) / kck;
@@ -2335,9 +2364,18 @@ int forward_checkpointed(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck) {
     for (int c = 0; c < 3; ++c)
       HG_TRY(hipMemcpyAsync(ctx->adj_ckpt + (sgi * 3 + c) * N, ctx->st[ctx->cur][c], N * sizeof(double),
                             hipMemcpyDeviceToDevice, ctx->stream));
-    int rc = advance_async(ctx, dt, std::min<int64_t>(kck, nsteps - sgi * kck));
-    if (rc) return rc;
+    if (sgi + 1 < nseg) {
+      int rc = advance_async(ctx, dt, kck);
+      if (rc) return rc;
+    }
   }
+  int rc = recompute_segment(ctx, dt, nsteps, kck, nseg - 1, 0, ctx->stream);
+  if (rc) return rc;
+  const int64_t ns = nsteps - (nseg - 1) * kck;
+  for (int c = 0; c < 3; ++c)
+    HG_TRY(hipMemcpyAsync(ctx->st[0][c], seg_state(ctx, ns, c, 0), N * sizeof(double), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  ctx->cur = 0;
   return HG_OK;
 }
 
@@ -2385,9 +2423,9 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
     HG_TRY(hipEventRecord(ctx->ev_fwd, ctx->stream));
     HG_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fwd, 0));
   }
-  int rc = recompute_segment(ctx, dt, nsteps, kck, nseg - 1, 0, rs);
-  if (rc) return rc;
-  if (overlap) HG_TRY(hipEventRecord(ctx->ev_seg[0], rs));
+  // the last segment is already in buffer 0 (recorded by the forward sweep)
+  int rc = HG_OK;
+  if (overlap) HG_TRY(hipEventRecord(ctx->ev_seg[0], ctx->stream));
   for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
     const int b = int((nseg - 1 - sgi) & 1);
     if (overlap && sgi > 0) {  // the previous segment, into the other buffer once its last reversal is done
@@ -2429,16 +2467,20 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
   return HG_OK;
 }
 
-int clear_flags(hg_ctx* ctx) {  // a fill kernel: no pageable host copy on the stream
-  hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, ctx->stream, ctx->flags);
-  HG_TRY(hipGetLastError());
+// The error flags live in pinned, coherent host memory that the kernels
+// write only on an error: clearing and reading them takes no copy on the
+// stream.  Every call that clears them has no kernel of its own in flight
+// (each entry point ends synchronized), and the stream is synchronized
+// before they are read.
+int clear_flags(hg_ctx* ctx) {
+  for (int k = 0; k < 4; ++k) ctx->flags_host[k] = 0;
   return HG_OK;
 }
 
 int check_flags(hg_ctx* ctx) {
-  int32_t f[4];
-  HG_TRY(hipMemcpyAsync(f, ctx->flags, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
   HG_TRY(hipStreamSynchronize(ctx->stream));
+  int32_t f[4];
+  for (int k = 0; k < 4; ++k) f[k] = ctx->flags_host[k];
   if (f[0]) {
     ctx->err = "Total cross-sectional conveyance for an inlet-q boundary is not positive (<= 1e-10)";
     return HG_ERR_CONVEYANCE;
@@ -2983,7 +3025,21 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = dalloc(ctx, &ctx->inletQ0, ctx->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->exitWSE, ctx->n_exit);
   if (!rc) rc = dalloc(ctx, &ctx->totalA, ctx->n_inlet);
-  if (!rc) rc = dalloc(ctx, &ctx->flags, 4);
+  if (!rc) {
+    void* hp = nullptr;
+    if (hipHostMalloc(&hp, 4 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+      ctx->err = "hipHostMalloc (error flags) failed";
+      rc = HG_ERR_HIP;
+    } else {
+      ctx->flags_host = static_cast<volatile int32_t*>(hp);
+      void* dp = nullptr;
+      if (hipHostGetDevicePointer(&dp, hp, 0) != hipSuccess) {
+        ctx->err = "hipHostGetDevicePointer (error flags) failed";
+        rc = HG_ERR_HIP;
+      }
+      ctx->flags = static_cast<int32_t*>(dp);
+    }
+  }
   if (!rc) rc = dalloc(ctx, &ctx->zone, N);
   ctx->scratch_n = size_t(std::max<int64_t>(3 * NT, 4096));
   if (!rc) rc = dalloc(ctx, &ctx->scratch, ctx->scratch_n);
@@ -3214,6 +3270,7 @@ int hg_destroy(hg_ctx* ctx) {
     (void)hipStreamDestroy(ctx->stream2);
   }
   for (void* p : ctx->allocs) (void)hipFree(p);
+  if (ctx->flags_host) (void)hipHostFree(const_cast<int32_t*>(ctx->flags_host));
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HG_OK;

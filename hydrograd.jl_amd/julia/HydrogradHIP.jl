@@ -21,7 +21,7 @@ module HydrogradHIP
 
 using ChainRulesCore
 
-export swe_2d_rhs, custom_ODE_update_cells, custom_ODE_solve, make_ode_f, tsit5_solve, set_ude!, ude_grad,
+export swe_2d_rhs, custom_ODE_update_cells, fields_context, custom_ODE_solve, make_ode_f, tsit5_solve, set_ude!, ude_grad,
        sensitivity_jvp
 
 const libhgswe = joinpath(@__DIR__, "..", "lib", "libhgswe.so")
@@ -155,6 +155,40 @@ function context(p)
         set_lux_ude!(ctx, p)
         ctx
     end
+end
+
+# K Manning-n fields on the mesh of p (hg_fields_create): the mesh is stored
+# once and every call advances all K fields, with the arrays of the K fields
+# laid end to end (field f after field f-1); ManningN is an N x K matrix.
+# Returns a Context for the same calls as a single mesh's (hg_step,
+# hg_grad_probes, ...), with K-fold arrays.
+function fields_context(p, ManningN::AbstractMatrix)
+    K = size(ManningN, 2)
+    d = flatten(p.my_mesh_2D, p.boundary_conditions)
+    c = p.swe_2D_constants
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve d begin
+        mesh = Ref(HgMesh(d.N, pointer(d.off), pointer(d.nbr), pointer(d.nx), pointer(d.ny), pointer(d.len),
+                          pointer(d.area), pointer(d.cen)))
+        bc = Ref(HgBcs(d.G, pointer(d.gslot), pointer(d.gkind), pointer(d.gbc), d.nI, d.nE,
+                       pointer(d.inlet_off), pointer(d.inlet_ghost)))
+        k = Ref(HgConsts(c.g, c.k_n, c.h_small))
+        rc = ccall((:hg_fields_create, libhgswe), Cint,
+                   (Ref{Ptr{Cvoid}}, Int32, Ref{HgMesh}, Ref{HgBcs}, Ref{HgConsts}, Int32), out, Int32(K), mesh, bc, k,
+                   Int32(1))
+        check(C_NULL, rc)
+    end
+    ctx = Context(out[], K * d.N, HG_PARAM_NONE, 0, 0)
+    finalizer(x -> ccall((:hg_destroy, libhgswe), Cint, (Ptr{Cvoid},), x.ptr), ctx)
+    rep(v) = repeat(Vector{Float64}(v), K)
+    S0 = p.S0_cells
+    check(ctx.ptr, ccall((:hg_set_fields, libhgswe), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                          Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, rep(p.hstill), rep(p.hstill_ghostCells), rep(p.zb_cells), rep(p.zb_ghostCells),
+                         rep(S0[:, 1]), rep(S0[:, 2]), Vector{Float64}(vec(ManningN)), rep(p.inletQ_TotalQ),
+                         rep(p.exitH_WSE)))
+    return ctx
 end
 
 # The forward simulation's variable Manning n (ManningN_option "variable",
