@@ -27,14 +27,20 @@ __host__ __device__ __forceinline__ void ext_zero(ExtBar& e) { e.xi = e.h = e.hu
 
 // u = hu/h, v = hv/h, s = sqrt(h + eps), p = 0.5 g ((xi+eps)^2 + 2 xi hs):
 // fold the derived cotangents into (xi, h, hu, hv).
-__host__ __device__ __forceinline__ SideBar ext_to_side(const ExtBar& e, const CellQ& c, double half_g) {
+// rh = 1 / h and r2s = 0.5 / s are the side's own reciprocals (a tiled
+// kernel forms them once per cell).
+__host__ __device__ __forceinline__ SideBar ext_to_side_r(const ExtBar& e, const CellQ& c, double rh, double r2s,
+                                                          double half_g) {
   SideBar s;
-  const double rh = 1.0 / c.h;
   s.hu = e.hu + e.u * rh;
   s.hv = e.hv + e.v * rh;
-  s.h = e.h - (e.u * c.u + e.v * c.v) * rh + e.s * (0.5 / c.s);
+  s.h = e.h - (e.u * c.u + e.v * c.v) * rh + e.s * r2s;
   s.xi = e.xi + e.p * half_g * (2.0 * (c.xi + EPS) + 2.0 * c.hs);
   return s;
+}
+
+__host__ __device__ __forceinline__ SideBar ext_to_side(const ExtBar& e, const CellQ& c, double half_g) {
+  return ext_to_side_r(e, c, 1.0 / c.h, 0.5 / c.s, half_g);
 }
 
 // Physical flux of one side (hu, hv, u, v, p), reverse.
@@ -49,19 +55,16 @@ __host__ __device__ __forceinline__ void phys_vjp(double hu, double hv, double u
 }
 
 // VJP of the flux F(A | B, n) seen from A (Riemann_2D_Roe,
-// swe_2D_solvers.jl:4-164) with cotangent Fb: cotangents of both sides'
-// (xi, h, hu, hv).  A and B are the flux inputs (clamped depth, masked
-// discharge, derived u, v, s, p) as make_cellq forms them.
-__host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double nx, double ny, double g, double half_g,
-                                        double hmin, const double Fb[3], SideBar& Ab, SideBar& Bb) {
-  ExtBar ea, eb;
+// swe_2D_solvers.jl:4-164) with cotangent Fb: cotangents of both sides' raw
+// and derived quantities (unfolded; ext_to_side folds them).  A and B are
+// the flux inputs (clamped depth, masked discharge, derived u, v, s, p) as
+// make_cellq forms them.
+__host__ __device__ __forceinline__ void roe_vjp_ext(const CellQ& A, const CellQ& B, double nx, double ny, double g,
+                                                     double half_g, double hmin, const double Fb[3], ExtBar& ea,
+                                                     ExtBar& eb) {
   ext_zero(ea);
   ext_zero(eb);
-  if (A.h <= hmin && B.h <= hmin) {
-    Ab = SideBar{0.0, 0.0, 0.0, 0.0};
-    Bb = SideBar{0.0, 0.0, 0.0, 0.0};
-    return;
-  }
+  if (A.h <= hmin && B.h <= hmin) return;
   int mode = 0;  // 0 general, 1 B mirrored from A, 2 A mirrored from B
   double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
   double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
@@ -82,8 +85,6 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
     ew.v += Fb[1] * ny * W.hu + Fb[2] * ny * W.hv;
     const double prb = Fb[1] * nx + Fb[2] * ny;
     ew.h += prb * half_g * 2.0 * (W.h + EPS);
-    Ab = ext_to_side(ea, A, half_g);
-    Bb = ext_to_side(eb, B, half_g);
     return;
   }
   // ---- forward recompute (general branch)
@@ -169,6 +170,13 @@ __host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double n
     ea = ma;
     eb = mb;
   }
+}
+
+// The same, folded: cotangents of both sides' (xi, h, hu, hv).
+__host__ __device__ inline void roe_vjp(const CellQ& A, const CellQ& B, double nx, double ny, double g, double half_g,
+                                        double hmin, const double Fb[3], SideBar& Ab, SideBar& Bb) {
+  ExtBar ea, eb;
+  roe_vjp_ext(A, B, nx, ny, g, half_g, hmin, Fb, ea, eb);
   Ab = ext_to_side(ea, A, half_g);
   Bb = ext_to_side(eb, B, half_g);
 }

@@ -173,6 +173,10 @@ struct AdjArgs {
   double* fbar;       // ... and its cotangent
   int32_t nbar_set;   // 1: nbar holds this evaluation's n-bar alone (set, not added to)
   int32_t fbar_set;   // 1: fbar likewise
+  // tiled reverse step (k_adj_tile): the step kernel's work items and block -> tile mapping
+  const int32_t* item_off;
+  const struct ItemRec* items;
+  int32_t xcd_chunk;
 };
 
 // Per-field views of a launch over K fields that share one mesh: field f's
@@ -403,6 +407,7 @@ struct hg_ctx {
   // with gexec whenever a launch they captured would change)
   std::map<SegGraphKey, hipGraphExec_t> seg_graphs;
   double gdt = 0.0;
+  bool use_graphs = true;  // false (environment HG_GRAPHS=0 at creation): every launch is a plain launch
 
   std::vector<void*> allocs;
 };

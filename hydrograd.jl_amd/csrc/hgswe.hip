@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -290,6 +291,26 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a0) {
 //     the source terms and applies the Euler update and dry clamp.
 // T (cells per tile) and MD (slots per cell) are compile-time constants so
 // every LDS address is a base register plus an immediate offset.
+// This work-group's tile t and field fld (the launch covers every tile of
+// each of K fields; see k_swe_tile).
+__device__ __forceinline__ void tile_of_block(int K, int xcd_chunk, int& t, int& fld) {
+  const int nt = int(gridDim.x) / K;
+  const int Gc = xcd_chunk > 0 ? xcd_chunk : 1;
+  const int blk = int(blockIdx.x);
+  const int W = kXCD * Gc * K;
+  const int w = blk / W, rr = blk - w * W;
+  const int tw = w * kXCD * Gc;
+  if (tw + kXCD * Gc <= nt) {
+    const int q = rr / kXCD;
+    t = tw + (rr % kXCD) * Gc + q % Gc;
+    fld = q / Gc;
+  } else {
+    const int rem = nt - tw;
+    t = tw + rr % rem;
+    fld = rr / rem;
+  }
+}
+
 template <bool STEP, int T, int MD>
 __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a0) {
   extern __shared__ double lds[];
@@ -301,22 +322,8 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a0) {
   // that XCD's L2, and the K fields of a tile read its mesh data from that
   // L2.  A final partial window takes its tiles in turn for each field.
   // With K = 1, G = 0 is the identity.  Bijection on [0, K nt).
-  const int nt = int(gridDim.x) / a0.K;
-  const int Gc = a0.xcd_chunk > 0 ? a0.xcd_chunk : 1;
-  const int blk = int(blockIdx.x);
-  const int W = kXCD * Gc * a0.K;
-  const int w = blk / W, rr = blk - w * W;
-  const int tw = w * kXCD * Gc;
   int t, fld;
-  if (tw + kXCD * Gc <= nt) {
-    const int q = rr / kXCD;
-    t = tw + (rr % kXCD) * Gc + q % Gc;
-    fld = q / Gc;
-  } else {
-    const int rem = nt - tw;
-    t = tw + rr % rem;
-    fld = rr / rem;
-  }
+  tile_of_block(a0.K, a0.xcd_chunk, t, fld);
   const hg::TileArgs a = hg::field_view(a0, fld);
   const int li = threadIdx.x;
   const int i = t * T + li;
@@ -668,6 +675,102 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
   return c;
 }
 
+// A boundary face's ghost state (bc_2D.jl:640-834) formed from its cell's
+// flux inputs `me`, and what the routing of its cotangent back to the cell
+// needs (the ghost's kind; inlet-Q: the ghost velocity v and wet flag w).
+struct GhostAdj {
+  int kind, bc;
+  double v, w;
+};
+
+__device__ __forceinline__ hg::CellQ adj_ghost_state(const hg::AdjArgs& a, const hg::CellQ& me, double n_i, int gi,
+                                                     double nx, double ny, GhostAdj& gs) {
+  const double hsmall = a.hsmall;
+  gs.kind = a.g_kind[gi];
+  gs.bc = a.g_bc[gi];
+  gs.v = 0.0;
+  gs.w = 0.0;
+  double hgh, qxg, qyg;
+  switch (gs.kind) {
+    case HG_BC_INLETQ: {
+      gs.w = (me.h > hsmall) ? 1.0 : 0.0;
+      gs.v = a.inletQ[gs.bc] / a.totalA[gs.bc] * a.g_L23[gi] / n_i;
+      hgh = me.h;
+      qxg = -me.h * gs.v * nx * gs.w;
+      qyg = -me.h * gs.v * ny * gs.w;
+      break;
+    }
+    case HG_BC_EXITH: {
+      const double hx = a.exitWSE[gs.bc] - me.zb;
+      hgh = (hx > hsmall) ? hx : hsmall;
+      qxg = me.hu;
+      qyg = me.hv;
+      break;
+    }
+    case HG_BC_WALL:
+      hgh = me.h;
+      qxg = -me.hu;
+      qyg = -me.hv;
+      break;
+    default: {
+      const double vdn = me.hu * nx + me.hv * ny;
+      hgh = me.h;
+      qxg = me.hu - 2.0 * vdn * nx;
+      qyg = me.hv - 2.0 * vdn * ny;
+      break;
+    }
+  }
+  hg::CellQ B;
+  B.hs = a.g_hstill[gi];
+  B.xi = hgh - B.hs;
+  B.h = hgh;
+  B.hu = qxg;
+  B.hv = qyg;
+  B.zb = a.g_zb[gi];
+  B.u = qxg / hgh;
+  B.v = qyg / hgh;
+  B.s = sqrt(hgh + hg::EPS);
+  B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
+  return B;
+}
+
+// Routes the ghost's cotangent Bb back through its boundary rule: adds to
+// the cell's U, and its n / exit-H bed cotangents; writes the inlet ghost's
+// velocity cotangent for the inlet conveyance adjoint.
+__device__ __forceinline__ void adj_ghost_route(const hg::AdjArgs& a, const hg::CellQ& me, double n_i, int gi,
+                                                const GhostAdj& gs, const hg::SideBar& Bb, double nx, double ny,
+                                                hg::SideBar& U, double& nb, double& zbb) {
+  const double hgb = Bb.h + Bb.xi;  // xi_g = h_g - hstill_g
+  switch (gs.kind) {
+    case HG_BC_INLETQ: {  // h_g = h, q_g = -h v n w, v = (Q / A) L^(2/3) / n
+      const double v = gs.v, w = gs.w;
+      U.h += hgb + Bb.hu * (-v * nx * w) + Bb.hv * (-v * ny * w);
+      const double vb = Bb.hu * (-me.h * nx * w) + Bb.hv * (-me.h * ny * w);
+      nb += vb * (-v / n_i);
+      a.vbar[gi] = vb;
+      break;
+    }
+    case HG_BC_EXITH: {  // h_g = max(h_small, WSE - zb_cell), q_g = q
+      if (a.exitWSE[gs.bc] - me.zb > a.hsmall) zbb -= hgb;
+      U.hu += Bb.hu;
+      U.hv += Bb.hv;
+      break;
+    }
+    case HG_BC_WALL:
+      U.h += hgb;
+      U.hu -= Bb.hu;
+      U.hv -= Bb.hv;
+      break;
+    default: {
+      const double vdnb = -2.0 * (Bb.hu * nx + Bb.hv * ny);
+      U.h += hgb;
+      U.hu += Bb.hu + vdnb * nx;
+      U.hv += Bb.hv + vdnb * ny;
+      break;
+    }
+  }
+}
+
 // The VJP of the boundary face in slot s (cell i = s % N, local face
 // k = s / N) seen from cell i: its contribution to the cotangent of the
 // cell's state (U), of n and of the exit-H bed, with the ghost's dependence
@@ -675,92 +778,17 @@ __device__ __forceinline__ CellAdj cell_adj(const hg::AdjArgs& a, int i) {
 template <bool STEP>
 __device__ __forceinline__ void adj_ghost_face(const hg::AdjArgs& a, const CellAdj& ca, int64_t s, hg::SideBar& U,
                                                double& nb, double& zbb) {
-  const double hsmall = a.hsmall;
-  const hg::CellQ& me = ca.me;
-  const int j = a.nbr[s];
+  const int gi = ~a.nbr[s];
   const double nx = a.fnx[s], ny = a.fny[s], L = a.flen[s];
   const double fi = -(L / ca.A);
-  U = hg::SideBar{0.0, 0.0, 0.0, 0.0};
   nb = 0.0;
   zbb = 0.0;
-  {
-    const int gi = ~j;
-    const int kind = a.g_kind[gi];
-    double hgh, qxg, qyg, v = 0.0, w = 0.0;
-    switch (kind) {
-      case HG_BC_INLETQ: {
-        const int b = a.g_bc[gi];
-        w = ca.wet ? 1.0 : 0.0;
-        v = a.inletQ[b] / a.totalA[b] * a.g_L23[gi] / ca.n;
-        hgh = me.h;
-        qxg = -me.h * v * nx * w;
-        qyg = -me.h * v * ny * w;
-        break;
-      }
-      case HG_BC_EXITH: {
-        const double hx = a.exitWSE[a.g_bc[gi]] - ca.zb;
-        hgh = (hx > hsmall) ? hx : hsmall;
-        qxg = me.hu;
-        qyg = me.hv;
-        break;
-      }
-      case HG_BC_WALL:
-        hgh = me.h;
-        qxg = -me.hu;
-        qyg = -me.hv;
-        break;
-      default: {
-        const double vdn = me.hu * nx + me.hv * ny;
-        hgh = me.h;
-        qxg = me.hu - 2.0 * vdn * nx;
-        qyg = me.hv - 2.0 * vdn * ny;
-        break;
-      }
-    }
-    hg::CellQ B;
-    B.hs = a.g_hstill[gi];
-    B.xi = hgh - B.hs;
-    B.h = hgh;
-    B.hu = qxg;
-    B.hv = qyg;
-    B.zb = a.g_zb[gi];
-    B.u = qxg / hgh;
-    B.v = qyg / hgh;
-    B.s = sqrt(hgh + hg::EPS);
-    B.p = a.half_g * ((B.xi + hg::EPS) * (B.xi + hg::EPS) + 2.0 * B.xi * B.hs);
-    const double c[3] = {fi * ca.y0, fi * ca.y1, fi * ca.y2};
-    hg::SideBar Ab, Bb;
-    hg::roe_vjp(me, B, nx, ny, a.g, a.half_g, hsmall, c, Ab, Bb);
-    U = Ab;
-    const double hgb = Bb.h + Bb.xi;  // xi_g = h_g - hstill_g
-    switch (kind) {
-      case HG_BC_INLETQ: {  // h_g = h, q_g = -h v n w, v = (Q / A) L^(2/3) / n
-        U.h += hgb + Bb.hu * (-v * nx * w) + Bb.hv * (-v * ny * w);
-        const double vb = Bb.hu * (-me.h * nx * w) + Bb.hv * (-me.h * ny * w);
-        nb += vb * (-v / ca.n);
-        a.vbar[gi] = vb;
-        break;
-      }
-      case HG_BC_EXITH: {  // h_g = max(h_small, WSE - zb_cell), q_g = q
-        if (a.exitWSE[a.g_bc[gi]] - ca.zb > hsmall) zbb -= hgb;
-        U.hu += Bb.hu;
-        U.hv += Bb.hv;
-        break;
-      }
-      case HG_BC_WALL:
-        U.h += hgb;
-        U.hu -= Bb.hu;
-        U.hv -= Bb.hv;
-        break;
-      default: {
-        const double vdnb = -2.0 * (Bb.hu * nx + Bb.hv * ny);
-        U.h += hgb;
-        U.hu += Bb.hu + vdnb * nx;
-        U.hv += Bb.hv + vdnb * ny;
-        break;
-      }
-    }
-  }
+  GhostAdj gs;
+  const hg::CellQ B = adj_ghost_state(a, ca.me, ca.n, gi, nx, ny, gs);
+  const double c[3] = {fi * ca.y0, fi * ca.y1, fi * ca.y2};
+  hg::SideBar Bb;
+  hg::roe_vjp(ca.me, B, nx, ny, a.g, a.half_g, a.hsmall, c, U, Bb);
+  adj_ghost_route(a, ca.me, ca.n, gi, gs, Bb, nx, ny, U, nb, zbb);
 }
 
 template <bool STEP>
@@ -861,6 +889,194 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
     a.S0bar[i] += S0xb;
     a.S0bar[a.N + i] += S0yb;
   }
+}
+
+// Tiled reverse step (production path on meshes the tiled step kernel
+// runs).  One work-group per tile of T cells, over the step kernel's work
+// items:
+//  1. every cell's flux inputs (make_cellq; the pressure is re-formed from
+//     xi and h_still where read), its output cotangent times dt over its
+//     area (z = y / A, y zero where the step's clamp fired), 1 / h and
+//     0.5 / sqrt(h + eps) go to LDS;
+//  2. each item forms one face's VJP: an intra-tile face once for both
+//     cells, a cross-tile face (neighbour gathered from memory) for the
+//     tile's cell only, a boundary face with its ghost formed from the cell
+//     and its cotangent routed back through the boundary rule.  The face
+//     cotangent combines both cells' outputs, c = L (z_b - z_a).  An
+//     internal face is evaluated from the side whose normal points to +x
+//     (or along +y when n_x = 0), so both cells receive the same bits
+//     whichever work-group evaluates the face: the results do not depend on
+//     the tiling, the numbering or the launch shape.  Each side's
+//     cotangent lands in its (cell, local face) LDS slot as (xi + h for a
+//     wet cell, xi for a dry one -- the step's clamp drops h-bar there),
+//     hu, hv and the boundary face's n or exit-H bed term (a per-cell mask
+//     marks exit-H slots);
+//  3. each cell sums its slots in local face order and finishes as
+//     k_adj_cell does (source VJP, identity part of the step).
+// LDS: (14 + 4 MD) doubles and one int per cell, 53 KiB for 256 triangles:
+// three work-groups per CU, as many as the registers allow.
+constexpr int kAdjRows = 14;  // CellQ without p (9), z (3), 1 / h, 0.5 / s
+
+size_t adj_tile_lds_bytes(int T, int MD) {
+  return size_t(kAdjRows * T + 4 * MD * T) * sizeof(double) + size_t(T) * sizeof(int32_t);
+}
+
+template <int T>
+__device__ __forceinline__ hg::CellQ lds_cellq(const double* C, int r, double half_g) {
+  hg::CellQ q;
+  q.xi = C[0 * T + r]; q.hs = C[1 * T + r]; q.h = C[2 * T + r]; q.hu = C[3 * T + r]; q.hv = C[4 * T + r];
+  q.zb = C[5 * T + r]; q.u = C[6 * T + r]; q.v = C[7 * T + r]; q.s = C[8 * T + r];
+  q.p = half_g * ((q.xi + hg::EPS) * (q.xi + hg::EPS) + 2.0 * q.xi * q.hs);  // as make_cellq forms it
+  return q;
+}
+
+template <bool STEP, int T, int MD>
+__global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
+  extern __shared__ double lds[];
+  int t, fld;
+  tile_of_block(a0.K, a0.xcd_chunk, t, fld);
+  const hg::AdjArgs a = hg::field_view(a0, fld);
+  const int li = threadIdx.x;
+  const int i = t * T + li;
+  const bool valid = i < a.N;
+  double* C = lds;                                         // kAdjRows x T
+  double* S = lds + kAdjRows * T;                          // 4 x MD x T: per slot xi(+h), hu, hv, n or zb
+  int32_t* Xm = reinterpret_cast<int32_t*>(S + 4 * MD * T);  // T: exit-H slots of each cell
+  const double hsmall = a.hsmall;
+  int d_i = 0;
+  if (valid) {
+    const hg::CellQ me = hg::make_cellq(a.xi[i], a.hstill[i], a.qx[i], a.qy[i], a.zb[i], a.half_g, hsmall);
+    double l0 = a.lx[i], l1 = a.lqx[i], l2 = a.lqy[i];
+    if (STEP && a.mask && a.mask[i]) l0 = l1 = l2 = 0.0;
+    const double dti = a.dtc ? a.dtc[i] : a.dt;
+    const double rA = 1.0 / a.area[i];
+    C[0 * T + li] = me.xi; C[1 * T + li] = me.hs; C[2 * T + li] = me.h; C[3 * T + li] = me.hu;
+    C[4 * T + li] = me.hv; C[5 * T + li] = me.zb; C[6 * T + li] = me.u; C[7 * T + li] = me.v;
+    C[8 * T + li] = me.s;
+    C[9 * T + li] = (STEP ? dti * l0 : l0) * rA;
+    C[10 * T + li] = (STEP ? dti * l1 : l1) * rA;
+    C[11 * T + li] = (STEP ? dti * l2 : l2) * rA;
+    C[12 * T + li] = 1.0 / me.h;
+    C[13 * T + li] = 0.5 / me.s;
+    d_i = a.deg[i];
+    for (int k = 0; k < MD; ++k) S[(3 * MD + k) * T + li] = 0.0;
+    Xm[li] = 0;
+  }
+  __syncthreads();
+
+  const double half_g = a.half_g;
+  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
+  for (int it = it0 + li; it < it1; it += T) {
+    const hg::ItemRec rec = a.items[it];
+    const int code = rec.code;
+    const int ia = code & 1023, ka = (code >> 10) & 15;
+    const int ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
+    const int type = (code >> 28) & 3;
+    const double nx = rec.nx, ny = rec.ny, L = rec.len;
+    // P / Q: the evaluation's first / second side (normal from P)
+    const bool sw = type != 2 && !(nx > 0.0 || (nx == 0.0 && ny > 0.0));
+    const double ex = sw ? -nx : nx, ey = sw ? -ny : ny;
+    hg::CellQ P, Q;
+    double zP0, zP1, zP2, zQ0, zQ1, zQ2, rhP, rhQ, r2P, r2Q;
+    int ip = ia, kp = ka, iq = ia, kq = ka;  // LDS slots of P and Q (type 0); the tile's cell otherwise
+    GhostAdj gs{};
+    double n_a = 0.0;
+    if (type == 0) {
+      if (sw) { ip = ib; kp = kb; } else { iq = ib; kq = kb; }
+      P = lds_cellq<T>(C, ip, half_g);
+      Q = lds_cellq<T>(C, iq, half_g);
+      zP0 = C[9 * T + ip]; zP1 = C[10 * T + ip]; zP2 = C[11 * T + ip];
+      zQ0 = C[9 * T + iq]; zQ1 = C[10 * T + iq]; zQ2 = C[11 * T + iq];
+      rhP = C[12 * T + ip]; r2P = C[13 * T + ip];
+      rhQ = C[12 * T + iq]; r2Q = C[13 * T + iq];
+    } else {
+      const hg::CellQ Lc = lds_cellq<T>(C, ia, half_g);
+      const double zL0 = C[9 * T + ia], zL1 = C[10 * T + ia], zL2 = C[11 * T + ia];
+      const double rhL = C[12 * T + ia], r2L = C[13 * T + ia];
+      hg::CellQ G;
+      double zG0 = 0.0, zG1 = 0.0, zG2 = 0.0;
+      if (type == 1) {  // cross-tile neighbour
+        const int j = rec.ref;
+        G = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], half_g, hsmall);
+        double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
+        if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
+        const double dtj = a.dtc ? a.dtc[j] : a.dt;
+        const double rAj = 1.0 / a.area[j];
+        zG0 = (STEP ? dtj * m0 : m0) * rAj;
+        zG1 = (STEP ? dtj * m1 : m1) * rAj;
+        zG2 = (STEP ? dtj * m2 : m2) * rAj;
+      } else {  // ghost (no cotangent of its own)
+        n_a = a.n[t * T + ia];
+        G = adj_ghost_state(a, Lc, n_a, rec.ref, nx, ny, gs);
+      }
+      const double rhG = 1.0 / G.h, r2G = 0.5 / G.s;
+      P = sw ? G : Lc;
+      Q = sw ? Lc : G;
+      zP0 = sw ? zG0 : zL0; zP1 = sw ? zG1 : zL1; zP2 = sw ? zG2 : zL2;
+      zQ0 = sw ? zL0 : zG0; zQ1 = sw ? zL1 : zG1; zQ2 = sw ? zL2 : zG2;
+      rhP = sw ? rhG : rhL; r2P = sw ? r2G : r2L;
+      rhQ = sw ? rhL : rhG; r2Q = sw ? r2L : r2G;
+    }
+    const double c[3] = {L * zQ0 - L * zP0, L * zQ1 - L * zP1, L * zQ2 - L * zP2};
+    hg::ExtBar eP, eQ;
+    hg::roe_vjp_ext(P, Q, ex, ey, a.g, half_g, hsmall, c, eP, eQ);
+    hg::SideBar sP = hg::ext_to_side_r(eP, P, rhP, r2P, half_g);
+    const hg::SideBar sQ = hg::ext_to_side_r(eQ, Q, rhQ, r2Q, half_g);
+    if (type == 2) {
+      double nb = 0.0, zbb = 0.0;
+      adj_ghost_route(a, P, n_a, rec.ref, gs, sQ, nx, ny, sP, nb, zbb);
+      if (gs.kind == HG_BC_EXITH) {
+        S[(3 * MD + ka) * T + ia] = zbb;
+        atomicOr(&Xm[ia], 1 << ka);
+      } else {
+        S[(3 * MD + ka) * T + ia] = nb;
+      }
+    }
+    if (type == 0 || !sw) {
+      S[(0 * MD + kp) * T + ip] = (P.h > hsmall) ? sP.xi + sP.h : sP.xi;
+      S[(1 * MD + kp) * T + ip] = sP.hu;
+      S[(2 * MD + kp) * T + ip] = sP.hv;
+    }
+    if (type == 0 || sw) {
+      S[(0 * MD + kq) * T + iq] = (Q.h > hsmall) ? sQ.xi + sQ.h : sQ.xi;
+      S[(1 * MD + kq) * T + iq] = sQ.hu;
+      S[(2 * MD + kq) * T + iq] = sQ.hv;
+    }
+  }
+  __syncthreads();
+
+  if (!valid) return;
+  double uxh = 0.0, uqx = 0.0, uqy = 0.0, nb = 0.0, zbb = 0.0;
+  const int xm = Xm[li];
+  for (int k = 0; k < MD; ++k) {  // local face order
+    if (k < d_i) {
+      uxh += S[(0 * MD + k) * T + li];
+      uqx += S[(1 * MD + k) * T + li];
+      uqy += S[(2 * MD + k) * T + li];
+      const double e = S[(3 * MD + k) * T + li];
+      if ((xm >> k) & 1)
+        zbb += e;
+      else
+        nb += e;
+    }
+  }
+  CellAdj ca;
+  ca.me = lds_cellq<T>(C, li, a.half_g);
+  ca.wet = ca.me.h > hsmall;
+  ca.xi = ca.me.xi;
+  ca.zb = ca.me.zb;
+  ca.n = a.n[i];
+  ca.A = 1.0;  // not used by the finish
+  ca.lt0 = a.lx[i];
+  ca.lt1 = a.lqx[i];
+  ca.lt2 = a.lqy[i];
+  if (STEP && a.mask && a.mask[i]) ca.lt0 = ca.lt1 = ca.lt2 = 0.0;
+  const double dti = a.dtc ? a.dtc[i] : a.dt;
+  ca.y0 = STEP ? dti * ca.lt0 : ca.lt0;
+  ca.y1 = STEP ? dti * ca.lt1 : ca.lt1;
+  ca.y2 = STEP ? dti * ca.lt2 : ca.lt2;
+  // the slots carry xi + h already (h-bar only where the cell is wet)
+  adj_cell_finish<STEP>(a, i, ca, uxh, 0.0, uqx, uqy, nb, zbb);
 }
 
 // Inlet conveyance adjoint (bc_2D.jl:640-694): v_f = (Q_b / A_b) L^(2/3) / n
@@ -2078,6 +2294,13 @@ int drop_graph(hg_ctx* ctx);
 // other streams join through events).
 template <class F>
 int launch_seg_graph(hg_ctx* ctx, hipStream_t stream, const SegGraphKey& key, F&& enqueue) {
+  if (!ctx->use_graphs) {  // plain launches
+    const hipStream_t saved = ctx->stream;
+    ctx->stream = stream;
+    const int rc = enqueue();
+    ctx->stream = saved;
+    return rc;
+  }
   auto it = ctx->seg_graphs.find(key);
   if (it == ctx->seg_graphs.end()) {
     hipGraph_t graph = nullptr;
@@ -2207,6 +2430,39 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
   return a;
 }
 
+// The reverse step runs tiled wherever the step does and its LDS fits.
+bool adj_tiled(const hg_ctx* ctx) {
+  return ctx->tiled && adj_tile_lds_bytes(ctx->tileT, ctx->tileMD) <= ctx->lds_max;
+}
+
+template <int T, int MD>
+void launch_adj_tile_tm(hg_ctx* ctx, const hg::AdjArgs& a) {
+  const dim3 grid(unsigned(ctx->ntiles) * unsigned(ctx->K));
+  hipLaunchKernelGGL((k_adj_tile<true, T, MD>), grid, dim3(T), adj_tile_lds_bytes(T, MD), ctx->stream, a);
+}
+
+template <int T>
+void launch_adj_tile_t(hg_ctx* ctx, const hg::AdjArgs& a) {
+  if (ctx->tileMD == 3)
+    launch_adj_tile_tm<T, 3>(ctx, a);
+  else if (ctx->tileMD == 4)
+    launch_adj_tile_tm<T, 4>(ctx, a);
+  else
+    launch_adj_tile_tm<T, 8>(ctx, a);
+}
+
+void launch_adj_tile(hg_ctx* ctx, hg::AdjArgs a) {
+  a.item_off = ctx->item_off;
+  a.items = ctx->items;
+  a.xcd_chunk = ctx->xcd_chunk;
+  if (ctx->tileT == 128)
+    launch_adj_tile_t<128>(ctx, a);
+  else if (ctx->tileT == 256)
+    launch_adj_tile_t<256>(ctx, a);
+  else
+    launch_adj_tile_t<512>(ctx, a);
+}
+
 // Cotangent of one RHS evaluation (step = false) or one step (step = true)
 // at state q, given the output cotangent lam; writes the state cotangent to
 // out and accumulates into adj_nbar / adj_zbbar / adj_Qbar / adj_S0bar.
@@ -2245,7 +2501,9 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   a.nbar_set = (ude || law || lux_n) ? 1 : 0;
   a.fbar_set = (lux && !lux_n) ? 1 : 0;
   if (totalA) a.totalA = totalA;
-  if (step)
+  if (step && adj_tiled(ctx))
+    launch_adj_tile(ctx, a);
+  else if (step)
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   else
     hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
@@ -2725,7 +2983,7 @@ int setup_tiles(hg_ctx* ctx, int T) {
 // replays a 128-step graph at half speed, so profiles would not describe
 // the unprofiled run.)
 int ensure_graph(hg_ctx* ctx, double dt) {
-  if (ctx->gexec && ctx->gdt == dt) return HG_OK;
+  if (!ctx->use_graphs || (ctx->gexec && ctx->gdt == dt)) return HG_OK;
   if (drop_graph(ctx)) return HG_ERR_HIP;
   hipGraph_t graph;
   HG_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -2752,7 +3010,7 @@ int advance_async(hg_ctx* ctx, double dt, int64_t nsteps) {
     ctx->cur = 0;
     --nsteps;
   }
-  if (nsteps >= kGraphSteps) {
+  if (ctx->use_graphs && nsteps >= kGraphSteps) {
     int rc = ensure_graph(ctx, dt);
     if (rc) return rc;
     for (; nsteps >= kGraphSteps; nsteps -= kGraphSteps) HG_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
@@ -2903,6 +3161,10 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   }
 
   hg_ctx* ctx = new hg_ctx();
+  {  // HG_GRAPHS=0: no hipGraphs (plain launches throughout; for tools that do not follow graph launches)
+    const char* e = std::getenv("HG_GRAPHS");
+    ctx->use_graphs = !(e && std::strcmp(e, "0") == 0);
+  }
   // K fields of this mesh (1 for a plain or batched context): the counts
   // below cover every field; the geometry is the mesh's (N cells, G ghosts)
   ctx->K = K;

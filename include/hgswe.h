@@ -56,7 +56,13 @@ enum {
   HG_REORDER = 1,    /* renumber cells along a Morton curve (needs centroid_xy) */
   HG_CELL_KERNEL = 2 /* one work-item per cell evaluating every face from its side, instead of the
                         default tiled kernel that evaluates intra-tile faces once for both sides
-                        (both are bit-identical to the reference's per-side evaluation) */
+                        (both are bit-identical to the reference's per-side evaluation).  The
+                        reverse step follows suit: per cell, every face's VJP from the cell's
+                        side, instead of the tiled one that forms each face's VJP for both
+                        cells from the side whose normal points to +x (or +y when n_x = 0).
+                        The two reverse steps agree to rounding, not bitwise; the tiled one's
+                        results do not depend on the tiling when the two sides' normals of a
+                        face are exact negatives (as the reference's mesh builder forms them) */
 };
 
 /* Mesh topology + geometry, flattened from mesh_2D (mesh_2D.jl:2-71).
@@ -101,7 +107,10 @@ typedef struct {
 /* Build a context: validates and copies the mesh and boundary data to the
  * device (the caller may free its arrays afterwards).  Replaces the set-up
  * of SWE2D_Extra_Parameters (solve_swe_2D.jl:240-277,
- * application_commons.jl:7-44). */
+ * application_commons.jl:7-44).  With the environment variable HG_GRAPHS=0
+ * set when it is created, the context launches every kernel directly
+ * instead of replaying cached hipGraphs (same results; for tools that do
+ * not follow graph launches). */
 int hg_create(hg_ctx** out, const hg_mesh* mesh, const hg_bcs* bcs, const hg_consts* consts, int32_t flags);
 
 int hg_destroy(hg_ctx* ctx);

@@ -155,6 +155,37 @@ def test_gradient_matches_oracle_dual_run(maker):
         assert abs(gv - dL) <= 1e-8 * abs(dL)
 
 
+@pytest.mark.parametrize("maker", [small_channel, wet_dry_case, corner_inlets_case])
+def test_tiled_reverse_step_matches_cell_reverse_step(maker):
+    """The tiled reverse step (each face's VJP formed once for both cells,
+    from the side whose normal points to +x) against the per-cell reverse
+    step (each cell forms every face's VJP from its own side): the same
+    gradient to rounding.  The tiled gradient is bitwise the same for every
+    tile size and block -> tile mapping (no face's bits depend on which
+    work-group evaluates it)."""
+    c = maker()
+    nsteps, kck = 40, 9
+    probe, target = _probe_problem(c, nsteps)
+    ref = capi.Context(c, cell_kernel=True).grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    ctx = capi.Context(c)
+    first = None
+    for T in (128, 256, 512):
+        for x in (0, 4):
+            ctx.set_tile_config(T, x)
+            out = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+            if first is None:
+                first = out
+                continue
+            assert out[0] == first[0], (T, x)
+            for a, b in zip(out[1:], first[1:]):
+                assert np.array_equal(a, b), (T, x)
+    assert first[0] == ref[0]  # the forward sweeps are bitwise the same
+    for a, b, name in zip(first[1:], ref[1:], ("n", "zb", "Q")):
+        rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        say(f"  {c.name} tiled vs per-cell reverse step [{name}]: max rel diff {rel:.2e}")
+        assert rel <= 1e-9
+
+
 def test_checkpoint_interval_invariance_and_determinism():
     c = small_channel()
     nsteps = 40
