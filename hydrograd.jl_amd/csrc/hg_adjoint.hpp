@@ -58,13 +58,15 @@ __host__ __device__ __forceinline__ void phys_vjp(double hu, double hv, double u
 // swe_2D_solvers.jl:4-164) with cotangent Fb: cotangents of both sides' raw
 // and derived quantities (unfolded; ext_to_side folds them).  A and B are
 // the flux inputs (clamped depth, masked discharge, derived u, v, s, p) as
-// make_cellq forms them.
+// make_cellq forms them.  Every field of ea and eb is assigned.
 __host__ __device__ __forceinline__ void roe_vjp_ext(const CellQ& A, const CellQ& B, double nx, double ny, double g,
                                                      double half_g, double hmin, const double Fb[3], ExtBar& ea,
                                                      ExtBar& eb) {
-  ext_zero(ea);
-  ext_zero(eb);
-  if (A.h <= hmin && B.h <= hmin) return;
+  if (A.h <= hmin && B.h <= hmin) {
+    ext_zero(ea);
+    ext_zero(eb);
+    return;
+  }
   int mode = 0;  // 0 general, 1 B mirrored from A, 2 A mirrored from B
   double hA = A.h, huA = A.hu, hvA = A.hv, uA = A.u, vA = A.v, sA = A.s;
   double hB = B.h, huB = B.hu, hvB = B.hv, uB = B.u, vB = B.v, sB = B.s;
@@ -75,27 +77,30 @@ __host__ __device__ __forceinline__ void roe_vjp_ext(const CellQ& A, const CellQ
     mode = 2;
     hA = B.h; huA = -B.hu; hvA = -B.hv; uA = -B.u; vA = -B.v; sA = B.s;
   } else if (A.h <= hmin || B.h <= hmin) {  // physical flux of the wet side, h^2 pressure
+    ext_zero(ea);
+    ext_zero(eb);
     const bool left_dry = A.h <= hmin;
     const CellQ& W = left_dry ? B : A;
     ExtBar& ew = left_dry ? eb : ea;
     const double un = nx * W.u + ny * W.v;
-    ew.hu += Fb[0] * nx + Fb[1] * un;
-    ew.hv += Fb[0] * ny + Fb[2] * un;
-    ew.u += Fb[1] * nx * W.hu + Fb[2] * nx * W.hv;
-    ew.v += Fb[1] * ny * W.hu + Fb[2] * ny * W.hv;
-    const double prb = Fb[1] * nx + Fb[2] * ny;
-    ew.h += prb * half_g * 2.0 * (W.h + EPS);
+    ew.hu = Fb[0] * nx + Fb[1] * un;
+    ew.hv = Fb[0] * ny + Fb[2] * un;
+    const double t = Fb[1] * W.hu + Fb[2] * W.hv;
+    ew.u = t * nx;
+    ew.v = t * ny;
+    ew.h = (Fb[1] * nx + Fb[2] * ny) * half_g * 2.0 * (W.h + EPS);
     return;
   }
-  // ---- forward recompute (general branch)
+  // ---- forward recompute (general branch).  sqrt(x) and 1 / sqrt(x) come
+  // from one reciprocal square root: x r and r, r = 1 / sqrt(x).
   const double hRoe = (hA + hB) * 0.5;
-  const double sAB = sA + sB;
-  const double rsAB = 1.0 / sAB;
+  const double rsAB = 1.0 / (sA + sB);
   const double uRoe = (sA * uA + sB * uB) * rsAB;
   const double vRoe = (sA * vA + sB * vB) * rsAB;
   const double un = uRoe * nx + vRoe * ny;
-  const double c = sqrt(g * hRoe + EPS);
-  const double rc = 1.0 / c;
+  const double xc = g * hRoe + EPS;
+  const double rc = rsqrt(xc);
+  const double c = xc * rc;
   const double o = 0.5 * rc;
   const double d0 = B.xi - A.xi, d1 = huB - huA, d2 = hvB - hvA;
   const double k0 = -(uRoe * ny - vRoe * nx);
@@ -103,72 +108,57 @@ __host__ __device__ __forceinline__ void roe_vjp_ext(const CellQ& A, const CellQ
   const double w1 = (un * o + 0.5) * d0 + (-nx * o) * d1 + (-ny * o) * d2;
   const double w2 = (-un * o + 0.5) * d0 + (nx * o) * d1 + (ny * o) * d2;
   const double um = un - c, up = un + c;
-  const double l0 = sqrt(un * un + EPS), l1 = sqrt(um * um + EPS), l2 = sqrt(up * up + EPS);
-  const double q0 = un / l0, q1 = um / l1, q2 = up / l2;  // d l_k / d (its argument)
-  const double a1 = l1 * w1, a2 = l2 * w2;
-  // ---- reverse: F = ((fA + fB) - r) / 2
-  const double hf[3] = {0.5 * Fb[0], 0.5 * Fb[1], 0.5 * Fb[2]};
-  ExtBar ma, mb;  // cotangents of the (possibly mirrored) flux states
-  ext_zero(ma);
-  ext_zero(mb);
-  phys_vjp(huA, hvA, uA, vA, nx, ny, hf, ma);
-  phys_vjp(huB, hvB, uB, vB, nx, ny, hf, mb);
-  const double rb0 = -hf[0], rb1 = -hf[1], rb2 = -hf[2];
-  // r0 = a1 + a2; r1 = ny a0 + (u - c nx) a1 + (u + c nx) a2; r2 = -nx a0 + (v - c ny) a1 + (v + c ny) a2
-  const double ab0 = rb1 * ny - rb2 * nx;
-  const double ab1 = rb0 + rb1 * (uRoe - c * nx) + rb2 * (vRoe - c * ny);
-  const double ab2 = rb0 + rb1 * (uRoe + c * nx) + rb2 * (vRoe + c * ny);
-  double uRb = rb1 * (a1 + a2);
-  double vRb = rb2 * (a1 + a2);
-  double cb = (rb1 * nx + rb2 * ny) * (a2 - a1);
-  // a_k = l_k w_k
-  const double l0b = ab0 * w0, l1b = ab1 * w1, l2b = ab2 * w2;
+  const double x0 = un * un + EPS, x1 = um * um + EPS, x2 = up * up + EPS;
+  const double r0 = rsqrt(x0), r1 = rsqrt(x1), r2 = rsqrt(x2);
+  const double l0 = x0 * r0, l1 = x1 * r1, l2 = x2 * r2;
+  // ---- reverse: F = ((fA + fB) - r) / 2, r = R (|Lambda| w)
+  const double hf0 = 0.5 * Fb[0], hf1 = 0.5 * Fb[1], hf2 = 0.5 * Fb[2];
+  const double ab0 = hf2 * nx - hf1 * ny;
+  const double ab1 = -(hf0 + hf1 * (uRoe - c * nx) + hf2 * (vRoe - c * ny));
+  const double ab2 = -(hf0 + hf1 * (uRoe + c * nx) + hf2 * (vRoe + c * ny));
+  const double a12 = l1 * w1 + l2 * w2;
+  double cb = (hf1 * nx + hf2 * ny) * (l1 * w1 - l2 * w2);
+  const double q1b = ab1 * w1 * (um * r1), q2b = ab2 * w2 * (up * r2);  // d l_k: cotangent times l_k'
   const double w0b = ab0 * l0, w1b = ab1 * l1, w2b = ab2 * l2;
-  double unb = l0b * q0 + l1b * q1 + l2b * q2;
-  cb += -l1b * q1 + l2b * q2;
-  // w's
+  double unb = ab0 * w0 * (un * r0) + q1b + q2b;
+  cb += q2b - q1b;
   const double k0b = w0b * d0;
+  const double wd = w2b - w1b;
   const double d0b = w0b * k0 + w1b * (un * o + 0.5) + w2b * (-un * o + 0.5);
-  const double d1b = w0b * ny - w1b * nx * o + w2b * nx * o;
-  const double d2b = -w0b * nx - w1b * ny * o + w2b * ny * o;
-  unb += (w1b - w2b) * o * d0;
-  const double ob = w1b * (un * d0 - nx * d1 - ny * d2) + w2b * (-un * d0 + nx * d1 + ny * d2);
-  uRb += -k0b * ny;
-  vRb += k0b * nx;
-  cb += -ob * o * rc;
-  const double hRb = cb * g * (0.5 * rc);
-  uRb += unb * nx;
-  vRb += unb * ny;
-  ma.s += (uRb * (uA - uRoe) + vRb * (vA - vRoe)) * rsAB;
-  mb.s += (uRb * (uB - uRoe) + vRb * (vB - vRoe)) * rsAB;
-  const double wa = sA * rsAB, wb = sB * rsAB;
-  ma.u += uRb * wa;
-  ma.v += vRb * wa;
-  mb.u += uRb * wb;
-  mb.v += vRb * wb;
-  ma.h += 0.5 * hRb;
-  mb.h += 0.5 * hRb;
-  // d = B - A (xi of both sides is never mirrored)
-  ma.xi -= d0b;
-  mb.xi += d0b;
-  ma.hu -= d1b;
-  mb.hu += d1b;
-  ma.hv -= d2b;
-  mb.hv += d2b;
-  // ---- route the mirrored copies back to their source side
+  const double d1b = w0b * ny + wd * nx * o;
+  const double d2b = -w0b * nx + wd * ny * o;
+  unb -= wd * o * d0;
+  cb += wd * (un * d0 - nx * d1 - ny * d2) * o * rc;
+  const double uRb = -hf1 * a12 - k0b * ny + unb * nx;
+  const double vRb = -hf2 * a12 + k0b * nx + unb * ny;
+  const double hRb2 = 0.5 * (cb * g * (0.5 * rc));  // half of hRoe's cotangent reaches each side's h
+  const double pb = hf1 * nx + hf2 * ny;
+  // the flux states' cotangents (d = B - A; xi is never mirrored)
+  const double unA = nx * uA + ny * vA, tA = hf1 * huA + hf2 * hvA;
+  const double unB = nx * uB + ny * vB, tB = hf1 * huB + hf2 * hvB;
+  ea.xi = -d0b;
+  ea.p = pb;
+  ea.hu = hf0 * nx + hf1 * unA - d1b;
+  ea.hv = hf0 * ny + hf2 * unA - d2b;
+  ea.u = tA * nx + uRb * sA * rsAB;
+  ea.v = tA * ny + vRb * sA * rsAB;
+  ea.s = (uRb * (uA - uRoe) + vRb * (vA - vRoe)) * rsAB;
+  ea.h = hRb2;
+  eb.xi = d0b;
+  eb.p = pb;
+  eb.hu = hf0 * nx + hf1 * unB + d1b;
+  eb.hv = hf0 * ny + hf2 * unB + d2b;
+  eb.u = tB * nx + uRb * sB * rsAB;
+  eb.v = tB * ny + vRb * sB * rsAB;
+  eb.s = (uRb * (uB - uRoe) + vRb * (vB - vRoe)) * rsAB;
+  eb.h = hRb2;
+  // ---- route a mirrored copy back to its source side
   if (mode == 1) {  // hB = hA, huB = -huA, hvB = -hvA, uB = -uA, vB = -vA, sB = sA
-    ea = ma;
-    ea.h += mb.h; ea.hu -= mb.hu; ea.hv -= mb.hv; ea.u -= mb.u; ea.v -= mb.v; ea.s += mb.s;
-    eb.xi = mb.xi;
-    eb.p = mb.p;
+    ea.h += eb.h; ea.hu -= eb.hu; ea.hv -= eb.hv; ea.u -= eb.u; ea.v -= eb.v; ea.s += eb.s;
+    eb.h = eb.hu = eb.hv = eb.u = eb.v = eb.s = 0.0;
   } else if (mode == 2) {
-    eb = mb;
-    eb.h += ma.h; eb.hu -= ma.hu; eb.hv -= ma.hv; eb.u -= ma.u; eb.v -= ma.v; eb.s += ma.s;
-    ea.xi = ma.xi;
-    ea.p = ma.p;
-  } else {
-    ea = ma;
-    eb = mb;
+    eb.h += ea.h; eb.hu -= ea.hu; eb.hv -= ea.hv; eb.u -= ea.u; eb.v -= ea.v; eb.s += ea.s;
+    ea.h = ea.hu = ea.hv = ea.u = ea.v = ea.s = 0.0;
   }
 }
 

@@ -846,27 +846,48 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(hg::AdjArgs a0) {
   adj_cell_finish<STEP>(a, i, ca, U.xi + gxi, U.h + gh, U.hu + gqx, U.hv + gqy, nb, zbb);
 }
 
+// What the finish reads of one cell besides its flux inputs and output
+// cotangent: n, friction magnitude, S0, and the sums it adds to.
+struct FinishIn {
+  double n, fmag, S0x, S0y, fbar, nbar, zbbar, S0bx, S0by;
+};
+
+__device__ __forceinline__ FinishIn adj_finish_load(const hg::AdjArgs& a, int i) {
+  FinishIn f;
+  f.n = a.n[i];
+  f.fmag = a.fmag ? a.fmag[i] : 0.0;
+  f.S0x = a.S0x[i];
+  f.S0y = a.S0y[i];
+  f.fbar = (a.fmag && !a.fbar_set) ? a.fbar[i] : 0.0;
+  f.nbar = (a.nbar && !a.nbar_set) ? a.nbar[i] : 0.0;
+  f.zbbar = a.zbbar ? a.zbbar[i] : 0.0;
+  f.S0bx = a.S0bar ? a.S0bar[i] : 0.0;
+  f.S0by = a.S0bar ? a.S0bar[a.N + i] : 0.0;
+  return f;
+}
+
 // The source VJP, the clamp and the identity part of the step, and the
-// writes of one cell, given its summed face cotangents.
+// writes of one cell, given its summed face cotangents and its FinishIn.
 template <bool STEP>
-__device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, const CellAdj& ca, double xib,
-                                                double hb, double qxb, double qyb, double nb, double zbb) {
+__device__ __forceinline__ void adj_finish_store(const hg::AdjArgs& a, int i, const CellAdj& ca, const FinishIn& f,
+                                                 double xib, double hb, double qxb, double qyb, double nb,
+                                                 double zbb) {
   const hg::CellQ& me = ca.me;
   double S0xb = 0.0, S0yb = 0.0;
   if (a.fmag) {
     double mb = 0.0;
     if (ca.wet)
-      hg::source_vjp_mag(a.g, ca.xi, me.hu, me.hv, a.fmag[i], a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, qxb, qyb, mb,
-                         S0xb, S0yb);
+      hg::source_vjp_mag(a.g, ca.xi, me.hu, me.hv, f.fmag, f.S0x, f.S0y, ca.y1, ca.y2, xib, qxb, qyb, mb, S0xb,
+                         S0yb);
     // this step's cotangent of the friction magnitude: set (the sum with 0.0
     // gives the bits of an add into a cleared array), or summed over the steps
     if (a.fbar_set)
       a.fbar[i] = mb + 0.0;
     else if (ca.wet)
-      a.fbar[i] += mb;
+      a.fbar[i] = f.fbar + mb;
   } else if (ca.wet) {
-    hg::source_vjp(a.g, a.kn2, a.hsmall, ca.xi, me.h, me.hu, me.hv, ca.n, a.S0x[i], a.S0y[i], ca.y1, ca.y2, xib, hb,
-                   qxb, qyb, nb, S0xb, S0yb);
+    hg::source_vjp(a.g, a.kn2, a.hsmall, ca.xi, me.h, me.hu, me.hv, f.n, f.S0x, f.S0y, ca.y1, ca.y2, xib, hb, qxb,
+                   qyb, nb, S0xb, S0yb);
   }
   // clamped depth / masked discharge are constants (semi_discretize_swe_2D.jl:101-106)
   double ox = xib, oqx = 0.0, oqy = 0.0;
@@ -883,12 +904,18 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
   a.ox[i] = ox;
   a.oqx[i] = oqx;
   a.oqy[i] = oqy;
-  if (a.nbar) a.nbar[i] = a.nbar_set ? nb + 0.0 : a.nbar[i] + nb;
-  if (a.zbbar) a.zbbar[i] += zbb;
+  if (a.nbar) a.nbar[i] = a.nbar_set ? nb + 0.0 : f.nbar + nb;
+  if (a.zbbar) a.zbbar[i] = f.zbbar + zbb;
   if (a.S0bar) {  // summed over the steps; the bed-slope transpose is applied once at the end
-    a.S0bar[i] += S0xb;
-    a.S0bar[a.N + i] += S0yb;
+    a.S0bar[i] = f.S0bx + S0xb;
+    a.S0bar[a.N + i] = f.S0by + S0yb;
   }
+}
+
+template <bool STEP>
+__device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, const CellAdj& ca, double xib,
+                                                double hb, double qxb, double qyb, double nb, double zbb) {
+  adj_finish_store<STEP>(a, i, ca, adj_finish_load(a, i), xib, hb, qxb, qyb, nb, zbb);
 }
 
 // Tiled reverse step (production path on meshes the tiled step kernel
@@ -1043,6 +1070,17 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
       S[(2 * MD + kq) * T + iq] = sQ.hv;
     }
   }
+  // the finish's own loads, in flight while the other waves finish their items
+  FinishIn fin{};
+  double lt0 = 0.0, lt1 = 0.0, lt2 = 0.0, dti = 0.0;
+  if (valid) {
+    fin = adj_finish_load(a, i);
+    lt0 = a.lx[i];
+    lt1 = a.lqx[i];
+    lt2 = a.lqy[i];
+    if (STEP && a.mask && a.mask[i]) lt0 = lt1 = lt2 = 0.0;
+    dti = a.dtc ? a.dtc[i] : a.dt;
+  }
   __syncthreads();
 
   if (!valid) return;
@@ -1065,18 +1103,16 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
   ca.wet = ca.me.h > hsmall;
   ca.xi = ca.me.xi;
   ca.zb = ca.me.zb;
-  ca.n = a.n[i];
+  ca.n = fin.n;
   ca.A = 1.0;  // not used by the finish
-  ca.lt0 = a.lx[i];
-  ca.lt1 = a.lqx[i];
-  ca.lt2 = a.lqy[i];
-  if (STEP && a.mask && a.mask[i]) ca.lt0 = ca.lt1 = ca.lt2 = 0.0;
-  const double dti = a.dtc ? a.dtc[i] : a.dt;
-  ca.y0 = STEP ? dti * ca.lt0 : ca.lt0;
-  ca.y1 = STEP ? dti * ca.lt1 : ca.lt1;
-  ca.y2 = STEP ? dti * ca.lt2 : ca.lt2;
+  ca.lt0 = lt0;
+  ca.lt1 = lt1;
+  ca.lt2 = lt2;
+  ca.y0 = STEP ? dti * lt0 : lt0;
+  ca.y1 = STEP ? dti * lt1 : lt1;
+  ca.y2 = STEP ? dti * lt2 : lt2;
   // the slots carry xi + h already (h-bar only where the cell is wet)
-  adj_cell_finish<STEP>(a, i, ca, uxh, 0.0, uqx, uqy, nb, zbb);
+  adj_finish_store<STEP>(a, i, ca, fin, uxh, 0.0, uqx, uqy, nb, zbb);
 }
 
 // Inlet conveyance adjoint (bc_2D.jl:640-694): v_f = (Q_b / A_b) L^(2/3) / n
