@@ -158,9 +158,10 @@ def measure_gradient(args, capi):
         print(f"# gradient autotune: {T}-cell tiles, xcd_chunk={x}", file=sys.stderr, flush=True)
     QT = ctx.step(c.Q0, c.dt, args.grad_steps, params=n_true)
     target = QT[probe] + c.hstill[probe] + c.zb[probe]
-    ctx.grad_probes(c.Q0, c.dt, 20, args.grad_kck, probe, target, params=c.n)  # warm-up
+    ctx.grad_probes(c.Q0, c.dt, 20, args.grad_kck, probe, target, params=c.n, wrt=("n",))  # warm-up
     t0 = time.perf_counter()
-    loss, gn, gzb, gq = ctx.grad_probes(c.Q0, c.dt, args.grad_steps, args.grad_kck, probe, target, params=c.n)
+    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, args.grad_steps, args.grad_kck, probe, target, params=c.n,
+                                     wrt=("n",))
     el = time.perf_counter() - t0
     out = {"value": c.N * args.grad_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
            "config": {"workload": "channel_100k_tri_probe_loss_gradient", "cells": c.N, "steps": args.grad_steps,
@@ -194,9 +195,9 @@ def measure_gradient_full(args, capi, case, tile):
     steps = args.grad_1m_steps
     QT = ctx.step(c.Q0, c.dt, steps, params=n_true)
     target = QT[probe] + c.hstill[probe] + c.zb[probe]
-    ctx.grad_probes(c.Q0, c.dt, 100, args.grad_kck, probe, target, params=c.n)  # warm-up
+    ctx.grad_probes(c.Q0, c.dt, 100, args.grad_kck, probe, target, params=c.n, wrt=("n",))  # warm-up
     t0 = time.perf_counter()
-    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, args.grad_kck, probe, target, params=c.n)
+    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, args.grad_kck, probe, target, params=c.n, wrt=("n",))
     el = time.perf_counter() - t0
     out = {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
            "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": steps,
@@ -213,9 +214,9 @@ def stored_gradient(ctx, c, steps, probe, target):
     interval = steps): the forward sweep keeps every state, nothing is
     recomputed (bitwise the checkpointed gradient; (steps + 1) 3N doubles of
     device memory)."""
-    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n)  # warm-up (allocates the trajectory)
+    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n, wrt=("n",))  # warm-up (allocates)
     t0 = time.perf_counter()
-    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n)
+    ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n, wrt=("n",))
     el = time.perf_counter() - t0
     avg_deg = float(c.mesh.face_off[-1]) / c.N
     bpc = bytes_per_cell_step(c.mesh) + 24 + (24 + 24 + 24 + 48 + 28 * avg_deg + 16)  # forward + store + reverse
@@ -321,9 +322,9 @@ def measure_ude(args, capi):
         # probe loss (200 probes per instance) w.r.t. its 353 weights
         QB = bctx.step(B.Q0, B.dt, args.ude_steps)
         btarget = QB[bprobe] + B.hstill[bprobe] + B.zb[bprobe] + noise
-        bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget)  # warm-up
+        bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget, wrt=())  # warm-up
         t0 = time.perf_counter()
-        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget)
+        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget, wrt=())  # the weights only
         gws = bctx.ude_grad()
         el = time.perf_counter() - t0
         del bctx
@@ -337,9 +338,9 @@ def measure_ude(args, capi):
     probe = rng.choice(c.N, size=200, replace=False).astype(np.int32)
     QN = ctx.step(c.Q0, c.dt, args.ude_steps)
     target = QN[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.02, probe.size)
-    ctx.grad_probes(c.Q0, c.dt, 10, 5, probe, target)  # warm-up
+    ctx.grad_probes(c.Q0, c.dt, 10, 5, probe, target, wrt=())  # warm-up
     t0 = time.perf_counter()
-    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.grad_kck, probe, target)
+    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.grad_kck, probe, target, wrt=())  # the weights only
     gw = ctx.ude_grad()
     el = time.perf_counter() - t0
     out["gradient"] = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
@@ -378,9 +379,9 @@ def measure_fields(args, capi):
              "achieved_GBps": bpc * B.N / (ms_k * 1e-3) / 1e9}
         QN = k.step(B.Q0, B.dt, args.ude_steps)
         target = QN[probe] + B.hstill[probe] + B.zb[probe] + noise
-        k.grad_probes(B.Q0, B.dt, 10, 5, probe, target)  # warm-up
+        k.grad_probes(B.Q0, B.dt, 10, 5, probe, target, wrt=("n",))  # warm-up
         t0 = time.perf_counter()
-        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, probe, target)
+        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, probe, target, wrt=("n",))
         el = time.perf_counter() - t0
         r["gradient"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
                          "wrt": "each field's own per-cell n"}

@@ -261,10 +261,12 @@ struct SegGraphKey {
   int kind, b, lam0;
   int64_t ns;
   double dt;
+  int want;  // the parameter cotangents the reversal accumulates (bit 0: n, bit 1: zb)
   bool operator<(const SegGraphKey& o) const {
     if (kind != o.kind) return kind < o.kind;
     if (b != o.b) return b < o.b;
     if (lam0 != o.lam0) return lam0 < o.lam0;
+    if (want != o.want) return want < o.want;
     if (ns != o.ns) return ns < o.ns;
     return dt < o.dt;
   }
@@ -338,6 +340,10 @@ struct hg_ctx {
   double* red = nullptr;      // reduction partials
   double* noalloc_scratch = nullptr;  // device address of the static scratch of hg_cfl_dt_noalloc on this device
 
+  // parameter cotangents the current adjoint call accumulates: those of the
+  // per-cell n and of zb (through S0 and the exit-H depth) are skipped when
+  // the caller does not ask for them
+  bool adj_want_n = true, adj_want_zb = true;
   // adjoint work buffers (allocated on first use)
   double* adj_lam[2] = {nullptr, nullptr};  // 3N each
   double *adj_nbar = nullptr, *adj_zbbar = nullptr, *adj_S0bar = nullptr, *adj_vbar = nullptr, *adj_Qbar = nullptr;

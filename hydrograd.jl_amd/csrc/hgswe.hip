@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
     const double* n = n_all + o;
     const double* vbar = vbar_all + int64_t(fl) * Gm;
     double* ox = ox_all + o;
-    double* nbar = nbar_all + o;
+    double* nbar = nbar_all ? nbar_all + o : nullptr;
     const double A = totalA[b];
     double pa = 0.0, pq = 0.0;
     for (int f = off[bl] + tid; f < off[bl + 1]; f += kBlock) {
@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
         const double h = xi[c] + hstill[c];
         if (!(h > hsmall)) continue;  // w = 0 and h clamped: no derivative
         ox[c] += Ab * L53[g] / n[c];
-        nbar[c] += Ab * (-L53[g] * h / (n[c] * n[c]));
+        if (nbar_all) nbar[c] += Ab * (-L53[g] * h / (n[c] * n[c]));
       }
     }
     __syncthreads();  // sA / sQ reuse; cells shared by two inlets take them in inlet order
@@ -2431,8 +2431,8 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
   a.mask = mask;
   a.ox = out[0]; a.oqx = out[1]; a.oqy = out[2];
   a.nbar = ctx->adj_nbar;
-  a.zbbar = ctx->adj_zbbar;
-  a.S0bar = ctx->adj_S0bar;
+  a.zbbar = ctx->adj_want_zb ? ctx->adj_zbbar : nullptr;
+  a.S0bar = ctx->adj_want_zb ? ctx->adj_S0bar : nullptr;
   a.vbar = ctx->adj_vbar;
   a.hstill = ctx->hstill;
   a.zb = ctx->zb;
@@ -2513,7 +2513,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   const bool lux = ctx->lux.choice != hg::LUX_OFF;
   const bool lux_n = lux && ctx->lux.choice != hg::LUX_FLOW_RESISTANCE;  // the network gives n
   // with a state-dependent n, n-bar of this step only
-  double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : lux_n ? ctx->lux_obar : ctx->adj_nbar;
+  double* nbar = ude ? ctx->ude_nbar : law ? ctx->nlaw_nbar : lux_n ? ctx->lux_obar
+                                                           : (ctx->adj_want_n ? ctx->adj_nbar : nullptr);
   // the closure's, law's or network's n-bar (or friction-magnitude bar) of
   // this step is set by the adjoint kernel, not cleared and added to
   if (ude && !n_ready) {
@@ -2595,7 +2596,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
 // summed its share over the steps into its own slot; the slots are summed
 // here by a fixed tree.
 int finish_adjoint(hg_ctx* ctx) {
-  hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), int(ctx->Nm),
+  if (ctx->adj_want_zb)
+    hipLaunchKernelGGL(k_bed_adj, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, int(ctx->N), int(ctx->Nm),
                      ctx->deg, ctx->nbr, ctx->fnx, ctx->fny, ctx->flen, ctx->area, ctx->adj_S0bar, ctx->adj_zbbar);
   if (ctx->ude_prec)
     hipLaunchKernelGGL(k_ude_wsum, dim3(hg::kUdeW, ctx->ude_nsets), dim3(kBlock), 0, ctx->stream, ctx->ude_nslots,
@@ -2617,9 +2619,11 @@ int zero_adj_accumulators(hg_ctx* ctx) {
     HG_TRY(hipMemsetAsync(ctx->lux_wpart, 0, size_t(ctx->lux.P) * lux_vjp_blocks(ctx) * sizeof(double),
                           ctx->stream));
   }
-  HG_TRY(hipMemsetAsync(ctx->adj_S0bar, 0, 2 * ctx->N * sizeof(double), ctx->stream));
-  HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
-  HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
+  if (ctx->adj_want_zb) {
+    HG_TRY(hipMemsetAsync(ctx->adj_S0bar, 0, 2 * ctx->N * sizeof(double), ctx->stream));
+    HG_TRY(hipMemsetAsync(ctx->adj_zbbar, 0, ctx->N * sizeof(double), ctx->stream));
+  }
+  if (ctx->adj_want_n) HG_TRY(hipMemsetAsync(ctx->adj_nbar, 0, ctx->N * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_Qbar, 0, std::max<int32_t>(ctx->n_inlet, 1) * sizeof(double), ctx->stream));
   HG_TRY(hipMemsetAsync(ctx->adj_vbar, 0, std::max<int64_t>(ctx->G, 1) * sizeof(double), ctx->stream));
   return HG_OK;
@@ -2688,7 +2692,7 @@ int recompute_segment(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int64
   ctx->stream = main;
   if (rc) return rc;
   // the ns steps from the segment's first state: one cached graph per (b, ns, dt)
-  return launch_seg_graph(ctx, stream, SegGraphKey{0, b, 0, ns, dt}, [&] {
+  return launch_seg_graph(ctx, stream, SegGraphKey{0, b, 0, ns, dt, 0}, [&] {
     double* const n_field = ctx->n;
     int r = HG_OK;
     for (int64_t t = 0; t < ns && !r; ++t) {
@@ -2732,7 +2736,7 @@ int reverse_sweep(hg_ctx* ctx, double dt, int64_t nsteps, int64_t kck, int& lam_
     const int64_t t0 = sgi * kck, ns = std::min<int64_t>(kck, nsteps - t0);
     // the segment's reversal: one cached graph per (b, starting cotangent buffer, ns, dt)
     const int lam0 = lam_cur;
-    rc = launch_seg_graph(ctx, ctx->stream, SegGraphKey{1, b, lam0, ns, dt}, [&] {
+    rc = launch_seg_graph(ctx, ctx->stream, SegGraphKey{1, b, lam0, ns, dt, (ctx->adj_want_n ? 1 : 0) | (ctx->adj_want_zb ? 2 : 0)}, [&] {
       int lc = lam0, r = HG_OK;
       for (int64_t t = ns - 1; t >= 0 && !r; --t) {
         const double* q[3] = {seg_state(ctx, t, 0, b), seg_state(ctx, t, 1, b), seg_state(ctx, t, 2, b)};
@@ -4775,6 +4779,8 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
                double* Q_bar, double* n_bar, double* zb_bar, double* inletQ_bar) {
   if (!ctx || !Q || !dQdt_bar) return ctx ? (ctx->err = "hg_rhs_vjp: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
   const int64_t N = ctx->N;
+  ctx->adj_want_n = n_bar != nullptr;  // cotangents nobody asked for are not accumulated
+  ctx->adj_want_zb = zb_bar != nullptr;
   int rc = hg_set_params(ctx, params, n_params);
   if (!rc) rc = ensure_adj(ctx, 1, 1);
   if (!rc) rc = state_in(ctx, Q, 0);
@@ -4790,6 +4796,23 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
   if (!rc) rc = adj_download(ctx, ctx->adj_lam[1], Q_bar, n_bar, zb_bar, inletQ_bar);
   if (!rc) rc = check_flags(ctx);
   return rc;
+}
+
+// The checkpoint interval the library picks for kck <= 0: the whole run as
+// one stored segment (nothing recomputed) when its states, clamp flags and
+// recorded conveyances (and the closure's n) take at most half of the device
+// memory free for them (the context's segment buffers count as free), else
+// ceil(sqrt(nsteps)) (two segment buffers and nsteps / kck checkpoints:
+// about 3 sqrt(nsteps) states).
+int64_t auto_kck(hg_ctx* ctx, int64_t nsteps) {
+  const double N = double(ctx->N);
+  const double per_step = 24.0 * N + N + 8.0 * std::max<int32_t>(ctx->n_inlet, 1) + (ctx->ude_prec ? 8.0 * N : 0.0);
+  size_t fr = 0, tot = 0;
+  double avail = 0.0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess) avail = double(fr);
+  for (int b = 0; b < 2; ++b) avail += double(ctx->adj_seg_cap[b]) * per_step;
+  if (double(nsteps) * per_step + 24.0 * N <= 0.5 * avail) return nsteps;
+  return std::max<int64_t>(1, int64_t(std::ceil(std::sqrt(double(nsteps)))));
 }
 
 static int adj_run(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
@@ -4811,6 +4834,9 @@ int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_pa
                int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar,
                double* inletQ_bar) {
   if (!ctx || !Q0 || !QN_bar) return ctx ? (ctx->err = "hg_adjoint: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  ctx->adj_want_n = n_bar != nullptr;  // cotangents nobody asked for are not accumulated
+  ctx->adj_want_zb = zb_bar != nullptr;
+  if (kck <= 0 && nsteps > 0) kck = auto_kck(ctx, nsteps);
   int rc = adj_run(ctx, Q0, params, n_params, dt, nsteps, kck);
   if (!rc) rc = adj_upload_lam(ctx, QN_bar, ctx->adj_lam[0]);
   if (!rc) rc = zero_adj_accumulators(ctx);
@@ -4828,6 +4854,9 @@ int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
   if (!ctx || !Q0 || (n_probe > 0 && (!probe || !target)) || !loss)
     return ctx ? (ctx->err = "hg_grad_probes: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
   const int64_t N = ctx->N;
+  ctx->adj_want_n = n_bar != nullptr;  // cotangents nobody asked for are not accumulated
+  ctx->adj_want_zb = zb_bar != nullptr;
+  if (kck <= 0 && nsteps > 0) kck = auto_kck(ctx, nsteps);
   for (int64_t k = 0; k < n_probe; ++k)
     if (probe[k] < 0 || probe[k] >= N) {
       ctx->err = "probe cell out of range";

@@ -314,20 +314,21 @@ class Context:
                                    _ptr(ib)))
         return qb, nb, zb, ib[: self.case.mesh.n_inlet]
 
-    def grad_probes(self, Q0, dt, nsteps, kck, probe, target, params=None):
-        """(loss, d/dn, d/dzb, d/dQ_inlet) of sum_p (WSE_p - target_p)^2 after nsteps."""
+    def grad_probes(self, Q0, dt, nsteps, kck, probe, target, params=None, wrt=("n", "zb", "Q")):
+        """(loss, d/dn, d/dzb, d/dQ_inlet) of sum_p (WSE_p - target_p)^2 after
+        nsteps; a cotangent not named in `wrt` is not formed (None)."""
         Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
         pr = np.ascontiguousarray(probe, dtype=np.int32)
         tg = np.ascontiguousarray(target, dtype=np.float64)
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         loss = C.c_double()
-        nb = np.empty(self.N)
-        zb = np.empty(self.N)
+        nb = np.empty(self.N) if "n" in wrt else None
+        zb = np.empty(self.N) if "zb" in wrt else None
         ib = np.zeros(max(1, self.case.mesh.n_inlet))
         self._chk(lib().hg_grad_probes(self.h, _ptr(Q0), _ptr(p), 0 if p is None else p.size, C.c_double(dt),
                                        C.c_int64(nsteps), C.c_int64(kck), C.c_int64(pr.size), _ptr(pr), _ptr(tg),
                                        C.byref(loss), _ptr(nb), _ptr(zb), _ptr(ib)))
-        return loss.value, nb, zb, ib[: self.case.mesh.n_inlet]
+        return loss.value, nb, zb, (ib[: self.case.mesh.n_inlet] if "Q" in wrt else None)
 
     def prepare(self, dt):
         """Instantiates the stepping graph for dt and the dispatch events

@@ -273,14 +273,20 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
  * custom_ODE_solvers.jl:5-33) from Q0: forward sweep storing a checkpoint
  * every kck steps, then a reverse sweep that recomputes each segment from
  * its checkpoint.  QN_bar is the cotangent of the final state; outputs are
- * the cotangents of Q0, n, zb and the inlet discharges (NULL to skip). */
+ * the cotangents of Q0, n, zb and the inlet discharges (NULL to skip).
+ * kck <= 0 lets the library choose: the whole trajectory kept on the device
+ * (nothing recomputed) when it takes at most half of the free device memory,
+ * else ceil(sqrt(nsteps)).  Results are bitwise the same for every kck. */
 int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
                int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar, double* inletQ_bar);
 
 /* Gradient of the probe loss L = sum_p (xi_p + h_still_p + zb_p - target_p)^2
  * at the final state (the WSE misfit of swe_2D_inversion.jl:387-399 without
  * its normalisation) after nsteps steps from Q0, with respect to n, zb and
- * the inlet discharges; one checkpointed forward + reverse sweep. */
+ * the inlet discharges; one checkpointed forward + reverse sweep.  As in
+ * hg_adjoint and hg_rhs_vjp, a NULL n_bar or zb_bar skips the accumulation of
+ * that cotangent (the reverse step then reads and writes less); kck as for
+ * hg_adjoint. */
 int hg_grad_probes(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
                    int64_t kck, int64_t n_probe, const int32_t* probe, const double* target, double* loss,
                    double* n_bar, double* zb_bar, double* inletQ_bar);

@@ -186,13 +186,35 @@ def test_tiled_reverse_step_matches_cell_reverse_step(maker):
         assert rel <= 1e-9
 
 
+def test_unrequested_cotangents_are_skipped_without_changing_the_others():
+    """A gradient asked only for d/dn (or only for d/dzb and d/dQ) gives
+    bitwise the loss and cotangents of the full request; alternating the
+    requests on one context (its cached reversal graphs are kept per
+    request) changes nothing."""
+    c = small_channel()
+    nsteps, kck = 40, 9
+    probe, target = _probe_problem(c, nsteps)
+    ctx = capi.Context(c)
+    full = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    only_n = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target, wrt=("n",))
+    assert only_n[2] is None and only_n[3] is None
+    assert only_n[0] == full[0] and np.array_equal(only_n[1], full[1])
+    no_n = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target, wrt=("zb", "Q"))
+    assert no_n[1] is None
+    assert np.array_equal(no_n[2], full[2]) and np.array_equal(no_n[3], full[3])
+    again = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    assert again[0] == full[0]
+    for a, b in zip(again[1:], full[1:]):
+        assert np.array_equal(a, b)
+
+
 def test_checkpoint_interval_invariance_and_determinism():
     c = small_channel()
     nsteps = 40
     probe, target = _probe_problem(c, nsteps)
     ctx = capi.Context(c)
     ref = ctx.grad_probes(c.Q0, c.dt, nsteps, nsteps, probe, target)
-    for kck in (1, 7, 13):
+    for kck in (1, 7, 13, 0):  # 0: the library's choice (here the whole trajectory)
         out = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
         assert out[0] == ref[0]
         for a, b in zip(out[1:], ref[1:]):
