@@ -940,21 +940,126 @@ __device__ __forceinline__ void adj_cell_finish(const hg::AdjArgs& a, int i, con
 //     marks exit-H slots);
 //  3. each cell sums its slots in local face order and finishes as
 //     k_adj_cell does (source VJP, identity part of the step).
-// LDS: (14 + 4 MD) doubles and one int per cell, 53 KiB for 256 triangles:
-// three work-groups per CU, as many as the registers allow.
-constexpr int kAdjRows = 14;  // CellQ without p (9), z (3), 1 / h, 0.5 / s
+// LDS: (13 + 4 MD) doubles and one int per cell, 51 KiB for 256 triangles:
+// three work-groups per CU, as many as the registers allow.  (At 53 KiB and
+// more per work-group only two of them run on a CU at once on MI355X.)
+constexpr int kAdjRows = 13;  // CellQ without h and p (8), z (3), 1 / h, 0.5 / s
 
 size_t adj_tile_lds_bytes(int T, int MD) {
   return size_t(kAdjRows * T + 4 * MD * T) * sizeof(double) + size_t(T) * sizeof(int32_t);
 }
 
+// Rows: 0 xi, 1 h_still, 2 hu, 3 hv, 4 zb, 5 u, 6 v, 7 s, 8-10 z, 11 1 / h,
+// 12 0.5 / s.  The depth and the pressure are re-formed where read, with
+// make_cellq's operations.
 template <int T>
-__device__ __forceinline__ hg::CellQ lds_cellq(const double* C, int r, double half_g) {
+__device__ __forceinline__ hg::CellQ lds_cellq(const double* C, int r, double half_g, double hsmall) {
   hg::CellQ q;
-  q.xi = C[0 * T + r]; q.hs = C[1 * T + r]; q.h = C[2 * T + r]; q.hu = C[3 * T + r]; q.hv = C[4 * T + r];
-  q.zb = C[5 * T + r]; q.u = C[6 * T + r]; q.v = C[7 * T + r]; q.s = C[8 * T + r];
-  q.p = half_g * ((q.xi + hg::EPS) * (q.xi + hg::EPS) + 2.0 * q.xi * q.hs);  // as make_cellq forms it
+  q.xi = C[0 * T + r]; q.hs = C[1 * T + r]; q.hu = C[2 * T + r]; q.hv = C[3 * T + r];
+  q.zb = C[4 * T + r]; q.u = C[5 * T + r]; q.v = C[6 * T + r]; q.s = C[7 * T + r];
+  double h = q.xi + q.hs;
+  if (h <= hsmall) h = hsmall;
+  q.h = h;
+  q.p = half_g * ((q.xi + hg::EPS) * (q.xi + hg::EPS) + 2.0 * q.xi * q.hs);
   return q;
+}
+
+// A cross-tile neighbour's raw data as a work item of the tiled reverse step
+// reads it.
+struct NbrRaw {
+  double xi, hs, qx, qy, zb, m0, m1, m2, dt, area;
+  uint8_t clamped;
+};
+
+template <bool STEP>
+__device__ __forceinline__ NbrRaw nbr_load(const hg::AdjArgs& a, int j) {
+  NbrRaw r;
+  r.xi = a.xi[j]; r.hs = a.hstill[j]; r.qx = a.qx[j]; r.qy = a.qy[j]; r.zb = a.zb[j];
+  r.m0 = a.lx[j]; r.m1 = a.lqx[j]; r.m2 = a.lqy[j];
+  r.clamped = (STEP && a.mask) ? a.mask[j] : uint8_t(0);
+  r.dt = a.dtc ? a.dtc[j] : a.dt;
+  r.area = a.area[j];
+  return r;
+}
+
+// One work item of the tiled reverse step (see k_adj_tile); nr is the
+// neighbour of a cross-tile item.
+template <bool STEP, int T, int MD>
+__device__ __forceinline__ void adj_item(const hg::AdjArgs& a, int t, const double* C, double* S, int32_t* Xm,
+                                         const hg::ItemRec& rec, const NbrRaw& nr) {
+  const double hsmall = a.hsmall, half_g = a.half_g;
+  const int code = rec.code;
+  const int ia = code & 1023, ka = (code >> 10) & 15;
+  const int ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
+  const int type = (code >> 28) & 3;
+  const double nx = rec.nx, ny = rec.ny, L = rec.len;
+  // P / Q: the evaluation's first / second side (normal from P)
+  const bool sw = type != 2 && !(nx > 0.0 || (nx == 0.0 && ny > 0.0));
+  const double ex = sw ? -nx : nx, ey = sw ? -ny : ny;
+  hg::CellQ P, Q;
+  double zP0, zP1, zP2, zQ0, zQ1, zQ2, rhP, rhQ, r2P, r2Q;
+  int ip = ia, kp = ka, iq = ia, kq = ka;  // LDS slots of P and Q (type 0); the tile's cell otherwise
+  GhostAdj gs{};
+  double n_a = 0.0;
+  if (type == 0) {
+    if (sw) { ip = ib; kp = kb; } else { iq = ib; kq = kb; }
+    P = lds_cellq<T>(C, ip, half_g, hsmall);
+    Q = lds_cellq<T>(C, iq, half_g, hsmall);
+    zP0 = C[8 * T + ip]; zP1 = C[9 * T + ip]; zP2 = C[10 * T + ip];
+    zQ0 = C[8 * T + iq]; zQ1 = C[9 * T + iq]; zQ2 = C[10 * T + iq];
+    rhP = C[11 * T + ip]; r2P = C[12 * T + ip];
+    rhQ = C[11 * T + iq]; r2Q = C[12 * T + iq];
+  } else {
+    const hg::CellQ Lc = lds_cellq<T>(C, ia, half_g, hsmall);
+    const double zL0 = C[8 * T + ia], zL1 = C[9 * T + ia], zL2 = C[10 * T + ia];
+    const double rhL = C[11 * T + ia], r2L = C[12 * T + ia];
+    hg::CellQ G;
+    double zG0 = 0.0, zG1 = 0.0, zG2 = 0.0;
+    if (type == 1) {  // cross-tile neighbour
+      G = hg::make_cellq(nr.xi, nr.hs, nr.qx, nr.qy, nr.zb, half_g, hsmall);
+      double m0 = nr.m0, m1 = nr.m1, m2 = nr.m2;
+      if (nr.clamped) m0 = m1 = m2 = 0.0;
+      const double rAj = 1.0 / nr.area;
+      zG0 = (STEP ? nr.dt * m0 : m0) * rAj;
+      zG1 = (STEP ? nr.dt * m1 : m1) * rAj;
+      zG2 = (STEP ? nr.dt * m2 : m2) * rAj;
+    } else {  // ghost (no cotangent of its own)
+      n_a = a.n[t * T + ia];
+      G = adj_ghost_state(a, Lc, n_a, rec.ref, nx, ny, gs);
+    }
+    const double rhG = 1.0 / G.h, r2G = 0.5 / G.s;
+    P = sw ? G : Lc;
+    Q = sw ? Lc : G;
+    zP0 = sw ? zG0 : zL0; zP1 = sw ? zG1 : zL1; zP2 = sw ? zG2 : zL2;
+    zQ0 = sw ? zL0 : zG0; zQ1 = sw ? zL1 : zG1; zQ2 = sw ? zL2 : zG2;
+    rhP = sw ? rhG : rhL; r2P = sw ? r2G : r2L;
+    rhQ = sw ? rhL : rhG; r2Q = sw ? r2L : r2G;
+  }
+  const double c[3] = {L * zQ0 - L * zP0, L * zQ1 - L * zP1, L * zQ2 - L * zP2};
+  hg::ExtBar eP, eQ;
+  hg::roe_vjp_ext(P, Q, ex, ey, a.g, half_g, hsmall, c, eP, eQ);
+  hg::SideBar sP = hg::ext_to_side_r(eP, P, rhP, r2P, half_g);
+  const hg::SideBar sQ = hg::ext_to_side_r(eQ, Q, rhQ, r2Q, half_g);
+  if (type == 2) {
+    double nb = 0.0, zbb = 0.0;
+    adj_ghost_route(a, P, n_a, rec.ref, gs, sQ, nx, ny, sP, nb, zbb);
+    if (gs.kind == HG_BC_EXITH) {
+      S[(3 * MD + ka) * T + ia] = zbb;
+      atomicOr(&Xm[ia], 1 << ka);
+    } else {
+      S[(3 * MD + ka) * T + ia] = nb;
+    }
+  }
+  if (type == 0 || !sw) {
+    S[(0 * MD + kp) * T + ip] = (P.h > hsmall) ? sP.xi + sP.h : sP.xi;
+    S[(1 * MD + kp) * T + ip] = sP.hu;
+    S[(2 * MD + kp) * T + ip] = sP.hv;
+  }
+  if (type == 0 || sw) {
+    S[(0 * MD + kq) * T + iq] = (Q.h > hsmall) ? sQ.xi + sQ.h : sQ.xi;
+    S[(1 * MD + kq) * T + iq] = sQ.hu;
+    S[(2 * MD + kq) * T + iq] = sQ.hv;
+  }
 }
 
 template <bool STEP, int T, int MD>
@@ -970,6 +1075,17 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
   double* S = lds + kAdjRows * T;                          // 4 x MD x T: per slot xi(+h), hu, hv, n or zb
   int32_t* Xm = reinterpret_cast<int32_t*>(S + 4 * MD * T);  // T: exit-H slots of each cell
   const double hsmall = a.hsmall;
+  // this work-item's first two work items, and the neighbour of a cross-tile
+  // first item, are read with the cell data: one round of memory latency
+  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
+  const int itA = it0 + li, itB = itA + T;
+  hg::ItemRec recA{}, recB{};
+  NbrRaw nbrA{};
+  if (itA < it1) {
+    recA = a.items[itA];
+    if (((recA.code >> 28) & 3) == 1) nbrA = nbr_load<STEP>(a, recA.ref);
+  }
+  if (itB < it1) recB = a.items[itB];
   int d_i = 0;
   if (valid) {
     const hg::CellQ me = hg::make_cellq(a.xi[i], a.hstill[i], a.qx[i], a.qy[i], a.zb[i], a.half_g, hsmall);
@@ -977,98 +1093,29 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
     if (STEP && a.mask && a.mask[i]) l0 = l1 = l2 = 0.0;
     const double dti = a.dtc ? a.dtc[i] : a.dt;
     const double rA = 1.0 / a.area[i];
-    C[0 * T + li] = me.xi; C[1 * T + li] = me.hs; C[2 * T + li] = me.h; C[3 * T + li] = me.hu;
-    C[4 * T + li] = me.hv; C[5 * T + li] = me.zb; C[6 * T + li] = me.u; C[7 * T + li] = me.v;
-    C[8 * T + li] = me.s;
-    C[9 * T + li] = (STEP ? dti * l0 : l0) * rA;
-    C[10 * T + li] = (STEP ? dti * l1 : l1) * rA;
-    C[11 * T + li] = (STEP ? dti * l2 : l2) * rA;
-    C[12 * T + li] = 1.0 / me.h;
-    C[13 * T + li] = 0.5 / me.s;
+    C[0 * T + li] = me.xi; C[1 * T + li] = me.hs; C[2 * T + li] = me.hu; C[3 * T + li] = me.hv;
+    C[4 * T + li] = me.zb; C[5 * T + li] = me.u; C[6 * T + li] = me.v; C[7 * T + li] = me.s;
+    C[8 * T + li] = (STEP ? dti * l0 : l0) * rA;
+    C[9 * T + li] = (STEP ? dti * l1 : l1) * rA;
+    C[10 * T + li] = (STEP ? dti * l2 : l2) * rA;
+    C[11 * T + li] = 1.0 / me.h;
+    C[12 * T + li] = 0.5 / me.s;
     d_i = a.deg[i];
     for (int k = 0; k < MD; ++k) S[(3 * MD + k) * T + li] = 0.0;
     Xm[li] = 0;
   }
   __syncthreads();
 
-  const double half_g = a.half_g;
-  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
-  for (int it = it0 + li; it < it1; it += T) {
-    const hg::ItemRec rec = a.items[it];
-    const int code = rec.code;
-    const int ia = code & 1023, ka = (code >> 10) & 15;
-    const int ib = (code >> 14) & 1023, kb = (code >> 24) & 15;
-    const int type = (code >> 28) & 3;
-    const double nx = rec.nx, ny = rec.ny, L = rec.len;
-    // P / Q: the evaluation's first / second side (normal from P)
-    const bool sw = type != 2 && !(nx > 0.0 || (nx == 0.0 && ny > 0.0));
-    const double ex = sw ? -nx : nx, ey = sw ? -ny : ny;
-    hg::CellQ P, Q;
-    double zP0, zP1, zP2, zQ0, zQ1, zQ2, rhP, rhQ, r2P, r2Q;
-    int ip = ia, kp = ka, iq = ia, kq = ka;  // LDS slots of P and Q (type 0); the tile's cell otherwise
-    GhostAdj gs{};
-    double n_a = 0.0;
-    if (type == 0) {
-      if (sw) { ip = ib; kp = kb; } else { iq = ib; kq = kb; }
-      P = lds_cellq<T>(C, ip, half_g);
-      Q = lds_cellq<T>(C, iq, half_g);
-      zP0 = C[9 * T + ip]; zP1 = C[10 * T + ip]; zP2 = C[11 * T + ip];
-      zQ0 = C[9 * T + iq]; zQ1 = C[10 * T + iq]; zQ2 = C[11 * T + iq];
-      rhP = C[12 * T + ip]; r2P = C[13 * T + ip];
-      rhQ = C[12 * T + iq]; r2Q = C[13 * T + iq];
-    } else {
-      const hg::CellQ Lc = lds_cellq<T>(C, ia, half_g);
-      const double zL0 = C[9 * T + ia], zL1 = C[10 * T + ia], zL2 = C[11 * T + ia];
-      const double rhL = C[12 * T + ia], r2L = C[13 * T + ia];
-      hg::CellQ G;
-      double zG0 = 0.0, zG1 = 0.0, zG2 = 0.0;
-      if (type == 1) {  // cross-tile neighbour
-        const int j = rec.ref;
-        G = hg::make_cellq(a.xi[j], a.hstill[j], a.qx[j], a.qy[j], a.zb[j], half_g, hsmall);
-        double m0 = a.lx[j], m1 = a.lqx[j], m2 = a.lqy[j];
-        if (STEP && a.mask && a.mask[j]) m0 = m1 = m2 = 0.0;
-        const double dtj = a.dtc ? a.dtc[j] : a.dt;
-        const double rAj = 1.0 / a.area[j];
-        zG0 = (STEP ? dtj * m0 : m0) * rAj;
-        zG1 = (STEP ? dtj * m1 : m1) * rAj;
-        zG2 = (STEP ? dtj * m2 : m2) * rAj;
-      } else {  // ghost (no cotangent of its own)
-        n_a = a.n[t * T + ia];
-        G = adj_ghost_state(a, Lc, n_a, rec.ref, nx, ny, gs);
-      }
-      const double rhG = 1.0 / G.h, r2G = 0.5 / G.s;
-      P = sw ? G : Lc;
-      Q = sw ? Lc : G;
-      zP0 = sw ? zG0 : zL0; zP1 = sw ? zG1 : zL1; zP2 = sw ? zG2 : zL2;
-      zQ0 = sw ? zL0 : zG0; zQ1 = sw ? zL1 : zG1; zQ2 = sw ? zL2 : zG2;
-      rhP = sw ? rhG : rhL; r2P = sw ? r2G : r2L;
-      rhQ = sw ? rhL : rhG; r2Q = sw ? r2L : r2G;
-    }
-    const double c[3] = {L * zQ0 - L * zP0, L * zQ1 - L * zP1, L * zQ2 - L * zP2};
-    hg::ExtBar eP, eQ;
-    hg::roe_vjp_ext(P, Q, ex, ey, a.g, half_g, hsmall, c, eP, eQ);
-    hg::SideBar sP = hg::ext_to_side_r(eP, P, rhP, r2P, half_g);
-    const hg::SideBar sQ = hg::ext_to_side_r(eQ, Q, rhQ, r2Q, half_g);
-    if (type == 2) {
-      double nb = 0.0, zbb = 0.0;
-      adj_ghost_route(a, P, n_a, rec.ref, gs, sQ, nx, ny, sP, nb, zbb);
-      if (gs.kind == HG_BC_EXITH) {
-        S[(3 * MD + ka) * T + ia] = zbb;
-        atomicOr(&Xm[ia], 1 << ka);
-      } else {
-        S[(3 * MD + ka) * T + ia] = nb;
-      }
-    }
-    if (type == 0 || !sw) {
-      S[(0 * MD + kp) * T + ip] = (P.h > hsmall) ? sP.xi + sP.h : sP.xi;
-      S[(1 * MD + kp) * T + ip] = sP.hu;
-      S[(2 * MD + kp) * T + ip] = sP.hv;
-    }
-    if (type == 0 || sw) {
-      S[(0 * MD + kq) * T + iq] = (Q.h > hsmall) ? sQ.xi + sQ.h : sQ.xi;
-      S[(1 * MD + kq) * T + iq] = sQ.hu;
-      S[(2 * MD + kq) * T + iq] = sQ.hv;
-    }
+  if (itA < it1) adj_item<STEP, T, MD>(a, t, C, S, Xm, recA, nbrA);
+  for (int it = itB; it < it1; it += T) {
+    hg::ItemRec rec;
+    if (it == itB)
+      rec = recB;
+    else
+      rec = a.items[it];
+    NbrRaw nr{};
+    if (((rec.code >> 28) & 3) == 1) nr = nbr_load<STEP>(a, rec.ref);
+    adj_item<STEP, T, MD>(a, t, C, S, Xm, rec, nr);
   }
   // the finish's own loads, in flight while the other waves finish their items
   FinishIn fin{};
@@ -1099,7 +1146,7 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
     }
   }
   CellAdj ca;
-  ca.me = lds_cellq<T>(C, li, a.half_g);
+  ca.me = lds_cellq<T>(C, li, a.half_g, hsmall);
   ca.wet = ca.me.h > hsmall;
   ca.xi = ca.me.xi;
   ca.zb = ca.me.zb;
