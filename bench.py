@@ -34,19 +34,23 @@ def bytes_per_cell_step(mesh, fields=1):
     return 24 + 24 + 8 + (40 + 28 * avg_deg) / fields
 
 
-def bytes_per_gradient_cell_step(mesh, kck):
-    """Algorithmic HBM bytes of one horizon cell-step of a checkpointed
-    gradient: the forward sweep and the segment recompute (one forward
-    cell-step each), the reverse step (read Q_n 24, read lambda_{n+1} 24,
-    write lambda_n 24, per-cell constants 48, faces 28 per cell-face, read and
-    write the n cotangent 16) and the checkpoint write (24 / kck)."""
+def bytes_per_gradient_cell_step(mesh, kck, steps=None):
+    """Algorithmic HBM bytes of one horizon cell-step of a gradient w.r.t.
+    n: the forward sweep (one forward cell-step), the reverse step (read
+    Q_n 24, read lambda_{n+1} 24, write lambda_n 24, per-cell constants 48,
+    faces 28 per cell-face, read and write the n cotangent 16) and either
+    the segment recompute (one more forward cell-step) plus the checkpoint
+    write (24 / kck), or, when one segment covers the run (kck >= steps),
+    the store of every state (24)."""
     avg_deg = float(mesh.face_off[-1]) / mesh.n_cells
     reverse = 24 + 24 + 24 + 48 + 28 * avg_deg + 16
+    if steps is not None and kck >= steps:
+        return bytes_per_cell_step(mesh) + 24 + reverse
     return 2 * bytes_per_cell_step(mesh) + reverse + 24.0 / kck
 
 
 def gradient_roofline(mesh, cells, steps, kck, seconds):
-    bpc = bytes_per_gradient_cell_step(mesh, kck)
+    bpc = bytes_per_gradient_cell_step(mesh, kck, steps)
     achieved = bpc * cells * steps / seconds / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "bytes_per_horizon_cell_step": bpc}
@@ -181,9 +185,11 @@ def measure_gradient(args, capi):
 def measure_gradient_full(args, capi, case, tile):
     """The probe-loss gradient w.r.t. per-cell n on the forward workload's
     own 1M-triangle mesh: 1,000 steps (the forward workload's length,
-    whatever --steps says), checkpoint every
-    --grad-kck steps, 200 probes, targets from a forward run with a 'true'
-    lognormal n (mean 0.035, sigma 0.3), initial guess n = 0.03."""
+    whatever --steps says), 200 probes, targets from a forward run with a
+    'true' lognormal n (mean 0.035, sigma 0.3), initial guess n = 0.03.  The
+    checkpoint interval is the library's choice (kck = 0: the whole
+    trajectory when it fits in half the free HBM, which 24 GB does), and the
+    same gradient is also timed with a checkpoint every --grad-kck steps."""
     rng = np.random.default_rng(102)
     sig = 0.3
     n_true = np.exp(rng.normal(np.log(0.035) - 0.5 * sig * sig, sig, case.N))
@@ -195,17 +201,26 @@ def measure_gradient_full(args, capi, case, tile):
     steps = args.grad_1m_steps
     QT = ctx.step(c.Q0, c.dt, steps, params=n_true)
     target = QT[probe] + c.hstill[probe] + c.zb[probe]
-    ctx.grad_probes(c.Q0, c.dt, 100, args.grad_kck, probe, target, params=c.n, wrt=("n",))  # warm-up
-    t0 = time.perf_counter()
-    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, args.grad_kck, probe, target, params=c.n, wrt=("n",))
-    el = time.perf_counter() - t0
+
+    def timed(kck):
+        ctx.grad_probes(c.Q0, c.dt, steps, kck, probe, target, params=c.n, wrt=("n",))  # warm-up (allocates)
+        t0 = time.perf_counter()
+        loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, kck, probe, target, params=c.n, wrt=("n",))
+        return time.perf_counter() - t0, loss, gn, ctx.last_checkpoint_interval()
+
+    el, loss, gn, k_used = timed(0)
     out = {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
            "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": steps,
-                      "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "per-cell Manning n",
-                      "dt": c.dt},
-           "roofline": gradient_roofline(c.mesh, c.N, steps, args.grad_kck, el),
+                      "checkpoint_every": k_used, "checkpoint_choice": "library (kck = 0)",
+                      "trajectory_GB": (steps + 1) * 3 * c.N * 8 / 1e9 if k_used >= steps else None,
+                      "probes": int(probe.size), "wrt": "per-cell Manning n", "dt": c.dt},
+           "roofline": gradient_roofline(c.mesh, c.N, steps, k_used, el),
            "loss": loss, "grad_norm": float(np.linalg.norm(gn))}
-    out["stored_trajectory"] = stored_gradient(ctx, c, steps, probe, target)
+    el2, loss2, gn2, _ = timed(args.grad_kck)
+    out["checkpointed"] = {"value": c.N * steps / el2, "unit": "cell-steps/s", "ms_per_gradient": el2 * 1e3,
+                           "checkpoint_every": args.grad_kck,
+                           "roofline": gradient_roofline(c.mesh, c.N, steps, args.grad_kck, el2),
+                           "bitwise_equal": bool(loss2 == loss and np.array_equal(gn2, gn))}
     return out
 
 
@@ -218,13 +233,9 @@ def stored_gradient(ctx, c, steps, probe, target):
     t0 = time.perf_counter()
     ctx.grad_probes(c.Q0, c.dt, steps, steps, probe, target, params=c.n, wrt=("n",))
     el = time.perf_counter() - t0
-    avg_deg = float(c.mesh.face_off[-1]) / c.N
-    bpc = bytes_per_cell_step(c.mesh) + 24 + (24 + 24 + 24 + 48 + 28 * avg_deg + 16)  # forward + store + reverse
-    achieved = bpc * c.N * steps / el / 1e9
     return {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
             "checkpoint_every": steps, "trajectory_GB": (steps + 1) * 3 * c.N * 8 / 1e9,
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_horizon_cell_step": bpc}}
+            "roofline": gradient_roofline(c.mesh, c.N, steps, steps, el)}
 
 
 def batch_instances(n_inst, seed=7, lo=5_000, hi=500_000):

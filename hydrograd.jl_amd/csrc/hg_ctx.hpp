@@ -321,6 +321,16 @@ struct hg_ctx {
   hg::ItemRec* items = nullptr;
   int32_t tileMD = 3;
   int32_t xcd_chunk = 0;  // identity block -> tile mapping by default (the autotuner re-checks)
+  int32_t ncu = 256;      // compute units of the device
+  // the reverse step's tiling (its own tile size; the step kernel's items
+  // when those have it, else its own copy built on first use)
+  int32_t adj_T_req = 0;  // hg_set_adjoint_tile (0: the library's choice)
+  int32_t adj_T = 0, adj_ntiles = 0;
+  const int32_t* adj_item_off = nullptr;
+  const hg::ItemRec* adj_items = nullptr;
+  int32_t adj_own_T = 0, adj_own_ntiles = 0;
+  int32_t* adj_own_off = nullptr;
+  hg::ItemRec* adj_own_items = nullptr;
   int32_t *tile_in_off = nullptr, *tile_in_list = nullptr;
   std::vector<void*> tile_allocs;  // device buffers of the current tiling (freed on re-tiling)
   size_t lds_max = 65536;          // dynamic LDS a work-group may use on this device
@@ -344,6 +354,7 @@ struct hg_ctx {
   // per-cell n and of zb (through S0 and the exit-H depth) are skipped when
   // the caller does not ask for them
   bool adj_want_n = true, adj_want_zb = true;
+  int64_t last_kck = 0;  // checkpoint interval of the last hg_adjoint / hg_grad_probes call
   // adjoint work buffers (allocated on first use)
   double* adj_lam[2] = {nullptr, nullptr};  // 3N each
   double *adj_nbar = nullptr, *adj_zbbar = nullptr, *adj_S0bar = nullptr, *adj_vbar = nullptr, *adj_Qbar = nullptr;

@@ -2371,6 +2371,7 @@ int enqueue_step_p(hg_ctx* ctx, const double* const in[3], double* const out[3],
 int advance_async(hg_ctx* ctx, double dt, int64_t nsteps);
 int advance_timed(hg_ctx* ctx, double dt, int64_t nsteps, double* kernel_ms);
 int drop_graph(hg_ctx* ctx);
+int ensure_adj_tiles(hg_ctx* ctx);
 
 // Launches the cached graph of `key` on `stream`, capturing it first from
 // enqueue() (which enqueues on ctx->stream, pointed at `stream` meanwhile;
@@ -2461,6 +2462,7 @@ int ensure_adj(hg_ctx* ctx, int64_t kck, int64_t nseg) {
     if (rc) return rc;
     ctx->adj_ckpt_cap = size_t(nseg);
   }
+  if (ctx->tiled) return ensure_adj_tiles(ctx);  // before any capture: it may build and upload
   return HG_OK;
 }
 
@@ -2513,14 +2515,51 @@ hg::AdjArgs make_adj_args(hg_ctx* ctx, const double* const q[3], const double* c
   return a;
 }
 
-// The reverse step runs tiled wherever the step does and its LDS fits.
-bool adj_tiled(const hg_ctx* ctx) {
-  return ctx->tiled && adj_tile_lds_bytes(ctx->tileT, ctx->tileMD) <= ctx->lds_max;
+// The reverse step runs tiled wherever the step does.  Its tile size is its
+// own: 256 cells, or 128 when 256-cell tiles would give fewer than two
+// rounds of three work-groups per CU (its results do not depend on the
+// tiling).  It uses the step kernel's work items when they have that tile
+// size, else its own copy, built once.
+int ensure_adj_tiles(hg_ctx* ctx) {
+  const int64_t blocks256 = (ctx->Nm + 255) / 256 * ctx->K;
+  int T = ctx->adj_T_req ? ctx->adj_T_req : (blocks256 >= int64_t(2) * 3 * ctx->ncu ? 256 : 128);
+  if (adj_tile_lds_bytes(T, ctx->tileMD) > ctx->lds_max) T = 128;
+  if (T == ctx->tileT) {
+    ctx->adj_T = T;
+    ctx->adj_ntiles = ctx->ntiles;
+    ctx->adj_item_off = ctx->item_off;
+    ctx->adj_items = ctx->items;
+    return HG_OK;
+  }
+  if (T != ctx->adj_own_T) {
+    const int64_t N = ctx->Nm;
+    TileBuild tb = build_tiles(N, T, ctx->h_deg, ctx->h_nbr, ctx->h_nx, ctx->h_ny, ctx->h_len, ctx->h_gkind,
+                               ctx->h_gbc);
+    const size_t ni = tb.code.size();
+    std::vector<hg::ItemRec> recs(ni);
+    for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    int rc = drop_graph(ctx);
+    if (!rc) rc = dfree(ctx, ctx->adj_own_off);
+    if (!rc) rc = dfree(ctx, ctx->adj_own_items);
+    if (!rc) rc = dalloc(ctx, &ctx->adj_own_off, tb.off.size());
+    if (!rc) rc = dalloc(ctx, &ctx->adj_own_items, ni);
+    if (!rc) rc = upload(ctx, ctx->adj_own_off, tb.off.data(), tb.off.size());
+    if (!rc) rc = upload(ctx, ctx->adj_own_items, recs.data(), ni);
+    if (rc) return rc;
+    ctx->adj_own_T = T;
+    ctx->adj_own_ntiles = int32_t((N + T - 1) / T);
+  }
+  ctx->adj_T = T;
+  ctx->adj_ntiles = ctx->adj_own_ntiles;
+  ctx->adj_item_off = ctx->adj_own_off;
+  ctx->adj_items = ctx->adj_own_items;
+  return HG_OK;
 }
 
 template <int T, int MD>
 void launch_adj_tile_tm(hg_ctx* ctx, const hg::AdjArgs& a) {
-  const dim3 grid(unsigned(ctx->ntiles) * unsigned(ctx->K));
+  const dim3 grid(unsigned(ctx->adj_ntiles) * unsigned(ctx->K));
   hipLaunchKernelGGL((k_adj_tile<true, T, MD>), grid, dim3(T), adj_tile_lds_bytes(T, MD), ctx->stream, a);
 }
 
@@ -2535,12 +2574,12 @@ void launch_adj_tile_t(hg_ctx* ctx, const hg::AdjArgs& a) {
 }
 
 void launch_adj_tile(hg_ctx* ctx, hg::AdjArgs a) {
-  a.item_off = ctx->item_off;
-  a.items = ctx->items;
+  a.item_off = ctx->adj_item_off;
+  a.items = ctx->adj_items;
   a.xcd_chunk = ctx->xcd_chunk;
-  if (ctx->tileT == 128)
+  if (ctx->adj_T == 128)
     launch_adj_tile_t<128>(ctx, a);
-  else if (ctx->tileT == 256)
+  else if (ctx->adj_T == 256)
     launch_adj_tile_t<256>(ctx, a);
   else
     launch_adj_tile_t<512>(ctx, a);
@@ -2585,7 +2624,7 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
   a.nbar_set = (ude || law || lux_n) ? 1 : 0;
   a.fbar_set = (lux && !lux_n) ? 1 : 0;
   if (totalA) a.totalA = totalA;
-  if (step && adj_tiled(ctx))
+  if (step && ctx->tiled)
     launch_adj_tile(ctx, a);
   else if (step)
     hipLaunchKernelGGL(k_adj_cell<true>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
@@ -3473,6 +3512,7 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu <= 0)
       ncu = 256;
+    ctx->ncu = ncu;
     const bool small = ctx->N < int64_t(2) * ncu * 256;
     const int T = (!small && tile_lds_bytes(256, ctx->tileMD) <= ctx->lds_max) ? 256 : 128;
     rc = setup_tiles(ctx, T);
@@ -4684,6 +4724,26 @@ int hg_batch_set_dt(hg_ctx* ctx, const double* dt_inst) {
   return HG_OK;
 }
 
+int hg_last_checkpoint_interval(const hg_ctx* ctx, int64_t* kck) {
+  if (!ctx || !kck) return HG_ERR_ARG;
+  *kck = ctx->last_kck;
+  return HG_OK;
+}
+
+int hg_set_adjoint_tile(hg_ctx* ctx, int32_t tile_cells) {
+  if (!ctx) return HG_ERR_ARG;
+  if (tile_cells != 0 && tile_cells != 128 && tile_cells != 256 && tile_cells != 512) {
+    ctx->err = "hg_set_adjoint_tile: tile size must be 0 (automatic), 128, 256 or 512 cells";
+    return HG_ERR_ARG;
+  }
+  if (!ctx->tiled) {
+    ctx->err = "hg_set_adjoint_tile: this context runs the per-cell kernels";
+    return HG_ERR_STATE;
+  }
+  ctx->adj_T_req = tile_cells;
+  return drop_graph(ctx);  // captured reversals hold the old tiling; the next adjoint call re-tiles
+}
+
 int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk) {
   if (!ctx) return HG_ERR_ARG;
   if (!ctx->tiled) {
@@ -4869,6 +4929,7 @@ static int adj_run(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
     return HG_ERR_ARG;
   }
   const int64_t nseg = (nsteps + kck - 1) / kck;
+  ctx->last_kck = kck;
   int rc = hg_set_params(ctx, params, n_params);
   if (!rc) rc = ensure_adj(ctx, kck, nseg);
   if (!rc) rc = hg_state_upload(ctx, Q0);

@@ -35,7 +35,7 @@ EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
-    "hg_set_tile_config", "hg_get_tile_config", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
+    "hg_set_tile_config", "hg_get_tile_config", "hg_set_adjoint_tile", "hg_last_checkpoint_interval", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
     "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
     "hg_advance_timed", "hg_advance_region", "hg_prepare_advance", "hg_cfl_dt_noalloc", "hg_fields_create",
@@ -472,6 +472,16 @@ class Context:
 
     def set_tile_config(self, tile_cells, xcd_chunk=0):
         self._chk(lib().hg_set_tile_config(self.h, int(tile_cells), int(xcd_chunk)))
+
+    def last_checkpoint_interval(self):
+        """Checkpoint interval of the last gradient call (what kck <= 0 chose)."""
+        k = C.c_int64()
+        self._chk(lib().hg_last_checkpoint_interval(self.h, C.byref(k)))
+        return k.value
+
+    def set_adjoint_tile(self, tile_cells):
+        """Tile size of the reverse step (0: the library's choice)."""
+        self._chk(lib().hg_set_adjoint_tile(self.h, int(tile_cells)))
 
     def tile_config(self):
         t, x = C.c_int32(), C.c_int32()

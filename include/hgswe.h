@@ -244,6 +244,13 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
 int hg_set_tile_config(hg_ctx* ctx, int32_t tile_cells, int32_t xcd_chunk);
 int hg_get_tile_config(hg_ctx* ctx, int32_t* tile_cells, int32_t* xcd_chunk);
 
+/* Tile size of the tiled reverse step (no reference counterpart): 128, 256
+ * or 512 cells, or 0 (default) for the library's choice (256, or 128 when
+ * the mesh has fewer than two rounds of three 256-cell work-groups per CU).
+ * It shares the step kernel's block -> tile mapping.  Every size gives
+ * bit-identical gradients. */
+int hg_set_adjoint_tile(hg_ctx* ctx, int32_t tile_cells);
+
 /* Autotuner: after 500 warm-up steps (the device clocks settle), times the
  * step kernel over nsteps steps of every launch shape, twice each (events on
  * each dispatch, after 16 warm-up steps), from the current device state
@@ -279,6 +286,11 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
  * else ceil(sqrt(nsteps)).  Results are bitwise the same for every kck. */
 int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
                int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar, double* inletQ_bar);
+
+/* The checkpoint interval the last hg_adjoint / hg_grad_probes call used
+ * (what kck <= 0 resolved to; 0 before any such call).  No reference
+ * counterpart. */
+int hg_last_checkpoint_interval(const hg_ctx* ctx, int64_t* kck);
 
 /* Gradient of the probe loss L = sum_p (xi_p + h_still_p + zb_p - target_p)^2
  * at the final state (the WSE misfit of swe_2D_inversion.jl:387-399 without

@@ -68,6 +68,9 @@ def test_gradient_bytes_per_horizon_cell_step():
     assert b.bytes_per_gradient_cell_step(tri.mesh, 50) == 180 + 180 + 220 + 24 / 50
     r = b.gradient_roofline(tri.mesh, 1_000_000, 1000, 50, 0.1)
     assert abs(r["achieved"] - 580.48 * 1e9 / 0.1 / 1e9) < 1e-6 and r["frac"] == r["achieved"] / 8000.0
+    # one segment covering the run: no recompute, every state stored (24 B)
+    assert b.bytes_per_gradient_cell_step(tri.mesh, 1000, 1000) == 180 + 24 + 220
+    assert b.gradient_roofline(tri.mesh, 1_000_000, 1000, 1000, 0.1)["bytes_per_horizon_cell_step"] == 424
 
 
 def test_shared_mesh_bytes_per_cell_step():

@@ -161,8 +161,9 @@ def test_tiled_reverse_step_matches_cell_reverse_step(maker):
     from the side whose normal points to +x) against the per-cell reverse
     step (each cell forms every face's VJP from its own side): the same
     gradient to rounding.  The tiled gradient is bitwise the same for every
-    tile size and block -> tile mapping (no face's bits depend on which
-    work-group evaluates it)."""
+    tile size of the reverse step, whether it shares the step kernel's work
+    items or has its own, and every block -> tile mapping (no face's bits
+    depend on which work-group evaluates it)."""
     c = maker()
     nsteps, kck = 40, 9
     probe, target = _probe_problem(c, nsteps)
@@ -170,15 +171,17 @@ def test_tiled_reverse_step_matches_cell_reverse_step(maker):
     ctx = capi.Context(c)
     first = None
     for T in (128, 256, 512):
-        for x in (0, 4):
-            ctx.set_tile_config(T, x)
-            out = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
-            if first is None:
-                first = out
-                continue
-            assert out[0] == first[0], (T, x)
-            for a, b in zip(out[1:], first[1:]):
-                assert np.array_equal(a, b), (T, x)
+        for TA in (0, 128, 256, 512):
+            for x in ((0, 4) if TA == 0 else (0,)):
+                ctx.set_tile_config(T, x)
+                ctx.set_adjoint_tile(TA)
+                out = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+                if first is None:
+                    first = out
+                    continue
+                assert out[0] == first[0], (T, TA, x)
+                for a, b in zip(out[1:], first[1:]):
+                    assert np.array_equal(a, b), (T, TA, x)
     assert first[0] == ref[0]  # the forward sweeps are bitwise the same
     for a, b, name in zip(first[1:], ref[1:], ("n", "zb", "Q")):
         rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
