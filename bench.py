@@ -187,7 +187,7 @@ def measure_gradient_full(args, capi, case, tile):
     own 1M-triangle mesh: 1,000 steps (the forward workload's length,
     whatever --steps says), 200 probes, targets from a forward run with a
     'true' lognormal n (mean 0.035, sigma 0.3), initial guess n = 0.03.  The
-    checkpoint interval is the library's choice (kck = 0: the whole
+    checkpoint interval is --kck (default 0, the library's choice: the whole
     trajectory when it fits in half the free HBM, which 24 GB does), and the
     same gradient is also timed with a checkpoint every --grad-kck steps."""
     rng = np.random.default_rng(102)
@@ -208,10 +208,11 @@ def measure_gradient_full(args, capi, case, tile):
         loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, steps, kck, probe, target, params=c.n, wrt=("n",))
         return time.perf_counter() - t0, loss, gn, ctx.last_checkpoint_interval()
 
-    el, loss, gn, k_used = timed(0)
+    el, loss, gn, k_used = timed(args.kck)
     out = {"value": c.N * steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
            "config": {"workload": "channel_1M_tri_probe_loss_gradient", "cells": c.N, "steps": steps,
-                      "checkpoint_every": k_used, "checkpoint_choice": "library (kck = 0)",
+                      "checkpoint_every": k_used,
+                      "checkpoint_choice": "library (kck = 0)" if args.kck <= 0 else "fixed",
                       "trajectory_GB": (steps + 1) * 3 * c.N * 8 / 1e9 if k_used >= steps else None,
                       "probes": int(probe.size), "wrt": "per-cell Manning n", "dt": c.dt},
            "roofline": gradient_roofline(c.mesh, c.N, steps, k_used, el),
@@ -277,13 +278,14 @@ def measure_batch(args, capi, single_value):
     probe = probe.astype(np.int32)
     QN = ctx.step(B.Q0, B.dt, args.batch_grad_steps)
     target = QN[probe] + B.hstill[probe] + B.zb[probe] + rng.normal(0.0, 0.02, probe.size)
-    ctx.grad_probes(B.Q0, B.dt, 10, 5, probe, target)  # warm-up
+    ctx.grad_probes(B.Q0, B.dt, args.batch_grad_steps, args.kck, probe, target)  # warm-up (allocates)
     t0 = time.perf_counter()
-    ctx.grad_probes(B.Q0, B.dt, args.batch_grad_steps, args.grad_kck, probe, target)
+    ctx.grad_probes(B.Q0, B.dt, args.batch_grad_steps, args.kck, probe, target)
     el = time.perf_counter() - t0
     out["gradient"] = {"value": B.N * args.batch_grad_steps / el, "unit": "cell-steps/s",
                        "ms_per_gradient": el * 1e3, "steps": args.batch_grad_steps,
-                       "checkpoint_every": args.grad_kck, "probes": int(probe.size), "wrt": "n, zb, inlet Q"}
+                       "checkpoint_every": ctx.last_checkpoint_interval(), "probes": int(probe.size),
+                       "wrt": "n, zb, inlet Q"}
     return out
 
 
@@ -333,15 +335,16 @@ def measure_ude(args, capi):
         # probe loss (200 probes per instance) w.r.t. its 353 weights
         QB = bctx.step(B.Q0, B.dt, args.ude_steps)
         btarget = QB[bprobe] + B.hstill[bprobe] + B.zb[bprobe] + noise
-        bctx.grad_probes(B.Q0, B.dt, 10, 5, bprobe, btarget, wrt=())  # warm-up
+        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.kck, bprobe, btarget, wrt=())  # warm-up (allocates)
         t0 = time.perf_counter()
-        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, bprobe, btarget, wrt=())  # the weights only
+        bctx.grad_probes(B.Q0, B.dt, args.ude_steps, args.kck, bprobe, btarget, wrt=())  # the weights only
         gws = bctx.ude_grad()
         el = time.perf_counter() - t0
+        k_used = bctx.last_checkpoint_interval()
         del bctx
         out[key]["gradient"] = {
             "value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
-            "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "probes": int(bprobe.size),
+            "steps": args.ude_steps, "checkpoint_every": k_used, "probes": int(bprobe.size),
             "wrt": f"{nset} x 353 closure weights (each set's own instance loss)",
             "grad_norm_min": float(np.min(np.linalg.norm(gws.reshape(nset, -1), axis=1)))}
     # gradient w.r.t. the weights
@@ -349,13 +352,14 @@ def measure_ude(args, capi):
     probe = rng.choice(c.N, size=200, replace=False).astype(np.int32)
     QN = ctx.step(c.Q0, c.dt, args.ude_steps)
     target = QN[probe] + c.hstill[probe] + c.zb[probe] + rng.normal(0.0, 0.02, probe.size)
-    ctx.grad_probes(c.Q0, c.dt, 10, 5, probe, target, wrt=())  # warm-up
+    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.kck, probe, target, wrt=())  # warm-up (allocates)
     t0 = time.perf_counter()
-    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.grad_kck, probe, target, wrt=())  # the weights only
+    ctx.grad_probes(c.Q0, c.dt, args.ude_steps, args.kck, probe, target, wrt=())  # the weights only
     gw = ctx.ude_grad()
     el = time.perf_counter() - t0
     out["gradient"] = {"value": c.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
-                       "steps": args.ude_steps, "checkpoint_every": args.grad_kck, "wrt": "353 closure weights",
+                       "steps": args.ude_steps, "checkpoint_every": ctx.last_checkpoint_interval(),
+                       "wrt": "353 closure weights",
                        "grad_norm": float(np.linalg.norm(gw))}
     return out
 
@@ -390,12 +394,12 @@ def measure_fields(args, capi):
              "achieved_GBps": bpc * B.N / (ms_k * 1e-3) / 1e9}
         QN = k.step(B.Q0, B.dt, args.ude_steps)
         target = QN[probe] + B.hstill[probe] + B.zb[probe] + noise
-        k.grad_probes(B.Q0, B.dt, 10, 5, probe, target, wrt=("n",))  # warm-up
+        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.kck, probe, target, wrt=("n",))  # warm-up (allocates)
         t0 = time.perf_counter()
-        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.grad_kck, probe, target, wrt=("n",))
+        k.grad_probes(B.Q0, B.dt, args.ude_steps, args.kck, probe, target, wrt=("n",))
         el = time.perf_counter() - t0
         r["gradient"] = {"value": B.N * args.ude_steps / el, "unit": "cell-steps/s", "ms_per_gradient": el * 1e3,
-                         "wrt": "each field's own per-cell n"}
+                         "checkpoint_every": k.last_checkpoint_interval(), "wrt": "each field's own per-cell n"}
         out[key] = r
         del k
     return out
@@ -412,7 +416,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grad-cells", type=int, default=100_000)
     ap.add_argument("--grad-steps", type=int, default=2000)
-    ap.add_argument("--grad-kck", type=int, default=50)
+    ap.add_argument("--grad-kck", type=int, default=50,
+                    help="checkpoint interval of the 100k-cell gradient (its stated 50) and of the 1M-cell one beside it")
+    ap.add_argument("--kck", type=int, default=0,
+                    help="checkpoint interval of the 1M-cell, batch, closure and fields gradients (0: the library's choice)")
     ap.add_argument("--grad-1m-steps", type=int, default=1000,
                     help="steps of the gradient on the forward workload's 1M-triangle mesh")
     ap.add_argument("--no-grad", action="store_true")

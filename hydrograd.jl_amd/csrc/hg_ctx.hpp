@@ -67,6 +67,14 @@ struct InletArgs {
   int32_t* flag;  // set to 1 when a conveyance is <= 1e-10
 };
 
+// One inlet-Q face as the inlet adjoint reads it (32 bytes): its ghost and
+// cell, whether it is the cell's first face in the inlet and the cell's next
+// face there (or -1), L^(2/3) and L^(5/3).
+struct InletFaceRec {
+  int32_t ghost, cell, first, next;
+  double L23, L53;
+};
+
 // One work item of the tiled kernel (32 bytes, one address per item).
 struct ItemRec {
   int32_t code;
@@ -304,10 +312,11 @@ struct hg_ctx {
   double *fnx = nullptr, *fny = nullptr, *flen = nullptr;
   int32_t *g_kind = nullptr, *g_bc = nullptr, *g_cell = nullptr;
   double *g_hstill = nullptr, *g_zb = nullptr, *g_zb0 = nullptr, *g_L23 = nullptr;
-  int32_t *in_off = nullptr, *in_cell = nullptr, *in_ghost = nullptr;
+  int32_t *in_off = nullptr, *in_cell = nullptr;
   double* in_L53 = nullptr;
-  int32_t* in_next = nullptr;   // inlet face -> next face of the same cell in that inlet, or -1
-  int32_t* in_first = nullptr;  // 1 on the first face of each cell in its inlet
+  // per inlet face: ghost, cell, first face of its cell in the inlet, the
+  // cell's next face there (or -1), L^(2/3), L^(5/3) (the inlet adjoint)
+  hg::InletFaceRec* in_rec = nullptr;
   bool inlets_disjoint = true;  // no cell sits on two inlets: the inlet adjoint runs one work-group per inlet
   double *inletQ = nullptr, *inletQ0 = nullptr, *exitWSE = nullptr, *totalA = nullptr;
   int32_t* flags = nullptr;  // device address of flags_host: [0] conveyance error, [1] non-finite state

@@ -1168,18 +1168,18 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
 // faces), adds d/dh of A_b to the state cotangent and accumulates d/dn and
 // d/dQ_b.
 __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, int Nm, int Gm, const int32_t* off,
-                                                     const int32_t* ghost, const int32_t* cell, const int32_t* next,
-                                                     const int32_t* first, const double* L53,
-                                                     const double* g_L23, const double* xi_all, const double* hstill,
-                                                     const double* n_all, const double* inletQ, const double* totalA,
-                                                     const double* vbar_all, double hsmall, double* ox_all,
-                                                     double* nbar_all, double* Qbar) {
+                                                     const hg::InletFaceRec* rec, const double* xi_all,
+                                                     const double* hstill, const double* n_all, const double* inletQ,
+                                                     const double* totalA, const double* vbar_all, double hsmall,
+                                                     double* ox_all, double* nbar_all, double* Qbar) {
   // Per inlet: the per-face terms are formed in parallel and summed by a
   // fixed tree; each cell's faces of the inlet are applied, in NodeString
-  // order, by the work-item of its first face (first[f] = 1; next[f]: the
-  // cell's next face in the inlet, or -1).  With one work-group, the inlets
-  // are taken in order (a cell may sit on two of them); when no cell does,
-  // work-group b takes inlet b alone (same arithmetic, same bits).
+  // order, by the work-item of its first face (rec.first; rec.next: the
+  // cell's next face in the inlet, or -1).  A work-item reads the record,
+  // cell data and ghost cotangent of its first face once, before the sums.
+  // With one work-group, the inlets are taken in order (a cell may sit on
+  // two of them); when no cell does, work-group b takes inlet b alone (same
+  // arithmetic, same bits).
   __shared__ double sA[kBlock], sQ[kBlock];
   const int tid = threadIdx.x;
   const int b0 = gridDim.x > 1 ? int(blockIdx.x) : 0;
@@ -1192,14 +1192,24 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
     const double* vbar = vbar_all + int64_t(fl) * Gm;
     double* ox = ox_all + o;
     double* nbar = nbar_all ? nbar_all + o : nullptr;
-    const double A = totalA[b];
+    const double A = totalA[b], Qb = inletQ[b];
+    const int f0 = off[bl], f1 = off[bl + 1];
+    const int fmine = f0 + tid;
+    hg::InletFaceRec r0{};
+    double n0 = 1.0, h0 = 0.0;
     double pa = 0.0, pq = 0.0;
-    for (int f = off[bl] + tid; f < off[bl + 1]; f += kBlock) {
-      const int gi = ghost[f], c = cell[f];
-      const double vb = vbar[gi];
-      const double v = inletQ[b] / A * g_L23[gi] / n[c];
+    for (int f = fmine; f < f1; f += kBlock) {
+      const hg::InletFaceRec r = rec[f];
+      const double nc = n[r.cell];
+      const double vb = vbar[r.ghost];
+      if (f == fmine) {
+        r0 = r;
+        n0 = nc;
+        h0 = xi[r.cell] + hstill[r.cell];
+      }
+      const double v = Qb / A * r.L23 / nc;
       pa += vb * (-v / A);
-      pq += vb * (g_L23[gi] / (A * n[c]));
+      pq += vb * (r.L23 / (A * nc));
     }
     sA[tid] = pa;
     sQ[tid] = pq;
@@ -1213,14 +1223,19 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
     }
     const double Ab = sA[0];
     if (tid == 0) Qbar[b] += sQ[0];
-    for (int f = off[bl] + tid; f < off[bl + 1]; f += kBlock) {
-      if (!first[f]) continue;
-      for (int g = f; g >= 0; g = next[g]) {
-        const int c = cell[g];
-        const double h = xi[c] + hstill[c];
-        if (!(h > hsmall)) continue;  // w = 0 and h clamped: no derivative
-        ox[c] += Ab * L53[g] / n[c];
-        if (nbar_all) nbar[c] += Ab * (-L53[g] * h / (n[c] * n[c]));
+    for (int f = fmine; f < f1; f += kBlock) {
+      const hg::InletFaceRec r = f == fmine ? r0 : rec[f];
+      if (!r.first) continue;
+      for (int g = f; g >= 0;) {
+        const hg::InletFaceRec rg = g == f ? r : rec[g];
+        const int c = rg.cell;
+        const double h = g == fmine ? h0 : xi[c] + hstill[c];
+        const double nc = g == fmine ? n0 : n[c];
+        if (h > hsmall) {  // else w = 0 and h clamped: no derivative
+          ox[c] += Ab * rg.L53 / nc;
+          if (nbar) nbar[c] += Ab * (-rg.L53 * h / (nc * nc));
+        }
+        g = rg.next;
       }
     }
     __syncthreads();  // sA / sQ reuse; cells shared by two inlets take them in inlet order
@@ -2632,8 +2647,8 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     hipLaunchKernelGGL(k_adj_cell<false>, dim3(grid_of(ctx->N)), dim3(kBlock), 0, ctx->stream, a);
   if (ctx->n_inlet > 0)
     hipLaunchKernelGGL(k_inlet_adj, dim3(ctx->inlets_disjoint ? ctx->n_inlet : 1), dim3(kBlock), 0, ctx->stream,
-                       ctx->n_inlet, ctx->nin_m, int(ctx->Nm), int(ctx->Gm), ctx->in_off, ctx->in_ghost, ctx->in_cell,
-                       ctx->in_next, ctx->in_first, ctx->in_L53, ctx->g_L23, q[0], ctx->hstill, ctx->n, ctx->inletQ,
+                       ctx->n_inlet, ctx->nin_m, int(ctx->Nm), int(ctx->Gm), ctx->in_off, ctx->in_rec, q[0],
+                       ctx->hstill, ctx->n, ctx->inletQ,
                        a.totalA, ctx->adj_vbar, ctx->c.h_small, out[0], nbar, ctx->adj_Qbar);
   if (ude) {  // through the closure: n(Q) adds to the state cotangent, and the weight cotangent accumulates
     hg::UdeArgs u = ude_args(ctx, q);
@@ -3405,10 +3420,8 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = dalloc(ctx, &ctx->g_L23, G);
   if (!rc) rc = dalloc(ctx, &ctx->in_off, b->n_inlet + 1);
   if (!rc) rc = dalloc(ctx, &ctx->in_cell, ctx->n_inlet_faces);
-  if (!rc) rc = dalloc(ctx, &ctx->in_ghost, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->in_L53, ctx->n_inlet_faces);
-  if (!rc) rc = dalloc(ctx, &ctx->in_next, ctx->n_inlet_faces);
-  if (!rc) rc = dalloc(ctx, &ctx->in_first, ctx->n_inlet_faces);
+  if (!rc) rc = dalloc(ctx, &ctx->in_rec, ctx->n_inlet_faces);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ, ctx->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->inletQ0, ctx->n_inlet);
   if (!rc) rc = dalloc(ctx, &ctx->exitWSE, ctx->n_exit);
@@ -3459,7 +3472,6 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   if (!rc) rc = upload(ctx, ctx->g_L23, g_L23.data(), G);
   if (!rc) rc = upload(ctx, ctx->in_off, in_off.data(), b->n_inlet + 1);
   if (!rc) rc = upload(ctx, ctx->in_cell, in_cell.data(), ctx->n_inlet_faces);
-  if (!rc && ctx->n_inlet_faces) rc = upload(ctx, ctx->in_ghost, b->inlet_ghost, ctx->n_inlet_faces);
   if (!rc) rc = upload(ctx, ctx->in_L53, in_L53.data(), ctx->n_inlet_faces);
   if (!rc) {  // per inlet, the faces of each cell in NodeString order (see k_inlet_adj)
     std::vector<int32_t> next(std::max<int32_t>(ctx->n_inlet_faces, 1), -1), first(next.size(), 1);
@@ -3476,8 +3488,12 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
         }
       }
     }
-    rc = upload(ctx, ctx->in_next, next.data(), ctx->n_inlet_faces);
-    if (!rc) rc = upload(ctx, ctx->in_first, first.data(), ctx->n_inlet_faces);
+    std::vector<hg::InletFaceRec> recs(std::max<int32_t>(ctx->n_inlet_faces, 1));
+    for (int32_t f = 0; f < ctx->n_inlet_faces; ++f) {
+      const int32_t gi = b->inlet_ghost[f];
+      recs[f] = hg::InletFaceRec{gi, in_cell[f], first[f], next[f], g_L23[gi], in_L53[f]};
+    }
+    rc = upload(ctx, ctx->in_rec, recs.data(), ctx->n_inlet_faces);
     std::unordered_map<int32_t, int32_t> owner;  // cell -> the inlet it sits on
     ctx->inlets_disjoint = true;
     for (int bi = 0; bi < b->n_inlet && ctx->inlets_disjoint; ++bi)
