@@ -239,6 +239,41 @@ def stored_gradient(ctx, c, steps, probe, target):
             "roofline": gradient_roofline(c.mesh, c.N, steps, steps, el)}
 
 
+def measure_wet_dry(args, capi, tile):
+    """The forward workload with wet/dry fronts: the same channel generator
+    and size with bumps up to 4 m (islands above the 3 m surface; seed 1) at
+    CFL 0.3, the step kernel's launch shape of the main leg.  The timed region
+    and the per-dispatch kernel time as for the headline, and the dry
+    fraction (h at h_small) before and after."""
+    from hgswe import cases
+
+    c = cases.channel(args.cells, seed=1, nsteps=args.steps, bump_amp=4.0, cfl=0.3)
+    ctx = capi.Context(c)
+    ctx.set_tile_config(*tile)
+    ctx.upload(c.Q0)
+    ctx.advance(c.dt, max(args.warmup, 16))
+    ctx.prepare(c.dt)
+    t0 = time.perf_counter()
+    ms_region = ctx.advance_region(c.dt, args.steps)
+    el = time.perf_counter() - t0
+    ms_dispatch = ctx.advance_timed(c.dt, args.steps)
+    q = ctx.download()
+
+    def dry(Q):
+        h = Q[: c.N] + c.hstill
+        return float(np.mean(h <= c.h_small))
+
+    bpc = bytes_per_cell_step(c.mesh)
+    return {"value": c.N * args.steps / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / args.steps,
+            "region_ms_per_step": ms_region, "kernel_ms": ms_dispatch,
+            "frac": bpc * c.N / (ms_region * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac_kernel": bpc * c.N / (ms_dispatch * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "config": {"workload": "channel_1M_tri_islands_forward_euler", "cells": c.N, "bump_amplitude_m": 4.0,
+                       "cfl": 0.3, "dt": c.dt, "cells_above_surface": float(np.mean(c.zb > 3.0)),
+                       "dry_fraction_start": dry(c.Q0), "dry_fraction_end": dry(q),
+                       "steps_before_end": args.warmup + 2 * args.steps}}
+
+
 def batch_instances(n_inst, seed=7, lo=5_000, hi=500_000):
     """64 independent channel instances with N_i log-uniform in [5k, 500k]
     triangles, each with its own bed (seeded bumps), zone n and inlet
@@ -423,6 +458,7 @@ def main():
     ap.add_argument("--grad-1m-steps", type=int, default=1000,
                     help="steps of the gradient on the forward workload's 1M-triangle mesh")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-wet-dry", action="store_true", help="skip the forward run on the bed with islands")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched workload (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--batch-grad-steps", type=int, default=200)
@@ -504,13 +540,14 @@ def main():
         out["copy_probe"] = {"bytes_each_way": nbytes, "GBps": capi.probe_copy(ctx, nbytes, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    tile = ctx.tile_config()
+    del ctx
+    if not args.no_wet_dry:
+        out["wet_dry"] = measure_wet_dry(args, capi, tile)
     if not args.no_grad:
-        tile = ctx.tile_config()
-        del ctx
         out["gradient_1M"] = measure_gradient_full(args, capi, case, tile)
         out["gradient"] = measure_gradient(args, capi)
     if args.batch > 0:
-        ctx = None
         out["batch"] = measure_batch(args, capi, value)
     if not args.no_ude:
         out["ude"] = measure_ude(args, capi)

@@ -151,10 +151,12 @@ def jittered_delaunay(nx, ny, Lx, Ly, seed, jitter=0.3):
 
 def channel(n_target=1_000_000, seed=0, Lx=10_000.0, Ly=1_000.0, Q_in=500.0, wse_out=3.0, n_bumps=50,
             n_zones=5, cfl=0.5, nsteps=1000, zone_n=None, n_cells=None, name=None, h_small=1e-3,
-            bc_types=None, wstill_level=None):
+            bc_types=None, wstill_level=None, bump_amp=0.5):
     """Channel case: seeded Delaunay triangulation of a jittered lattice over
     Lx x Ly (~n_target triangles); bed zb = 1e-4 (Lx - x) + 50 Gaussian bumps
-    (amplitude U[0, 0.5] m, width U[20, 200] m); Manning n in n_zones
+    (amplitude U[0, bump_amp] m, bump_amp = 0.5 by default: every cell wet;
+    larger amplitudes raise islands above the 3 m surface, with wet/dry
+    fronts around them; width U[20, 200] m); Manning n in n_zones
     streamwise zones, each U[0.02, 0.06]; wstill = zb, h = max(h_small,
     3 - zb), q = 0; INLET-Q on x = 0, EXIT-H WSE 3 m on x = Lx, walls
     elsewhere; dt from CFL at the initial state.  ``bc_types`` overrides the
@@ -169,7 +171,7 @@ def channel(n_target=1_000_000, seed=0, Lx=10_000.0, Ly=1_000.0, Q_in=500.0, wse
         bcs = {k + 1: t for k, t in enumerate(bc_types)}
     mesh = hm.build_mesh(xy, tri, ns, bcs)
     rng = np.random.default_rng(seed + 1)
-    A = rng.uniform(0.0, 0.5, n_bumps)
+    A = rng.uniform(0.0, bump_amp, n_bumps)
     W = rng.uniform(20.0, 200.0, n_bumps)
     cx = rng.uniform(0.0, Lx, n_bumps)
     cy = rng.uniform(0.0, Ly, n_bumps)
