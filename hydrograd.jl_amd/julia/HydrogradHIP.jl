@@ -428,15 +428,27 @@ end
 
 dense(x, n) = x isa AbstractZero ? zeros(n) : Vector{Float64}(unthunk(x))
 
+# The per-cell n and zb cotangent buffers a pullback asks the library for:
+# only the active parameter's (a NULL output skips that cotangent's
+# accumulation in the reverse step; the inlet-Q cotangent is always formed).
+function cotangent_buffers(ctx, N)
+    k = ctx.active
+    nb = (k == HG_PARAM_N_CELL || k == HG_PARAM_N_ZONE) ? zeros(N) : nothing
+    zbb = k == HG_PARAM_ZB ? zeros(N) : nothing
+    return nb, zbb
+end
+orNULL(x) = x === nothing ? Ptr{Float64}(C_NULL) : x
+
 # Pullback of one RHS evaluation: (Q, params) -> dQdt.
 function rhs_pullback(ctx, p_extra, q, prm, ybar, p = nothing)
     N = ctx.N
     nI = length(p_extra.inletQ_TotalQ)
-    Qb = zeros(3N); nb = zeros(N); zbb = zeros(N); qb = zeros(max(nI, 1))
+    Qb = zeros(3N); qb = zeros(max(nI, 1))
+    nb, zbb = cotangent_buffers(ctx, N)
     check(ctx.ptr, ccall((:hg_rhs_vjp, libhgswe), Cint,
                          (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
                           Ptr{Float64}, Ptr{Float64}),
-                         ctx.ptr, q, prm, length(prm), dense(ybar, 3N), Qb, nb, zbb, qb))
+                         ctx.ptr, q, prm, length(prm), dense(ybar, 3N), Qb, orNULL(nb), orNULL(zbb), qb))
     return Qb, params_bar(ctx, p_extra, prm, nb, zbb, qb[1:nI], p)
 end
 
@@ -472,15 +484,65 @@ function ChainRulesCore.rrule(::typeof(custom_ODE_update_cells), ode_f::DeviceRH
     N = ctx.N
     nI = length(swe2d_extra_params.inletQ_TotalQ)
     function update_cells_pullback(ybar)
-        Qb = zeros(3N); nb = zeros(N); zbb = zeros(N); qb = zeros(max(nI, 1))
-        check(ctx.ptr, ccall((:hg_adjoint, libhgswe), Cint,
-                             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64, Int64, Ptr{Float64},
-                              Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                             ctx.ptr, q, prm, length(prm), dt, 1, 1, dense(ybar, 3N), Qb, nb, zbb, qb))
-        pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb[1:nI], params_vector)
+        Qb, nb, zbb, qb = step_adjoint(ctx, q, prm, dt, 1, dense(ybar, 3N), nI)
+        pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb, params_vector)
         return NoTangent(), NoTangent(), Qb, pb, NoTangent(), NoTangent(), NoTangent()
     end
     return y, update_cells_pullback
+end
+
+# hg_adjoint over nsteps Euler steps from q with final-state cotangent
+# lamN (checkpoint interval: the library's choice): the cotangents of q, of
+# the active parameter's per-cell field (or nothing) and of the inlet Qs.
+function step_adjoint(ctx, q, prm, dt, nsteps, lamN, nI)
+    N = ctx.N
+    Qb = zeros(3N); qb = zeros(max(nI, 1))
+    nb, zbb = cotangent_buffers(ctx, N)
+    check(ctx.ptr, ccall((:hg_adjoint, libhgswe), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Int64, Int64, Ptr{Float64},
+                          Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.ptr, q, prm, length(prm), dt, nsteps, 0, lamN, Qb, orNULL(nb), orNULL(zbb), qb))
+    return Qb, nb, zbb, qb[1:nI]
+end
+
+addto(a, b) = a === nothing ? b : (b === nothing ? a : a .+ b)
+
+# Pullback of the whole fixed-dt solve (custom_ODE_solvers.jl:36-95; column k
+# of the result is the state after k steps).  When the cotangent reaches the
+# final state only (the inversion and UDE losses read the last column), one
+# device-resident reverse sweep over all the steps; otherwise the steps are
+# reversed one at a time, each column's cotangent joining the state's
+# cotangent after its step, and the parameter cotangents summed.
+function ChainRulesCore.rrule(::typeof(custom_ODE_solve), ode_f::DeviceRHS, Q0, params_vector, swe2d_extra_params)
+    sol = custom_ODE_solve(ode_f, Q0, params_vector, swe2d_extra_params)
+    ctx = context(swe2d_extra_params)
+    dt = swe2d_extra_params.swe_2D_constants.dt
+    q0 = Vector{Float64}(Q0); prm = flatparams(ctx, params_vector)
+    nI = length(swe2d_extra_params.inletQ_TotalQ)
+    K = size(sol, 2)
+    function solve_pullback(sbar)
+        S = sbar isa AbstractZero ? zeros(size(sol)) : Matrix{Float64}(unthunk(sbar))
+        if K == 1 || all(iszero, view(S, :, 1:K-1))
+            Qb, nb, zbb, qb = step_adjoint(ctx, q0, prm, dt, K, S[:, K], nI)
+            pb = params_bar(ctx, swe2d_extra_params, prm, nb, zbb, qb, params_vector)
+        else
+            # the parameter cotangent is linear in each step's outputs (and a
+            # network's weight cotangent is read right after its step), so the
+            # flat per-step cotangents add up
+            lam = S[:, K]
+            pflat = nothing
+            for k in K:-1:1
+                qk = k == 1 ? q0 : sol[:, k-1]
+                lam, nbk, zbbk, qbk = step_adjoint(ctx, qk, prm, dt, 1, lam, nI)
+                pflat = addto(pflat, params_bar(ctx, swe2d_extra_params, prm, nbk, zbbk, qbk))
+                k > 1 && (lam .+= view(S, :, k-1))
+            end
+            Qb = lam
+            pb = ctx.active == HG_PARAM_UDE ? lux_unflat(params_vector, pflat, ctx.lux_hidden) : pflat
+        end
+        return NoTangent(), NoTangent(), Qb, pb, NoTangent()
+    end
+    return sol, solve_pullback
 end
 
 end # module
