@@ -964,6 +964,37 @@ __device__ __forceinline__ hg::CellQ lds_cellq(const double* C, int r, double ha
   return q;
 }
 
+// A cell's flux inputs as the tiled reverse step forms them, for the tile's
+// own cells and for cross-tile neighbours alike (so a face's VJP has the same
+// bits whichever work-group evaluates it): make_cellq's clamp and pressure,
+// with u and v from one reciprocal of h and sqrt(h + eps) and its reciprocal
+// from one rsqrt.  rh = 1 / h, r2s = 0.5 / sqrt(h + eps).
+__device__ __forceinline__ hg::CellQ adj_cellq(double xi, double hs, double qx, double qy, double zb, double half_g,
+                                               double hsmall, double& rh, double& r2s) {
+  hg::CellQ c;
+  c.xi = xi;
+  c.hs = hs;
+  double h = xi + hs, hu = qx, hv = qy;
+  if (h <= hsmall) {
+    h = hsmall;
+    hu = 0.0;
+    hv = 0.0;
+  }
+  c.h = h;
+  c.hu = hu;
+  c.hv = hv;
+  c.zb = zb;
+  rh = 1.0 / h;
+  c.u = hu * rh;
+  c.v = hv * rh;
+  const double he = h + hg::EPS;
+  const double rs = rsqrt(he);
+  c.s = he * rs;
+  r2s = 0.5 * rs;
+  c.p = half_g * ((xi + hg::EPS) * (xi + hg::EPS) + 2.0 * xi * hs);
+  return c;
+}
+
 // A cross-tile neighbour's raw data as a work item of the tiled reverse step
 // reads it.
 struct NbrRaw {
@@ -1015,8 +1046,9 @@ __device__ __forceinline__ void adj_item(const hg::AdjArgs& a, int t, const doub
     const double rhL = C[11 * T + ia], r2L = C[12 * T + ia];
     hg::CellQ G;
     double zG0 = 0.0, zG1 = 0.0, zG2 = 0.0;
+    double rhG, r2G;
     if (type == 1) {  // cross-tile neighbour
-      G = hg::make_cellq(nr.xi, nr.hs, nr.qx, nr.qy, nr.zb, half_g, hsmall);
+      G = adj_cellq(nr.xi, nr.hs, nr.qx, nr.qy, nr.zb, half_g, hsmall, rhG, r2G);
       double m0 = nr.m0, m1 = nr.m1, m2 = nr.m2;
       if (nr.clamped) m0 = m1 = m2 = 0.0;
       const double rAj = 1.0 / nr.area;
@@ -1026,8 +1058,9 @@ __device__ __forceinline__ void adj_item(const hg::AdjArgs& a, int t, const doub
     } else {  // ghost (no cotangent of its own)
       n_a = a.n[t * T + ia];
       G = adj_ghost_state(a, Lc, n_a, rec.ref, nx, ny, gs);
+      rhG = 1.0 / G.h;
+      r2G = 0.5 / G.s;
     }
-    const double rhG = 1.0 / G.h, r2G = 0.5 / G.s;
     P = sw ? G : Lc;
     Q = sw ? Lc : G;
     zP0 = sw ? zG0 : zL0; zP1 = sw ? zG1 : zL1; zP2 = sw ? zG2 : zL2;
@@ -1088,7 +1121,8 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
   if (itB < it1) recB = a.items[itB];
   int d_i = 0;
   if (valid) {
-    const hg::CellQ me = hg::make_cellq(a.xi[i], a.hstill[i], a.qx[i], a.qy[i], a.zb[i], a.half_g, hsmall);
+    double rh, r2s;
+    const hg::CellQ me = adj_cellq(a.xi[i], a.hstill[i], a.qx[i], a.qy[i], a.zb[i], a.half_g, hsmall, rh, r2s);
     double l0 = a.lx[i], l1 = a.lqx[i], l2 = a.lqy[i];
     if (STEP && a.mask && a.mask[i]) l0 = l1 = l2 = 0.0;
     const double dti = a.dtc ? a.dtc[i] : a.dt;
@@ -1098,8 +1132,8 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
     C[8 * T + li] = (STEP ? dti * l0 : l0) * rA;
     C[9 * T + li] = (STEP ? dti * l1 : l1) * rA;
     C[10 * T + li] = (STEP ? dti * l2 : l2) * rA;
-    C[11 * T + li] = 1.0 / me.h;
-    C[12 * T + li] = 0.5 / me.s;
+    C[11 * T + li] = rh;
+    C[12 * T + li] = r2s;
     d_i = a.deg[i];
     for (int k = 0; k < MD; ++k) S[(3 * MD + k) * T + li] = 0.0;
     Xm[li] = 0;
