@@ -277,8 +277,9 @@ __host__ __device__ __forceinline__ void source_vjp(double g, double kn2, double
   const double p73 = pow_7_3(x);
   const double rp = 1.0 / (kn2 * p73);
   const double cc = g * (n * n) * rp;
-  const double s = sqrt(qx * qx + qy * qy + EPS);
-  const double rs = 1.0 / s;
+  const double q2 = qx * qx + qy * qy + EPS;
+  const double rs = rsqrt(q2);  // 1 / |q|_eps, and |q|_eps = q2 rs
+  const double s = q2 * rs;
   const double cb = fxb * s * qx + fyb * s * qy;
   const double sb = fxb * cc * qx + fyb * cc * qy;
   qxb += fxb * cc * s + sb * qx * rs;
@@ -297,7 +298,7 @@ __host__ __device__ __forceinline__ void source_vjp_mag(double g, double xi, dou
   S0xb = g * xi * y1;
   S0yb = g * xi * y2;
   const double fxb = -y1, fyb = -y2;
-  const double rs = 1.0 / sqrt(qx * qx + qy * qy + EPS);
+  const double rs = rsqrt(qx * qx + qy * qy + EPS);
   const double dq = fxb * qx + fyb * qy;
   mb += dq * rs;
   const double sb = -dq * m * rs * rs;  // through |q|_eps
