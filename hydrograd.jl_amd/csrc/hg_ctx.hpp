@@ -364,6 +364,12 @@ struct hg_ctx {
   // the caller does not ask for them
   bool adj_want_n = true, adj_want_zb = true;
   int64_t last_kck = 0;  // checkpoint interval of the last hg_adjoint / hg_grad_probes call
+  // a forward sweep recorded by hg_adjoint_forward, awaiting hg_adjoint_reverse
+  // (cleared by any other gradient call and by the calls that set fields or
+  // parameters)
+  bool adj_pending = false;
+  double pend_dt = 0.0;
+  int64_t pend_nsteps = 0, pend_kck = 0;
   // adjoint work buffers (allocated on first use)
   double* adj_lam[2] = {nullptr, nullptr};  // 3N each
   double *adj_nbar = nullptr, *adj_zbbar = nullptr, *adj_S0bar = nullptr, *adj_vbar = nullptr, *adj_Qbar = nullptr;

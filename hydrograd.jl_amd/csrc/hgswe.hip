@@ -3719,6 +3719,7 @@ int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost,
                   const double* zb_ghost, const double* S0x, const double* S0y, const double* n,
                   const double* inletQ, const double* exitWSE) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if (!hstill || !zb || !S0x || !S0y || !n || (ctx->G && (!hstill_ghost || !zb_ghost)) ||
       (ctx->n_inlet && !inletQ) || (ctx->n_exit && !exitWSE)) {
     ctx->err = "hg_set_fields: NULL array";
@@ -3757,6 +3758,7 @@ int hg_set_fields(hg_ctx* ctx, const double* hstill, const double* hstill_ghost,
 
 int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, int32_t n_zones) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if ((kind == HG_PARAM_ZB || kind == HG_PARAM_N_ZONE || kind == HG_PARAM_UDE) &&
       refuse_fields(ctx, "hg_set_active_param (zb, zone n or network weights)"))
     return HG_ERR_STATE;
@@ -3807,6 +3809,7 @@ int hg_set_active_param(hg_ctx* ctx, int32_t kind, const int32_t* zone_of_cell, 
 
 int hg_set_params(hg_ctx* ctx, const double* params, int64_t n_params) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if (!ctx->fields_set) {
     ctx->err = "hg_set_fields has not been called";
     return HG_ERR_STATE;
@@ -3960,6 +3963,7 @@ int hg_cfl_dt_noalloc(hg_ctx* ctx, const double* Q, double cfl, double* dt) {
 int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t* set_of_cell,
                int32_t precision) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   HG_TRY(hipStreamSynchronize(ctx->stream));
   if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the closure
   for (void* p : ctx->ude_allocs) {
@@ -4099,6 +4103,7 @@ int hg_ude_grad(hg_ctx* ctx, double* w_bar) {
 
 int hg_set_n_law(hg_ctx* ctx, int32_t kind, const double* params, const double* ks) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if (kind != HG_NLAW_OFF && refuse_fields(ctx, "hg_set_n_law")) return HG_ERR_STATE;
   if (kind < HG_NLAW_OFF || kind > HG_NLAW_H_UMAG_KS) {
     ctx->err = "hg_set_n_law: unknown law";
@@ -4270,6 +4275,7 @@ int64_t hg_lux_n_params(const hg_lux_net* net) {
 
 int hg_set_lux_ude(hg_ctx* ctx, const hg_lux_net* net, const double* params, const double* ks) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if (net && refuse_fields(ctx, "hg_set_lux_ude")) return HG_ERR_STATE;
   HG_TRY(hipStreamSynchronize(ctx->stream));
   if (drop_graph(ctx)) return HG_ERR_HIP;  // captured steps may include (or lack) the network
@@ -4745,6 +4751,7 @@ int hg_time_advance(hg_ctx* ctx, double dt, int64_t nsteps, int32_t reps, double
 
 int hg_batch_set_dt(hg_ctx* ctx, const double* dt_inst) {
   if (!ctx) return HG_ERR_ARG;
+  ctx->adj_pending = false;  // a recorded forward sweep no longer matches
   if (dt_inst && refuse_fields(ctx, "hg_batch_set_dt")) return HG_ERR_STATE;
   HG_TRY(hipStreamSynchronize(ctx->stream));
   if (drop_graph(ctx)) return HG_ERR_HIP;  // captured launches hold the previous step pointer
@@ -4980,6 +4987,7 @@ static int adj_run(hg_ctx* ctx, const double* Q0, const double* params, int64_t 
   }
   const int64_t nseg = (nsteps + kck - 1) / kck;
   ctx->last_kck = kck;
+  ctx->adj_pending = false;
   int rc = hg_set_params(ctx, params, n_params);
   if (!rc) rc = ensure_adj(ctx, kck, nseg);
   if (!rc) rc = hg_state_upload(ctx, Q0);
@@ -5000,6 +5008,42 @@ int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_pa
   if (!rc) rc = zero_adj_accumulators(ctx);
   int cur = 0;
   if (!rc) rc = reverse_sweep(ctx, dt, nsteps, kck, cur);
+  if (!rc) rc = finish_adjoint(ctx);
+  if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], Q0_bar, n_bar, zb_bar, inletQ_bar);
+  if (!rc) rc = check_flags(ctx);
+  return rc;
+}
+
+int hg_adjoint_forward(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt,
+                       int64_t nsteps, int64_t kck, double* QN) {
+  if (!ctx || !Q0) return ctx ? (ctx->err = "hg_adjoint_forward: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (kck <= 0 && nsteps > 0) kck = auto_kck(ctx, nsteps);
+  int rc = adj_run(ctx, Q0, params, n_params, dt, nsteps, kck);
+  if (!rc) rc = check_flags(ctx);
+  if (!rc && QN) rc = hg_state_download(ctx, QN);
+  if (rc) return rc;
+  ctx->adj_pending = true;
+  ctx->pend_dt = dt;
+  ctx->pend_nsteps = nsteps;
+  ctx->pend_kck = kck;
+  return HG_OK;
+}
+
+int hg_adjoint_reverse(hg_ctx* ctx, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar,
+                       double* inletQ_bar) {
+  if (!ctx || !QN_bar) return ctx ? (ctx->err = "hg_adjoint_reverse: NULL array", HG_ERR_ARG) : HG_ERR_ARG;
+  if (!ctx->adj_pending) {
+    ctx->err = "hg_adjoint_reverse: no forward sweep recorded (hg_adjoint_forward must come right before, with no "
+               "gradient call or field / parameter change in between)";
+    return HG_ERR_STATE;
+  }
+  ctx->adj_pending = false;  // the reversal recomputes into the segment buffers the forward sweep recorded
+  ctx->adj_want_n = n_bar != nullptr;
+  ctx->adj_want_zb = zb_bar != nullptr;
+  int rc = adj_upload_lam(ctx, QN_bar, ctx->adj_lam[0]);
+  if (!rc) rc = zero_adj_accumulators(ctx);
+  int cur = 0;
+  if (!rc) rc = reverse_sweep(ctx, ctx->pend_dt, ctx->pend_nsteps, ctx->pend_kck, cur);
   if (!rc) rc = finish_adjoint(ctx);
   if (!rc) rc = adj_download(ctx, ctx->adj_lam[cur], Q0_bar, n_bar, zb_bar, inletQ_bar);
   if (!rc) rc = check_flags(ctx);

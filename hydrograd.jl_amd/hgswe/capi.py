@@ -35,7 +35,8 @@ EXPORTS = [
     "hg_create", "hg_destroy", "hg_last_error", "hg_set_fields", "hg_set_active_param", "hg_rhs", "hg_step",
     "hg_set_params", "hg_state_upload", "hg_state_download", "hg_advance", "hg_cfl_dt", "hg_time_advance",
     "hg_info", "hg_probe_copy", "hg_rhs_vjp", "hg_adjoint", "hg_grad_probes", "hg_batch_create",
-    "hg_set_tile_config", "hg_get_tile_config", "hg_set_adjoint_tile", "hg_last_checkpoint_interval", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
+    "hg_set_tile_config", "hg_get_tile_config", "hg_set_adjoint_tile", "hg_last_checkpoint_interval", "hg_adjoint_forward",
+    "hg_adjoint_reverse", "hg_autotune", "hg_set_ude", "hg_ude_n", "hg_ude_grad",
     "hg_mfma_probe", "hg_tsit5_solve", "hg_tsit5_tableau", "hg_jvp", "hg_batch_set_dt", "hg_set_n_law",
     "hg_n_law_eval", "hg_cell_n", "hg_lux_n_params", "hg_set_lux_ude", "hg_lux_eval", "hg_lux_grad",
     "hg_advance_timed", "hg_advance_region", "hg_prepare_advance", "hg_cfl_dt_noalloc", "hg_fields_create",
@@ -313,6 +314,26 @@ class Context:
                                    C.c_int64(nsteps), C.c_int64(kck), _ptr(lb), _ptr(qb), _ptr(nb), _ptr(zb),
                                    _ptr(ib)))
         return qb, nb, zb, ib[: self.case.mesh.n_inlet]
+
+    def adjoint_forward(self, Q0, dt, nsteps, kck, params=None):
+        """The recorded forward sweep of adjoint(): returns the final state."""
+        Q0 = np.ascontiguousarray(Q0, dtype=np.float64)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        QN = np.empty(3 * self.N)
+        self._chk(lib().hg_adjoint_forward(self.h, _ptr(Q0), _ptr(p), 0 if p is None else p.size, C.c_double(dt),
+                                           C.c_int64(nsteps), C.c_int64(kck), _ptr(QN)))
+        return QN
+
+    def adjoint_reverse(self, QN_bar, wrt=("n", "zb", "Q")):
+        """The reverse sweep after adjoint_forward: (Q0_bar, n_bar, zb_bar,
+        inletQ_bar); a cotangent not named in `wrt` is not formed (None)."""
+        lb = np.ascontiguousarray(QN_bar, dtype=np.float64)
+        qb = np.empty(3 * self.N)
+        nb = np.empty(self.N) if "n" in wrt else None
+        zb = np.empty(self.N) if "zb" in wrt else None
+        ib = np.zeros(max(1, self.case.mesh.n_inlet))
+        self._chk(lib().hg_adjoint_reverse(self.h, _ptr(lb), _ptr(qb), _ptr(nb), _ptr(zb), _ptr(ib)))
+        return qb, nb, zb, (ib[: self.case.mesh.n_inlet] if "Q" in wrt else None)
 
     def grad_probes(self, Q0, dt, nsteps, kck, probe, target, params=None, wrt=("n", "zb", "Q")):
         """(loss, d/dn, d/dzb, d/dQ_inlet) of sum_p (WSE_p - target_p)^2 after

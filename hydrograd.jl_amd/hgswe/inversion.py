@@ -341,11 +341,14 @@ class InversionResult:
 
 def loss_and_grad(ctx, case, truth, params, active, dt, nsteps, kck, settings: LossSettings, zone=None, G=None):
     """Loss of the final state after nsteps Euler steps and its gradient
-    w.r.t. the active parameter vector (zone n, per-cell n, zb or inlet Q)."""
-    QN = ctx.step(case.Q0, dt, nsteps, params=params)
+    w.r.t. the active parameter vector (zone n, per-cell n, zb or inlet Q).
+    One recorded forward sweep serves both the loss and the reversal."""
+    QN = ctx.adjoint_forward(case.Q0, dt, nsteps, kck, params=params)
     zb = params if active == capi.HG_PARAM_ZB else case.zb
     total, parts, QN_bar, pb, zbd = final_state_loss(case, QN, truth, settings, params=params, zb=zb, G=G)
-    _, nb, zbb, qb = ctx.adjoint(case.Q0, dt, nsteps, kck, QN_bar, params=params)
+    wrt = {capi.HG_PARAM_N_ZONE: ("n",), capi.HG_PARAM_N_CELL: ("n",), capi.HG_PARAM_ZB: ("zb",),
+           capi.HG_PARAM_Q: ("Q",)}.get(active, ())
+    _, nb, zbb, qb = ctx.adjoint_reverse(QN_bar, wrt=wrt)
     if active == capi.HG_PARAM_N_ZONE:
         g = np.bincount(zone, weights=nb, minlength=np.size(params))
     elif active == capi.HG_PARAM_N_CELL:
@@ -359,7 +362,7 @@ def loss_and_grad(ctx, case, truth, params, active, dt, nsteps, kck, settings: L
     return total, parts, g + pb
 
 
-def invert(ctx, case, truth, params0, active, dt, nsteps, kck=50, lr=1e-3, iters=100, settings=None, zone=None,
+def invert(ctx, case, truth, params0, active, dt, nsteps, kck=0, lr=1e-3, iters=100, settings=None, zone=None,
            callback=None):
     """Adam on the reference's inversion loss; returns the parameter and
     loss history (the callback bookkeeping of swe_2D_inversion.jl:600-640)."""
@@ -384,7 +387,7 @@ def invert(ctx, case, truth, params0, active, dt, nsteps, kck=50, lr=1e-3, iters
     return res
 
 
-def invert_staged(ctx, case, truth, params0, active, dt, nsteps, stages, kck=50, settings=None, zone=None):
+def invert_staged(ctx, case, truth, params0, active, dt, nsteps, stages, kck=0, settings=None, zone=None):
     """The reference's optimiser schedule on the inversion loss, e.g.
     stages = [("Adam", 2e-3, 60), ("LBFGS", None, 20)] as the examples'
     run_control.json lists ["Adam", "LBFGS"] (swe_2D_inversion.jl:664-729).
@@ -424,14 +427,14 @@ def ude_loss_and_grad(ctx, case, truth, weights, dt, nsteps, kck, settings=None)
     truth: WSE_truth, u_truth, v_truth and optionally zb_cell_truth (the bed
     the reference adds to the predicted depth)."""
     settings = settings or ude_settings()
-    QN = ctx.step(case.Q0, dt, nsteps, params=weights)
+    QN = ctx.adjoint_forward(case.Q0, dt, nsteps, kck, params=weights)
     zb = truth.get("zb_cell_truth", case.zb)
     total, parts, QN_bar, _, _ = final_state_loss(case, QN, truth, settings, zb=zb)
-    ctx.adjoint(case.Q0, dt, nsteps, kck, QN_bar, params=weights)
+    ctx.adjoint_reverse(QN_bar, wrt=())
     return total, parts, ctx.lux_grad()
 
 
-def train_ude(ctx, case, truth, weights0, dt, nsteps, kck=50, lr=0.1, iters=100, settings=None, callback=None):
+def train_ude(ctx, case, truth, weights0, dt, nsteps, kck=0, lr=0.1, iters=100, settings=None, callback=None):
     """Adam on the reference's UDE loss (the training loop of
     swe_2D_UDE.jl:360-420 with its Adam stage); returns the weights and loss
     history."""
@@ -451,7 +454,7 @@ def train_ude(ctx, case, truth, weights0, dt, nsteps, kck=50, lr=0.1, iters=100,
     return res
 
 
-def train_ude_staged(ctx, case, truth, weights0, dt, nsteps, stages, kck=50, settings=None):
+def train_ude_staged(ctx, case, truth, weights0, dt, nsteps, stages, kck=0, settings=None):
     """The reference's UDE optimiser schedule (swe_2D_UDE.jl:263-320), e.g.
     stages = [("Adam", 0.1, 100), ("LBFGS", None, 50)]; every loss
     evaluation is recorded."""

@@ -287,6 +287,20 @@ int hg_rhs_vjp(hg_ctx* ctx, const double* Q, const double* params, int64_t n_par
 int hg_adjoint(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt, int64_t nsteps,
                int64_t kck, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar, double* inletQ_bar);
 
+/* hg_adjoint in two calls, for a loss formed by the caller from the final
+ * state (swe_2D_inversion.jl:247-483 and swe_2D_UDE.jl form theirs on the
+ * host): hg_adjoint_forward runs and records the forward sweep of hg_adjoint
+ * (QN: the final state, caller order; NULL to skip), hg_adjoint_reverse then
+ * takes the final state's cotangent and returns what hg_adjoint returns,
+ * without running the forward sweep again.  The reverse must follow its
+ * forward with no other gradient call and no call that sets fields or
+ * parameters in between (else HG_ERR_STATE); one reverse per forward.  The
+ * results are bitwise those of hg_adjoint. */
+int hg_adjoint_forward(hg_ctx* ctx, const double* Q0, const double* params, int64_t n_params, double dt,
+                       int64_t nsteps, int64_t kck, double* QN);
+int hg_adjoint_reverse(hg_ctx* ctx, const double* QN_bar, double* Q0_bar, double* n_bar, double* zb_bar,
+                       double* inletQ_bar);
+
 /* The checkpoint interval the last hg_adjoint / hg_grad_probes call used
  * (what kck <= 0 resolved to; 0 before any such call).  No reference
  * counterpart. */

@@ -211,6 +211,41 @@ def test_unrequested_cotangents_are_skipped_without_changing_the_others():
         assert np.array_equal(a, b)
 
 
+def test_split_adjoint_is_the_adjoint():
+    """hg_adjoint_forward then hg_adjoint_reverse (a loss formed from the
+    final state between them): the final state is bitwise the device
+    forward run's and the cotangents bitwise hg_adjoint's, for a fixed and
+    the library's checkpoint interval; the reverse needs its own forward right
+    before it (HG_ERR_STATE after another gradient call, a parameter change,
+    or a second reverse)."""
+    c = small_channel()
+    nsteps = 40
+    ctx = capi.Context(c)
+    lamN = np.random.default_rng(17).normal(size=3 * c.N)
+    QN_ref = capi.Context(c).step(c.Q0, c.dt, nsteps)
+    for kck in (7, 0):
+        ref = ctx.adjoint(c.Q0, c.dt, nsteps, kck, lamN)
+        QN = ctx.adjoint_forward(c.Q0, c.dt, nsteps, kck)
+        assert np.array_equal(QN, QN_ref)
+        out = ctx.adjoint_reverse(lamN)
+        for a, b in zip(out, ref):
+            assert np.array_equal(a, b), kck
+    with pytest.raises(capi.HGError) as e:
+        ctx.adjoint_reverse(lamN)
+    assert e.value.code == capi.HG_ERR_STATE
+    ctx.adjoint_forward(c.Q0, c.dt, nsteps, 7)
+    probe = np.arange(10, dtype=np.int32)
+    ctx.grad_probes(c.Q0, c.dt, 5, 5, probe, QN_ref[probe])
+    with pytest.raises(capi.HGError) as e:
+        ctx.adjoint_reverse(lamN)
+    assert e.value.code == capi.HG_ERR_STATE
+    ctx.adjoint_forward(c.Q0, c.dt, nsteps, 7)
+    ctx.set_params(None)
+    with pytest.raises(capi.HGError) as e:
+        ctx.adjoint_reverse(lamN)
+    assert e.value.code == capi.HG_ERR_STATE
+
+
 def test_checkpoint_interval_invariance_and_determinism():
     c = small_channel()
     nsteps = 40
