@@ -111,7 +111,7 @@ struct TileArgs {
   const double* S0x;
   const double* S0y;
   const double* area;
-  const int32_t* deg;
+  const int32_t* deg;  // faces per cell, or NULL: every cell has MD faces
   const int32_t* item_off;
   const struct ItemRec* items;
   const int32_t* tile_in_off;   // [ntiles+1] into tile_in_list
@@ -162,7 +162,7 @@ struct AdjArgs {
   const double* S0x;
   const double* S0y;
   const double* area;
-  const int32_t* deg;
+  const int32_t* deg;  // faces per cell, or NULL (tiled reverse step): every cell has MD faces
   const int32_t* nbr;
   const double* fnx;
   const double* fny;
@@ -329,6 +329,7 @@ struct hg_ctx {
   int32_t* item_off = nullptr;
   hg::ItemRec* items = nullptr;
   int32_t tileMD = 3;
+  bool deg_full = false;  // every cell has tileMD faces: the tile kernels need not read deg
   int32_t xcd_chunk = 0;  // identity block -> tile mapping by default (the autotuner re-checks)
   int32_t ncu = 256;      // compute units of the device
   // the reverse step's tiling (its own tile size; the step kernel's items
@@ -392,8 +393,10 @@ struct hg_ctx {
   double* ude_wpart = nullptr;      // [ude_nslots][353] weight cotangents per (block, set) slot
   double* ude_wbar = nullptr;       // accumulated weight cotangents [ude_nsets][353]
   int32_t ude_nslots = 0;           // slots, ordered by set then block
-  int32_t* ude_blk_off = nullptr;   // [blocks + 1] first entry of each 64-cell block in ude_blk_slot
-  int32_t* ude_blk_slot = nullptr;  // slot of each (block, set) pair, a block's sets in order of first appearance
+  int32_t ude_cpb = 1;              // 64-cell chunks per reverse-mode block
+  int32_t* ude_blk_off = nullptr;   // [blocks + 1] first entry of each reverse-mode block in ude_blk_set / _slot
+  int32_t* ude_blk_set = nullptr;   // set of each (block, set) pair, a block's sets in order of first appearance
+  int32_t* ude_blk_slot = nullptr;  // slot of each (block, set) pair
   int32_t* ude_set_start = nullptr; // [ude_nsets + 1] first slot of each set
   std::vector<void*> ude_allocs;
 

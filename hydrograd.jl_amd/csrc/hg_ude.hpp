@@ -40,23 +40,27 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
+  // The f32 network's own products and sums fuse (one rounding): its values
+  // are checked against a float64 evaluation to f32 tolerance, not bit for
+  // bit against the reference's Float64 arithmetic, which the solver keeps.
+  static __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
   // tanh in f32 from its definitions: below |x| = 0.4 the odd Taylor series
   // to x^13 (truncation < 4e-9 relative), above it 1 - 2 / (e^(2|x|) + 1)
   // with the hardware exp and reciprocal; within 4 ulp of tanh over the
-  // whole range (checked against float64 tanh on the host), about half the
-  // instructions of tanhf.  The closure's 32 activations per cell dominate
-  // its forward and reverse kernels.
+  // whole range (checked against float64 tanh on the host), about a third of
+  // the instructions of tanhf.  The closure's 32 activations per cell
+  // dominate its forward and reverse kernels.
   static __device__ __forceinline__ float tanh_(float x) {
     const float ax = fabsf(x), x2 = x * x;
     float q = 21844.0f / 6081075.0f;
-    q = -1382.0f / 155925.0f + x2 * q;
-    q = 62.0f / 2835.0f + x2 * q;
-    q = -17.0f / 315.0f + x2 * q;
-    q = 2.0f / 15.0f + x2 * q;
-    q = -1.0f / 3.0f + x2 * q;
-    const float p = x + x * x2 * q;
+    q = fma_(x2, q, -1382.0f / 155925.0f);
+    q = fma_(x2, q, 62.0f / 2835.0f);
+    q = fma_(x2, q, -17.0f / 315.0f);
+    q = fma_(x2, q, 2.0f / 15.0f);
+    q = fma_(x2, q, -1.0f / 3.0f);
+    const float p = fma_(x * x2, q, x);
     const float e = __expf(2.0f * ax);
-    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    const float t = fma_(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
     return ax < 0.4f ? p : copysignf(t, x);
   }
   static __device__ __forceinline__ float exp_(float x) { return expf(x); }
@@ -70,6 +74,7 @@ struct Mfma<double> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int kq, int r) { return kq + 4 * r; }
+  static __device__ __forceinline__ double fma_(double a, double b, double c) { return a * b + c; }
   static __device__ __forceinline__ double tanh_(double x) { return tanh(x); }
   static __device__ __forceinline__ double exp_(double x) { return exp(x); }
   static __device__ __forceinline__ double log1p_(double x) { return log1p(x); }
@@ -111,9 +116,11 @@ struct UdeArgs {
   double* oqy;
   double* zbbar;               // optional, += d/d zb
   double* wpart;               // [nslots][kUdeW] weight cotangents, one slot per (block, set) pair
-  const int32_t* blk_off;      // slots of block b: blk_slot[blk_off[b] ...], its sets in order of
-  const int32_t* blk_slot;     //   first appearance; both NULL: slot = block (one set)
+  const int32_t* blk_off;      // entries of block b: blk_off[b] .. blk_off[b + 1] - 1, one per set
+  const int32_t* blk_set;      //   its cells use (order of first appearance): that set and its
+  const int32_t* blk_slot;     //   slot; all NULL: slot = block (one set)
   int32_t nslots;
+  int32_t cpb;                 // 64-cell chunks per reverse-mode block
   double g_unused;
   double hsmall;
 };

@@ -346,7 +346,7 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a0) {
     S0x_i = a.S0x[i];
     S0y_i = a.S0y[i];
     A_i = a.area[i];
-    d_i = a.deg[i];
+    d_i = a.deg ? a.deg[i] : MD;
   }
   hg::CellQ me{};
   if (valid) {
@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
     C[10 * T + li] = (STEP ? dti * l2 : l2) * rA;
     C[11 * T + li] = rh;
     C[12 * T + li] = r2s;
-    d_i = a.deg[i];
+    d_i = a.deg ? a.deg[i] : MD;
     for (int k = 0; k < MD; ++k) S[(3 * MD + k) * T + li] = 0.0;
     Xm[li] = 0;
   }
@@ -1209,11 +1209,15 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
   // Per inlet: the per-face terms are formed in parallel and summed by a
   // fixed tree; each cell's faces of the inlet are applied, in NodeString
   // order, by the work-item of its first face (rec.first; rec.next: the
-  // cell's next face in the inlet, or -1).  A work-item reads the record,
-  // cell data and ghost cotangent of its first face once, before the sums.
-  // With one work-group, the inlets are taken in order (a cell may sit on
-  // two of them); when no cell does, work-group b takes inlet b alone (same
+  // cell's next face in the inlet, or -1).  A cell's faces all belong to it,
+  // so its depth and n are read once.  A work-item reads the record, cell data
+  // and ghost cotangent of its first face, and for a cell's first face the
+  // cell's cotangent values and its next face's record, before the sums; each
+  // cotangent value is then read once and written once.  With one
+  // work-group, the inlets are taken in order (a cell may sit on two of
+  // them); when no cell does, work-group b takes inlet b alone (same
   // arithmetic, same bits).
+  static_assert(kBlock == 256, "the tree below is written for 256 work-items");
   __shared__ double sA[kBlock], sQ[kBlock];
   const int tid = threadIdx.x;
   const int b0 = gridDim.x > 1 ? int(blockIdx.x) : 0;
@@ -1229,48 +1233,84 @@ __global__ __launch_bounds__(kBlock) void k_inlet_adj(int n_inlet, int nin_m, in
     const double A = totalA[b], Qb = inletQ[b];
     const int f0 = off[bl], f1 = off[bl + 1];
     const int fmine = f0 + tid;
-    hg::InletFaceRec r0{};
-    double n0 = 1.0, h0 = 0.0;
+    const double qbar0 = tid == 0 ? Qbar[b] : 0.0;
+    hg::InletFaceRec r0{}, rn{};
+    double n0 = 1.0, h0 = 0.0, ox0 = 0.0, nb0 = 0.0;
     double pa = 0.0, pq = 0.0;
-    for (int f = fmine; f < f1; f += kBlock) {
+    if (fmine < f1) {
+      r0 = rec[fmine];
+      const int c = r0.cell;
+      n0 = n[c];
+      h0 = xi[c] + hstill[c];
+      if (r0.first) {
+        ox0 = ox[c];
+        if (nbar) nb0 = nbar[c];
+        if (r0.next >= 0) rn = rec[r0.next];
+      }
+      const double vb = vbar[r0.ghost];
+      const double v = Qb / A * r0.L23 / n0;
+      pa += vb * (-v / A);
+      pq += vb * (r0.L23 / (A * n0));
+    }
+    for (int f = fmine + kBlock; f < f1; f += kBlock) {
       const hg::InletFaceRec r = rec[f];
       const double nc = n[r.cell];
       const double vb = vbar[r.ghost];
-      if (f == fmine) {
-        r0 = r;
-        n0 = nc;
-        h0 = xi[r.cell] + hstill[r.cell];
-      }
       const double v = Qb / A * r.L23 / nc;
       pa += vb * (-v / A);
       pq += vb * (r.L23 / (A * nc));
     }
+    // pairwise tree: level w adds entry t + w into entry t (t < w), w = 128 .. 1;
+    // the levels below 64 stay in wave 0 and run as lane exchanges
     sA[tid] = pa;
     sQ[tid] = pq;
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        sA[tid] += sA[tid + w];
-        sQ[tid] += sQ[tid + w];
-      }
-      __syncthreads();
+    if (tid < 128) {
+      sA[tid] += sA[tid + 128];
+      sQ[tid] += sQ[tid + 128];
     }
+    __syncthreads();
+    if (tid < 64) {
+      double xa = sA[tid] + sA[tid + 64], xq = sQ[tid] + sQ[tid + 64];
+      for (int w = 32; w > 0; w >>= 1) {
+        xa = xa + __shfl_xor(xa, w, 64);
+        xq = xq + __shfl_xor(xq, w, 64);
+      }
+      if (tid == 0) {
+        sA[0] = xa;
+        Qbar[b] = qbar0 + xq;
+      }
+    }
+    __syncthreads();
     const double Ab = sA[0];
-    if (tid == 0) Qbar[b] += sQ[0];
-    for (int f = fmine; f < f1; f += kBlock) {
-      const hg::InletFaceRec r = f == fmine ? r0 : rec[f];
-      if (!r.first) continue;
-      for (int g = f; g >= 0;) {
-        const hg::InletFaceRec rg = g == f ? r : rec[g];
-        const int c = rg.cell;
-        const double h = g == fmine ? h0 : xi[c] + hstill[c];
-        const double nc = g == fmine ? n0 : n[c];
-        if (h > hsmall) {  // else w = 0 and h clamped: no derivative
-          ox[c] += Ab * rg.L53 / nc;
-          if (nbar) nbar[c] += Ab * (-rg.L53 * h / (nc * nc));
-        }
+    if (fmine < f1 && r0.first && h0 > hsmall) {  // else w = 0 and h clamped: no derivative
+      double ov = ox0, nv = nb0;
+      ov += Ab * r0.L53 / n0;
+      if (nbar) nv += Ab * (-r0.L53 * h0 / (n0 * n0));
+      for (int g = r0.next; g >= 0;) {
+        const hg::InletFaceRec rg = g == r0.next ? rn : rec[g];
+        ov += Ab * rg.L53 / n0;
+        if (nbar) nv += Ab * (-rg.L53 * h0 / (n0 * n0));
         g = rg.next;
       }
+      ox[r0.cell] = ov;
+      if (nbar) nbar[r0.cell] = nv;
+    }
+    for (int f = fmine + kBlock; f < f1; f += kBlock) {  // inlets of more than 256 faces
+      const hg::InletFaceRec r = rec[f];
+      if (!r.first) continue;
+      const int c = r.cell;
+      const double h = xi[c] + hstill[c], nc = n[c];
+      if (!(h > hsmall)) continue;
+      double ov = ox[c], nv = nbar ? nbar[c] : 0.0;
+      for (int g = f; g >= 0;) {
+        const hg::InletFaceRec rg = g == f ? r : rec[g];
+        ov += Ab * rg.L53 / nc;
+        if (nbar) nv += Ab * (-rg.L53 * h / (nc * nc));
+        g = rg.next;
+      }
+      ox[c] = ov;
+      if (nbar) nbar[c] = nv;
     }
     __syncthreads();  // sA / sQ reuse; cells shared by two inlets take them in inlet order
   }
@@ -1346,29 +1386,31 @@ std::vector<int32_t> morton_perm(const std::vector<int64_t>& seg, const double* 
 
 // ---- UDE closure for Manning's n (matrix cores; one wave = 64 cells) ----
 
-// Hidden layers of 64 cells: features F[64][4] -> A1 = tanh(F W1^T + b1)
-// -> A2 = tanh(A1 W2^T + b2), each layer as 16-row blocks of
-// 16x16x4 matrix-core products; results land in LDS row-major [cell][unit].
+// Hidden layers of 64 cells: features F[64][4] (column 3 = 1) -> A1 =
+// tanh(F [W1 | b1]^T) -> A2 = tanh(A1 W2^T + b2), each layer as 16-row blocks
+// of 16x16x4 matrix-core products (b2 as their initial accumulator); results
+// land in LDS row-major [cell][unit].
 template <class T>
 __device__ __forceinline__ void ude_hidden(const T* __restrict__ W, const T* F, T* A1, T* A2, int l) {
   using M = hg::Mfma<T>;
   const int col = l & 15, kq = l >> 4;
   for (int rb = 0; rb < 4; ++rb) {
-    const T av = F[(16 * rb + col) * 4 + kq];                         // F[16 rb + i][k]
-    const T bv = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : T(0);         // W1[j][k]
+    const T av = F[(16 * rb + col) * 4 + kq];                                    // F[16 rb + i][k]
+    const T bv = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : W[hg::kUdeB1 + col];  // [W1 | b1][j][k]
     typename M::V acc = {T(0), T(0), T(0), T(0)};
     acc = M::op(av, bv, acc);
-    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + W[hg::kUdeB1 + col]);
+    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r]);
   }
   hg::wave_sync();
+  const T b2 = W[hg::kUdeB2 + col];
   for (int rb = 0; rb < 4; ++rb) {
-    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    typename M::V acc = {b2, b2, b2, b2};
     for (int kc = 0; kc < 4; ++kc) {
       const T av = A1[(16 * rb + col) * 16 + 4 * kc + kq];           // A1[16 rb + i][4 kc + k]
       const T bv = W[hg::kUdeW2 + col * 16 + 4 * kc + kq];           // W2[j][4 kc + k]
       acc = M::op(av, bv, acc);
     }
-    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + W[hg::kUdeB2 + col]);
+    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r]);
   }
   hg::wave_sync();
 }
@@ -1377,7 +1419,7 @@ __device__ __forceinline__ void ude_hidden(const T* __restrict__ W, const T* F, 
 template <class T>
 __device__ __forceinline__ T ude_out(const T* __restrict__ W, const T* A2, int l) {
   T z3 = W[hg::kUdeB3];
-  for (int j = 0; j < 16; ++j) z3 = z3 + A2[l * 16 + j] * W[hg::kUdeW3 + j];
+  for (int j = 0; j < 16; ++j) z3 = hg::Mfma<T>::fma_(A2[l * 16 + j], W[hg::kUdeW3 + j], z3);
   return z3;
 }
 
@@ -1385,10 +1427,9 @@ __device__ __forceinline__ T ude_out(const T* __restrict__ W, const T* A2, int l
 // matrix-core operands (see ude_hidden) and for the output layer.
 template <class T>
 struct UdeRegs {
-  T w1, b1, w2[4], b2, w3[16], b3;
+  T w1, w2[4], b2, w3[16], b3;
   __device__ __forceinline__ void load(const T* __restrict__ W, int col, int kq) {
-    w1 = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : T(0);
-    b1 = W[hg::kUdeB1 + col];
+    w1 = kq < 3 ? W[hg::kUdeW1 + col * 3 + kq] : W[hg::kUdeB1 + col];  // [W1 | b1]
     for (int kc = 0; kc < 4; ++kc) w2[kc] = W[hg::kUdeW2 + col * 16 + 4 * kc + kq];
     b2 = W[hg::kUdeB2 + col];
     for (int j = 0; j < 16; ++j) w3[j] = W[hg::kUdeW3 + j];
@@ -1405,13 +1446,13 @@ __device__ __forceinline__ void ude_hidden_r(const UdeRegs<T>& R, const T* F, T*
   for (int rb = 0; rb < ROWS / 16; ++rb) {
     typename M::V acc = {T(0), T(0), T(0), T(0)};
     acc = M::op(F[(16 * rb + col) * 4 + kq], R.w1, acc);
-    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b1);
+    for (int r = 0; r < 4; ++r) A1[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r]);
   }
   hg::wave_sync();
   for (int rb = 0; rb < ROWS / 16; ++rb) {
-    typename M::V acc = {T(0), T(0), T(0), T(0)};
+    typename M::V acc = {R.b2, R.b2, R.b2, R.b2};
     for (int kc = 0; kc < 4; ++kc) acc = M::op(A1[(16 * rb + col) * 16 + 4 * kc + kq], R.w2[kc], acc);
-    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r] + R.b2);
+    for (int r = 0; r < 4; ++r) A2[(16 * rb + M::row(kq, r)) * 16 + col] = M::tanh_(acc[r]);
   }
   hg::wave_sync();
 }
@@ -1450,7 +1491,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
       F[l * 4 + 0] = T(h);
       F[l * 4 + 1] = T(um);
       F[l * 4 + 2] = T(zb);
-      F[l * 4 + 3] = T(0);
+      F[l * 4 + 3] = T(1);  // the bias column of layer 1
     }
     hg::wave_sync();
     uint64_t pending = __ballot(valid);
@@ -1466,7 +1507,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
       ude_hidden_r<T, CH>(R, F, A1, A2, l);
       if (mine) {
         T z3 = R.b3;
-        for (int j = 0; j < 16; ++j) z3 = z3 + A2[l * 16 + j] * R.w3[j];
+        for (int j = 0; j < 16; ++j) z3 = hg::Mfma<T>::fma_(A2[l * 16 + j], R.w3[j], z3);
         a.n[i] = 0.01 + double(hg::softplus<T>(z3));
       }
       hg::wave_sync();  // the next pass rewrites A1, A2
@@ -1474,12 +1515,62 @@ __global__ __launch_bounds__(64 * WPB) void k_ude_n(hg::UdeArgs a) {
   }
 }
 
+// Slot of (this block, set s): the entry of s among the block's sets.
+__device__ __forceinline__ int ude_slot(const hg::UdeArgs& a, int s) {
+  if (!a.blk_slot) return int(blockIdx.x);
+  const int e0 = a.blk_off[blockIdx.x], e1 = a.blk_off[blockIdx.x + 1];
+  int slot = a.blk_slot[e0];
+  for (int e = e0; e < e1; ++e)
+    if (a.blk_set[e] == s) {
+      slot = a.blk_slot[e];
+      break;
+    }
+  return slot;
+}
+
+// Adds one run of weight cotangents (the matrix-core accumulators of
+// k_ude_vjp, row ir = M::row(kq, r), column col) into slot wp; each of the 353
+// weights comes from exactly one lane.  All of the lane's slot values are read
+// before any is written, so the reads take one memory round trip.
+template <class T>
+__device__ __forceinline__ void ude_flush(double* __restrict__ wp, const typename hg::Mfma<T>::V& w1,
+                                          const typename hg::Mfma<T>::V& w2, const typename hg::Mfma<T>::V& w3,
+                                          double b3, int l) {
+  using M = hg::Mfma<T>;
+  const int col = l & 15, kq = l >> 4;
+  int i2[4], i1[4], i3[4];
+  double x2[4], x1[4] = {0.0, 0.0, 0.0, 0.0}, x3[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < 4; ++r) {
+    const int ir = M::row(kq, r);
+    i2[r] = hg::kUdeW2 + ir * 16 + col;                            // W2bar
+    i1[r] = col < 3 ? hg::kUdeW1 + ir * 3 + col : hg::kUdeB1 + ir;  // W1bar, b1bar (col < 4)
+    i3[r] = col == 0 ? hg::kUdeB2 + ir : hg::kUdeW3 + ir;           // b2bar, W3bar (col < 2)
+  }
+  for (int r = 0; r < 4; ++r) {
+    x2[r] = wp[i2[r]];
+    if (col < 4) x1[r] = wp[i1[r]];
+    if (col < 2) x3[r] = wp[i3[r]];
+  }
+  const double xb3 = l == 0 ? wp[hg::kUdeB3] : 0.0;
+  for (int r = 0; r < 4; ++r) {
+    wp[i2[r]] = x2[r] + double(w2[r]);
+    if (col < 4) wp[i1[r]] = x1[r] + double(w1[r]);
+    if (col < 2) wp[i3[r]] = x3[r] + double(w3[r]);
+  }
+  if (l == 0) wp[hg::kUdeB3] = xb3 + b3;
+}
+
 // Reverse mode of k_ude_n: given the cotangent of n, adds the cotangents of
 // the features to the state cotangent (through the clamp and u = q / h, as
-// the forward forms them) and adds this block's weight cotangents into its
-// slot for each weight set its 64 cells use (one pass per distinct set, in
-// order of first appearance, as k_ude_n).  The backward activations overwrite
-// the forward ones once read: Z2 replaces A2, then Z1 replaces A1.
+// the forward forms them) and the weight cotangents into per-(block, set)
+// slots.  Block b (one wave) takes the 64-cell chunks b cpb .. (b + 1) cpb - 1
+// in order, one pass per distinct set of a chunk (order of first appearance,
+// as k_ude_n).  The weight cotangents are sums over cells on the matrix cores
+// (K = cells); consecutive passes of one set keep adding into the same
+// accumulators, which go into the (block, set) slot when the set changes and
+// at the end.  A chunk's inputs are read before its passes, its
+// state-cotangent values after them.  The backward activations overwrite the forward ones
+// once read: Z2 replaces A2, then Z1 replaces A1.
 template <class T>
 __global__ __launch_bounds__(64, sizeof(T) == 4 ? 3 : 1) void k_ude_vjp(hg::UdeArgs a) {
   using M = hg::Mfma<T>;
@@ -1489,116 +1580,130 @@ __global__ __launch_bounds__(64, sizeof(T) == 4 ? 3 : 1) void k_ude_vjp(hg::UdeA
   __shared__ T Z3[64];
   const int l = threadIdx.x;
   const int col = l & 15, kq = l >> 4;
-  const int i = blockIdx.x * 64 + l;
-  const bool valid = i < a.N;
-  double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
-  bool wet = false;
-  int set = 0;
-  if (valid) {
-    hg::ude_features(a, i, h, u, v, um, wet);
-    zb = a.zb[i % a.Nm];
-    if (a.set_of_cell) set = a.set_of_cell[i];
-  }
-  F[l * 4 + 0] = T(h);
-  F[l * 4 + 1] = T(um);
-  F[l * 4 + 2] = T(zb);
-  F[l * 4 + 3] = T(0);
-  hg::wave_sync();
-  uint64_t pending = __ballot(valid);
-  for (int pass = 0; pending; ++pass) {
-    const int lead = __ffsll((unsigned long long)pending) - 1;
-    const int s = __builtin_amdgcn_readfirstlane(__shfl(set, lead, 64));  // wave-uniform: scalar weight loads
-    const bool mine = valid && set == s;
-    pending &= ~uint64_t(__ballot(mine));
-    const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
-    ude_hidden<T>(W, F, A1, A2, l);
-    // output layer and softplus, lane = cell; cells of other sets carry a zero cotangent
-    const T z3 = ude_out<T>(W, A2, l);
-    const T z3b = mine ? T(a.nbar[i]) * hg::sigmoid<T>(z3) : T(0);
-    Z3[l] = z3b;
-    hg::wave_sync();
-    // Weight cotangents as sums over the 64 cells on the matrix cores (K = cells):
-    // w3 column 1: W3bar = A2^T Z3; column 0: b2bar = Z2^T 1; w2: W2bar = Z2^T A1;
-    // w1: [W1bar | b1bar] = Z1^T [F | 1]; b3bar = sum of Z3 by a fixed xor tree.
-    typename M::V w1 = {T(0), T(0), T(0), T(0)}, w2 = w1, w3 = w1;
-    for (int kc = 0; kc < 16; ++kc) {
-      const int c = 4 * kc + kq;  // cell of this lane's K-row
-      w3 = M::op(A2[c * 16 + col], col == 1 ? Z3[c] : T(0), w3);
+  const int nchunks = (a.N + 63) / 64;
+  const int ch0 = int(blockIdx.x) * a.cpb, ch1 = min(nchunks, ch0 + a.cpb);
+  typename M::V w1 = {T(0), T(0), T(0), T(0)}, w2 = w1, w3 = w1;
+  double b3 = 0.0;
+  int acc_set = -1;  // set of the running accumulators (-1: none yet)
+  for (int ch = ch0; ch < ch1; ++ch) {
+    const int i = ch * 64 + l;
+    const bool valid = i < a.N;
+    double h = 0.0, u = 0.0, v = 0.0, um = 0.0, zb = 0.0;
+    T nb = T(0);
+    bool wet = false;
+    int set = 0;
+    if (valid) {
+      hg::ude_features(a, i, h, u, v, um, wet);
+      zb = a.zb[i % a.Nm];
+      if (a.set_of_cell) set = a.set_of_cell[i];
+      nb = T(a.nbar[i]);
     }
+    F[l * 4 + 0] = T(h);
+    F[l * 4 + 1] = T(um);
+    F[l * 4 + 2] = T(zb);
+    F[l * 4 + 3] = T(1);  // the bias column of layer 1
     hg::wave_sync();
-    for (int j = 0; j < 16; ++j) {  // Z2 over A2, own row
-      const T a2 = A2[l * 16 + j];
-      A2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * (T(1) - a2 * a2);
-    }
-    hg::wave_sync();
-    const T* Z2 = A2;
-    for (int kc = 0; kc < 16; ++kc) {
-      const int c = 4 * kc + kq;
-      const T z2 = Z2[c * 16 + col];
-      w2 = M::op(z2, A1[c * 16 + col], w2);
-      w3 = M::op(z2, T(col == 0), w3);
-    }
-    // A1bar = Z2 W2 (rows = cells)
-    typename M::V ab[4];
-    for (int rb = 0; rb < 4; ++rb) {
-      typename M::V acc = {T(0), T(0), T(0), T(0)};
-      for (int kc = 0; kc < 4; ++kc) {
-        const T av = Z2[(16 * rb + col) * 16 + 4 * kc + kq];           // Z2[16 rb + i][4 kc + k]
-        const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];         // W2[4 kc + k][j]
-        acc = M::op(av, bv, acc);
+    T g0 = T(0), g1 = T(0), g2 = T(0);  // the cell's feature cotangents, from its set's pass
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+      const int lead = __ffsll((unsigned long long)pending) - 1;
+      const int s = __builtin_amdgcn_readfirstlane(__shfl(set, lead, 64));  // wave-uniform: scalar weight loads
+      const bool mine = valid && set == s;
+      pending &= ~uint64_t(__ballot(mine));
+      if (s != acc_set) {
+        if (acc_set >= 0) ude_flush<T>(a.wpart + size_t(ude_slot(a, acc_set)) * hg::kUdeW, w1, w2, w3, b3, l);
+        w1 = typename M::V{T(0), T(0), T(0), T(0)};
+        w2 = w1;
+        w3 = w1;
+        b3 = 0.0;
+        acc_set = s;
       }
-      ab[rb] = acc;
-    }
-    hg::wave_sync();
-    for (int rb = 0; rb < 4; ++rb)  // Z1 = A1bar (1 - A1^2) over A1
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * rb + M::row(kq, r);
-        const T a1 = A1[row * 16 + col];
-        A1[row * 16 + col] = ab[rb][r] * (T(1) - a1 * a1);
+      const T* W = static_cast<const T*>(a.w) + size_t(s) * hg::kUdeW;
+      ude_hidden<T>(W, F, A1, A2, l);
+      // output layer and softplus, lane = cell; cells of other sets carry a zero cotangent
+      const T z3 = ude_out<T>(W, A2, l);
+      const T z3b = mine ? nb * hg::sigmoid<T>(z3) : T(0);
+      Z3[l] = z3b;
+      hg::wave_sync();
+      // Weight cotangents as sums over the cells on the matrix cores (K = cells):
+      // w3 column 1: W3bar = A2^T Z3; column 0: b2bar = Z2^T 1; w2: W2bar = Z2^T A1;
+      // w1: [W1bar | b1bar] = Z1^T [F | 1]; b3bar = sum of Z3 by a fixed xor tree.
+      for (int kc = 0; kc < 16; ++kc) {
+        const int c = 4 * kc + kq;  // cell of this lane's K-row
+        w3 = M::op(A2[c * 16 + col], col == 1 ? Z3[c] : T(0), w3);
       }
-    hg::wave_sync();
-    const T* Z1 = A1;
-    // feature cotangents -> state cotangent (lane = cell)
-    T fb0 = T(0), fb1 = T(0), fb2 = T(0);
-    for (int j = 0; j < 16; ++j) {
-      const T z = Z1[l * 16 + j];
-      fb0 = fb0 + z * W[hg::kUdeW1 + j * 3 + 0];
-      fb1 = fb1 + z * W[hg::kUdeW1 + j * 3 + 1];
-      fb2 = fb2 + z * W[hg::kUdeW1 + j * 3 + 2];
+      hg::wave_sync();
+      for (int j = 0; j < 16; ++j) {  // Z2 over A2, own row
+        const T a2 = A2[l * 16 + j];
+        A2[l * 16 + j] = (z3b * W[hg::kUdeW3 + j]) * M::fma_(-a2, a2, T(1));
+      }
+      hg::wave_sync();
+      const T* Z2 = A2;
+      for (int kc = 0; kc < 16; ++kc) {
+        const int c = 4 * kc + kq;
+        const T z2 = Z2[c * 16 + col];
+        w2 = M::op(z2, A1[c * 16 + col], w2);
+        w3 = M::op(z2, T(col == 0), w3);
+      }
+      // A1bar = Z2 W2 (rows = cells)
+      typename M::V ab[4];
+      for (int rb = 0; rb < 4; ++rb) {
+        typename M::V acc = {T(0), T(0), T(0), T(0)};
+        for (int kc = 0; kc < 4; ++kc) {
+          const T av = Z2[(16 * rb + col) * 16 + 4 * kc + kq];    // Z2[16 rb + i][4 kc + k]
+          const T bv = W[hg::kUdeW2 + (4 * kc + kq) * 16 + col];  // W2[4 kc + k][j]
+          acc = M::op(av, bv, acc);
+        }
+        ab[rb] = acc;
+      }
+      hg::wave_sync();
+      for (int rb = 0; rb < 4; ++rb)  // Z1 = A1bar (1 - A1^2) over A1
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * rb + M::row(kq, r);
+          const T a1 = A1[row * 16 + col];
+          A1[row * 16 + col] = ab[rb][r] * M::fma_(-a1, a1, T(1));
+        }
+      hg::wave_sync();
+      const T* Z1 = A1;
+      // feature cotangents -> state cotangent (lane = cell)
+      T fb0 = T(0), fb1 = T(0), fb2 = T(0);
+      for (int j = 0; j < 16; ++j) {
+        const T z = Z1[l * 16 + j];
+        fb0 = M::fma_(z, W[hg::kUdeW1 + j * 3 + 0], fb0);
+        fb1 = M::fma_(z, W[hg::kUdeW1 + j * 3 + 1], fb1);
+        fb2 = M::fma_(z, W[hg::kUdeW1 + j * 3 + 2], fb2);
+      }
+      if (mine) {
+        g0 = fb0;
+        g1 = fb1;
+        g2 = fb2;
+      }
+      for (int kc = 0; kc < 16; ++kc) {
+        const int c = 4 * kc + kq;
+        w1 = M::op(Z1[c * 16 + col], col < 3 ? F[c * 4 + col] : T(col == 3), w1);
+      }
+      double z3s = double(Z3[l]);
+      for (int m = 32; m > 0; m >>= 1) z3s += __shfl_xor(z3s, m, 64);
+      b3 += z3s;
+      hg::wave_sync();  // the next pass rewrites A1, A2 and Z3
     }
-    if (mine) {
+    if (valid) {  // all four read before any is written: one round trip
+      const double o0 = a.ox[i], o1 = a.oqx[i], o2 = a.oqy[i], oz = a.zbbar ? a.zbbar[i] : 0.0;
       double xib = 0.0, qxb = 0.0, qyb = 0.0;
       if (wet) {  // h = xi + h_still, |u| = sqrt(u^2 + v^2 + eps), u = q_x / h, v = q_y / h
-        const double ub = double(fb1) * u / um, vb = double(fb1) * v / um;
-        xib = double(fb0) - (ub * u + vb * v) / h;
+        const double ub = double(g1) * u / um, vb = double(g1) * v / um;
+        xib = double(g0) - (ub * u + vb * v) / h;
         qxb = ub / h;
         qyb = vb / h;
       }
-      a.ox[i] += xib;
-      a.oqx[i] += qxb;
-      a.oqy[i] += qyb;
-      if (a.zbbar) a.zbbar[i] += double(fb2);
+      const double zbb = double(g2);
+      a.ox[i] = o0 + xib;
+      a.oqx[i] = o1 + qxb;
+      a.oqy[i] = o2 + qyb;
+      if (a.zbbar) a.zbbar[i] = oz + zbb;
     }
-    for (int kc = 0; kc < 16; ++kc) {
-      const int c = 4 * kc + kq;
-      w1 = M::op(Z1[c * 16 + col], col < 3 ? F[c * 4 + col] : T(col == 3), w1);
-    }
-    double b3 = double(Z3[l]);
-    for (int m = 32; m > 0; m >>= 1) b3 += __shfl_xor(b3, m, 64);
-    // this (block, set) slot: each of the 353 weights from exactly one lane
-    double* wp = a.wpart + size_t(a.blk_slot ? a.blk_slot[a.blk_off[blockIdx.x] + pass] : int(blockIdx.x)) *
-                              hg::kUdeW;
-    for (int r = 0; r < 4; ++r) {
-      const int ir = M::row(kq, r);
-      wp[hg::kUdeW2 + ir * 16 + col] += double(w2[r]);
-      if (col < 3) wp[hg::kUdeW1 + ir * 3 + col] += double(w1[r]);
-      if (col == 3) wp[hg::kUdeB1 + ir] += double(w1[r]);
-      if (col == 0) wp[hg::kUdeB2 + ir] += double(w3[r]);
-      if (col == 1) wp[hg::kUdeW3 + ir] += double(w3[r]);
-    }
-    if (l == 0) wp[hg::kUdeB3] += b3;
-    hg::wave_sync();  // the next pass rewrites A1, A2 and Z3
   }
+  if (acc_set >= 0) ude_flush<T>(a.wpart + size_t(ude_slot(a, acc_set)) * hg::kUdeW, w1, w2, w3, b3, l);
 }
 
 // wbar[s][k] += the sum of the slots of weight set s (set_start[s] ..
@@ -2099,7 +2204,7 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.S0x = ctx->S0x;
   a.S0y = ctx->S0y;
   a.area = ctx->area;
-  a.deg = ctx->deg;
+  a.deg = ctx->deg_full ? nullptr : ctx->deg;  // NULL: every cell has tileMD faces
   a.item_off = ctx->item_off;
   a.items = ctx->items;
   a.tile_in_off = ctx->tile_in_off;
@@ -2273,6 +2378,14 @@ hg::UdeArgs ude_args(hg_ctx* ctx, const double* const q[3]) {
 }
 
 int ude_blocks(const hg_ctx* ctx) { return int((ctx->N + 63) / 64); }
+
+// Chunks per reverse-mode block of the closure: one while the mesh has at
+// most 3,072 chunks (the waves the kernel holds at once on 256 CUs, 3 per
+// SIMD), else the fewest that keep the blocks within 3,072.  A function of the
+// chunk count alone, so a mesh's weight cotangents group its cells the same
+// way on every run.
+int ude_vjp_cpb(int nchunks) { return std::max(1, (nchunks + 3071) / 3072); }
+int ude_vjp_blocks(const hg_ctx* ctx) { return (ude_blocks(ctx) + ctx->ude_cpb - 1) / ctx->ude_cpb; }
 
 // n <- 0.01 + softplus(MLP(features of state q)) when the closure is on.
 int enqueue_ude_n(hg_ctx* ctx, const double* const q[3]) {
@@ -2623,6 +2736,7 @@ void launch_adj_tile_t(hg_ctx* ctx, const hg::AdjArgs& a) {
 }
 
 void launch_adj_tile(hg_ctx* ctx, hg::AdjArgs a) {
+  if (ctx->deg_full) a.deg = nullptr;  // every cell has tileMD faces
   a.item_off = ctx->adj_item_off;
   a.items = ctx->adj_items;
   a.xcd_chunk = ctx->xcd_chunk;
@@ -2693,12 +2807,14 @@ int enqueue_adjoint(hg_ctx* ctx, const double* const q[3], const double* const l
     u.zbbar = ctx->adj_zbbar;
     u.wpart = ctx->ude_wpart;
     u.blk_off = ctx->ude_blk_off;
+    u.blk_set = ctx->ude_blk_set;
     u.blk_slot = ctx->ude_blk_slot;
     u.nslots = ctx->ude_nslots;
+    u.cpb = ctx->ude_cpb;
     if (ctx->ude_prec == 32)
-      hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+      hipLaunchKernelGGL(k_ude_vjp<float>, dim3(ude_vjp_blocks(ctx)), dim3(64), 0, ctx->stream, u);
     else
-      hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_blocks(ctx)), dim3(64), 0, ctx->stream, u);
+      hipLaunchKernelGGL(k_ude_vjp<double>, dim3(ude_vjp_blocks(ctx)), dim3(64), 0, ctx->stream, u);
   }
   if (law) {  // through the law: n(Q) adds to the state cotangent
     hg::NLawArgs u = nlaw_args(ctx, q);
@@ -3286,11 +3402,12 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   }
   // ---- validate topology
   if (m->face_off[0] != 0) { g_create_err = "face_off[0] != 0"; return HG_ERR_MESH; }
-  int32_t maxdeg = 0;
+  int32_t maxdeg = 0, mindeg = 16;
   for (int64_t i = 0; i < N; ++i) {
     const int32_t d = m->face_off[i + 1] - m->face_off[i];
     if (d < 3 || d > 16) { g_create_err = "cell " + std::to_string(i) + " has " + std::to_string(d) + " faces"; return HG_ERR_MESH; }
     maxdeg = std::max(maxdeg, d);
+    mindeg = std::min(mindeg, d);
     if (!(m->area[i] > 0.0)) { g_create_err = "cell " + std::to_string(i) + " has non-positive area"; return HG_ERR_MESH; }
   }
   const int64_t M = m->face_off[N];
@@ -3557,6 +3674,7 @@ int create_impl(hg_ctx** out, const hg_mesh* m, const hg_bcs* b, const hg_consts
   ctx->tiled = !(flags & HG_CELL_KERNEL) && maxdeg <= 8;
   if (ctx->tiled) {
     ctx->tileMD = maxdeg <= 3 ? 3 : (maxdeg <= 4 ? 4 : 8);
+    ctx->deg_full = mindeg == ctx->tileMD;  // all triangles or all quadrilaterals
     // 256-cell tiles, unless that leaves fewer than two work-groups per CU
     // (a small mesh fills the chip better with 128-cell tiles)
     int ncu = 0;
@@ -3976,7 +4094,8 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
   ctx->ude_w = nullptr;
   ctx->ude_set = nullptr;
   ctx->ude_nbar = ctx->ude_wpart = ctx->ude_wbar = nullptr;
-  ctx->ude_blk_off = ctx->ude_blk_slot = ctx->ude_set_start = nullptr;
+  ctx->ude_blk_off = ctx->ude_blk_set = ctx->ude_blk_slot = ctx->ude_set_start = nullptr;
+  ctx->ude_cpb = 1;
   ctx->ude_nslots = 0;
   if (n_sets == 0) {  // closure off: back to the per-cell n of hg_set_fields
     HG_TRY(hipMemcpyAsync(ctx->n, ctx->n0, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -4038,13 +4157,16 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
     if (!rc) rc = upload(ctx, ctx->ude_set, set.data(), N);
   }
   if (!rc) rc = dalloc(ctx, &ctx->ude_nbar, N);
-  // Reverse-mode weight slots: one per (64-cell block, set used in it), ordered
-  // by set and then block, so that each set's cotangent sums a contiguous run.
-  const int nb = ude_blocks(ctx);
+  // Reverse-mode weight slots: one per (reverse-mode block, set used in it),
+  // ordered by set and then block, so that each set's cotangent sums a
+  // contiguous run.  A block takes cpb consecutive 64-cell chunks.
+  ctx->ude_cpb = ude_vjp_cpb(ude_blocks(ctx));
+  const int nb = ude_vjp_blocks(ctx);
+  const int64_t cells_pb = int64_t(ctx->ude_cpb) * 64;
   std::vector<int32_t> blk_off(size_t(nb) + 1, 0), blk_set, set_start(size_t(n_sets) + 1, 0);
   for (int b = 0; b < nb; ++b) {
-    const int64_t i1 = std::min<int64_t>(N, int64_t(b + 1) * 64);
-    for (int64_t i = int64_t(b) * 64; i < i1; ++i) {
+    const int64_t i1 = std::min<int64_t>(N, int64_t(b + 1) * cells_pb);
+    for (int64_t i = int64_t(b) * cells_pb; i < i1; ++i) {
       const int32_t v = set_of_cell ? set[i] : 0;
       if (std::find(blk_set.begin() + blk_off[b], blk_set.end(), v) == blk_set.end()) blk_set.push_back(v);
     }
@@ -4059,6 +4181,8 @@ int hg_set_ude(hg_ctx* ctx, int32_t n_sets, const double* weights, const int32_t
   if (!rc) rc = dalloc(ctx, &ctx->ude_wbar, nw);
   if (!rc) rc = dalloc(ctx, &ctx->ude_blk_off, blk_off.size());
   if (!rc) rc = upload(ctx, ctx->ude_blk_off, blk_off.data(), blk_off.size());
+  if (!rc) rc = dalloc(ctx, &ctx->ude_blk_set, blk_set.size());
+  if (!rc) rc = upload(ctx, ctx->ude_blk_set, blk_set.data(), blk_set.size());
   if (!rc) rc = dalloc(ctx, &ctx->ude_blk_slot, blk_slot.size());
   if (!rc) rc = upload(ctx, ctx->ude_blk_slot, blk_slot.data(), blk_slot.size());
   if (!rc) rc = dalloc(ctx, &ctx->ude_set_start, set_start.size());
