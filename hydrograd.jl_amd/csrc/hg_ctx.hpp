@@ -75,16 +75,33 @@ struct InletFaceRec {
   double L23, L53;
 };
 
-// One work item of the tiled kernel (32 bytes, one address per item).
+// One work item of the tiled kernels, as they use it.
 struct ItemRec {
   int32_t code;
   int32_t ref;
   double nx, ny, len;
 };
 
+// Face geometry of one work item.
+struct ItemGeo {
+  double nx, ny, len;
+};
+
+// The work items of one tiling in device memory.  Tile t's items are
+// off[t] .. off[t+1]-1, its cross-tile and boundary items first; those, up to
+// x0[t], also carry a ref (the global neighbour or ghost id), so the
+// intra-tile items, most of them, are read as 28 bytes (code + geometry).
+struct ItemTable {
+  int32_t* off;
+  int32_t* x0;
+  int32_t* code;
+  int32_t* ref;
+  ItemGeo* geo;
+};
+
 // Arguments of the tiled (face-once) step kernel.  Cells are split into
 // tiles of `T` consecutive (Morton-ordered) cells, one work-group per tile.
-// Work items of tile t are item_off[t] .. item_off[t+1]-1:
+// Work items of tile t are items.off[t] .. items.off[t+1]-1:
 //   code bits  0..9  a  (local cell of the face's first side)
 //              10..13 ka (local face index of that side)
 //              14..23 b  (local cell of the second side, intra-tile items)
@@ -112,8 +129,7 @@ struct TileArgs {
   const double* S0y;
   const double* area;
   const int32_t* deg;  // faces per cell, or NULL: every cell has MD faces
-  const int32_t* item_off;
-  const struct ItemRec* items;
+  struct ItemTable items;
   const int32_t* tile_in_off;   // [ntiles+1] into tile_in_list
   const int32_t* tile_in_list;  // inlets with ghosts in the tile; a ghost item's local slot indexes this list
   const int32_t* g_kind;
@@ -182,8 +198,7 @@ struct AdjArgs {
   int32_t nbar_set;   // 1: nbar holds this evaluation's n-bar alone (set, not added to)
   int32_t fbar_set;   // 1: fbar likewise
   // tiled reverse step (k_adj_tile): the step kernel's work items and block -> tile mapping
-  const int32_t* item_off;
-  const struct ItemRec* items;
+  struct ItemTable items;
   int32_t xcd_chunk;
 };
 
@@ -326,8 +341,7 @@ struct hg_ctx {
   bool tiled = false;
   int32_t tileT = 256, ntiles = 0;
   bool fuse_inlet = false;
-  int32_t* item_off = nullptr;
-  hg::ItemRec* items = nullptr;
+  hg::ItemTable items{};  // the step kernel's work items
   int32_t tileMD = 3;
   bool deg_full = false;  // every cell has tileMD faces: the tile kernels need not read deg
   int32_t xcd_chunk = 0;  // identity block -> tile mapping by default (the autotuner re-checks)
@@ -336,11 +350,9 @@ struct hg_ctx {
   // when those have it, else its own copy built on first use)
   int32_t adj_T_req = 0;  // hg_set_adjoint_tile (0: the library's choice)
   int32_t adj_T = 0, adj_ntiles = 0;
-  const int32_t* adj_item_off = nullptr;
-  const hg::ItemRec* adj_items = nullptr;
+  hg::ItemTable adj_items{};  // the reverse step's: items or adj_own
   int32_t adj_own_T = 0, adj_own_ntiles = 0;
-  int32_t* adj_own_off = nullptr;
-  hg::ItemRec* adj_own_items = nullptr;
+  hg::ItemTable adj_own{};
   int32_t *tile_in_off = nullptr, *tile_in_list = nullptr;
   std::vector<void*> tile_allocs;  // device buffers of the current tiling (freed on re-tiling)
   size_t lds_max = 65536;          // dynamic LDS a work-group may use on this device

@@ -291,6 +291,18 @@ __global__ __launch_bounds__(kBlock) void k_swe_step(StepArgs a0) {
 //     the source terms and applies the Euler update and dry clamp.
 // T (cells per tile) and MD (slots per cell) are compile-time constants so
 // every LDS address is a base register plus an immediate offset.
+// Work item it of a tile whose items before ix0 carry a ref (see ItemTable).
+__device__ __forceinline__ hg::ItemRec load_item(const hg::ItemTable& tb, int it, int ix0) {
+  hg::ItemRec r;
+  r.code = tb.code[it];
+  r.ref = it < ix0 ? tb.ref[it] : 0;
+  const hg::ItemGeo g = tb.geo[it];
+  r.nx = g.nx;
+  r.ny = g.ny;
+  r.len = g.len;
+  return r;
+}
+
 // This work-group's tile t and field fld (the launch covers every tile of
 // each of K fields; see k_swe_tile).
 __device__ __forceinline__ void tile_of_block(int K, int xcd_chunk, int& t, int& fld) {
@@ -386,9 +398,9 @@ __global__ __launch_bounds__(T) void k_swe_tile(hg::TileArgs a0) {
   }
   __syncthreads();
 
-  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
+  const int it0 = a.items.off[t], it1 = a.items.off[t + 1], ix0 = a.items.x0[t];
   for (int it = it0 + li; it < it1; it += T) {
-    const hg::ItemRec rec = a.items[it];
+    const hg::ItemRec rec = load_item(a.items, it, ix0);
     const int code = rec.code;
     const int ia = code & 1023;
     const int ka = (code >> 10) & 15;
@@ -1096,7 +1108,7 @@ __device__ __forceinline__ void adj_item(const hg::AdjArgs& a, int t, const doub
 }
 
 template <bool STEP, int T, int MD>
-__global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
+__global__ __launch_bounds__(T, 3) void k_adj_tile(hg::AdjArgs a0) {
   extern __shared__ double lds[];
   int t, fld;
   tile_of_block(a0.K, a0.xcd_chunk, t, fld);
@@ -1110,15 +1122,15 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
   const double hsmall = a.hsmall;
   // this work-item's first two work items, and the neighbour of a cross-tile
   // first item, are read with the cell data: one round of memory latency
-  const int it0 = a.item_off[t], it1 = a.item_off[t + 1];
+  const int it0 = a.items.off[t], it1 = a.items.off[t + 1], ix0 = a.items.x0[t];
   const int itA = it0 + li, itB = itA + T;
   hg::ItemRec recA{}, recB{};
   NbrRaw nbrA{};
   if (itA < it1) {
-    recA = a.items[itA];
+    recA = load_item(a.items, itA, ix0);
     if (((recA.code >> 28) & 3) == 1) nbrA = nbr_load<STEP>(a, recA.ref);
   }
-  if (itB < it1) recB = a.items[itB];
+  if (itB < it1) recB = load_item(a.items, itB, ix0);
   int d_i = 0;
   if (valid) {
     double rh, r2s;
@@ -1146,7 +1158,7 @@ __global__ __launch_bounds__(T) void k_adj_tile(hg::AdjArgs a0) {
     if (it == itB)
       rec = recB;
     else
-      rec = a.items[it];
+      rec = load_item(a.items, it, ix0);
     NbrRaw nr{};
     if (((rec.code >> 28) & 3) == 1) nr = nbr_load<STEP>(a, rec.ref);
     adj_item<STEP, T, MD>(a, t, C, S, Xm, rec, nr);
@@ -2105,7 +2117,7 @@ __global__ __launch_bounds__(kBlock) void k_swe_step_jvp(JvpArgs J) {
 // Inlet ghosts carry the slot of their inlet in the tile's inlet list in
 // code bits 14..19; max_inlets is the longest such list.
 struct TileBuild {
-  std::vector<int32_t> off, code, ref;
+  std::vector<int32_t> off, x0, code, ref;
   std::vector<double> nx, ny, len;
   std::vector<int32_t> in_off, in_list;
   int32_t max_inlets = 0;
@@ -2117,6 +2129,7 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
   TileBuild tb;
   const int64_t ntiles = (N + T - 1) / T;
   tb.off.assign(ntiles + 1, 0);
+  tb.x0.assign(ntiles, 0);
   tb.in_off.assign(ntiles + 1, 0);
   std::vector<int64_t> grp[3];
   std::vector<int32_t> local;  // this tile's inlets in order of first appearance
@@ -2157,7 +2170,8 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
         (void)s;
       }
     }
-    for (int type : {1, 2, 0})  // cross-tile first: their neighbour gathers are prefetched
+    for (int type : {1, 2, 0}) {  // cross-tile first: their neighbour gathers are prefetched
+      if (type == 0) tb.x0[t] = int32_t(tb.code.size());
       for (int64_t e : grp[type]) {
         const int32_t code = int32_t(e & 0xffffffff);
         const int32_t ref = int32_t(uint32_t(uint64_t(e) >> 32));
@@ -2169,12 +2183,42 @@ TileBuild build_tiles(int64_t N, int T, const std::vector<int32_t>& deg, const s
         tb.ny.push_back(fny[s]);
         tb.len.push_back(flen[s]);
       }
+    }
     tb.off[t + 1] = int32_t(tb.code.size());
     tb.in_list.insert(tb.in_list.end(), local.begin(), local.end());
     tb.in_off[t + 1] = int32_t(tb.in_list.size());
     tb.max_inlets = std::max(tb.max_inlets, int32_t(local.size()));
   }
   return tb;
+}
+
+// Device copy of a tiling's work items (dalloc'd: freed with the context,
+// or by free_items).
+int upload_items(hg_ctx* ctx, const TileBuild& tb, hg::ItemTable& d) {
+  const size_t ni = tb.code.size();
+  std::vector<hg::ItemGeo> geo(ni);
+  for (size_t k = 0; k < ni; ++k) geo[k] = hg::ItemGeo{tb.nx[k], tb.ny[k], tb.len[k]};
+  d = hg::ItemTable{};
+  int rc = dalloc(ctx, &d.off, tb.off.size());
+  if (!rc) rc = dalloc(ctx, &d.x0, tb.x0.size());
+  if (!rc) rc = dalloc(ctx, &d.code, ni);
+  if (!rc) rc = dalloc(ctx, &d.ref, ni);
+  if (!rc) rc = dalloc(ctx, &d.geo, ni);
+  if (!rc) rc = upload(ctx, d.off, tb.off.data(), tb.off.size());
+  if (!rc) rc = upload(ctx, d.x0, tb.x0.data(), tb.x0.size());
+  if (!rc) rc = upload(ctx, d.code, tb.code.data(), ni);
+  if (!rc) rc = upload(ctx, d.ref, tb.ref.data(), ni);
+  if (!rc) rc = upload(ctx, d.geo, geo.data(), ni);
+  return rc;
+}
+
+int free_items(hg_ctx* ctx, hg::ItemTable& d) {
+  int rc = dfree(ctx, d.off);
+  if (!rc) rc = dfree(ctx, d.x0);
+  if (!rc) rc = dfree(ctx, d.code);
+  if (!rc) rc = dfree(ctx, d.ref);
+  if (!rc) rc = dfree(ctx, d.geo);
+  return rc;
 }
 
 size_t tile_lds_bytes(int T, int maxdeg) {
@@ -2205,7 +2249,6 @@ hg::TileArgs make_tile_args_p(hg_ctx* ctx, const double* const in[3], double* co
   a.S0y = ctx->S0y;
   a.area = ctx->area;
   a.deg = ctx->deg_full ? nullptr : ctx->deg;  // NULL: every cell has tileMD faces
-  a.item_off = ctx->item_off;
   a.items = ctx->items;
   a.tile_in_off = ctx->tile_in_off;
   a.tile_in_list = ctx->tile_in_list;
@@ -2689,7 +2732,6 @@ int ensure_adj_tiles(hg_ctx* ctx) {
   if (T == ctx->tileT) {
     ctx->adj_T = T;
     ctx->adj_ntiles = ctx->ntiles;
-    ctx->adj_item_off = ctx->item_off;
     ctx->adj_items = ctx->items;
     return HG_OK;
   }
@@ -2697,25 +2739,17 @@ int ensure_adj_tiles(hg_ctx* ctx) {
     const int64_t N = ctx->Nm;
     TileBuild tb = build_tiles(N, T, ctx->h_deg, ctx->h_nbr, ctx->h_nx, ctx->h_ny, ctx->h_len, ctx->h_gkind,
                                ctx->h_gbc);
-    const size_t ni = tb.code.size();
-    std::vector<hg::ItemRec> recs(ni);
-    for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
     HG_TRY(hipStreamSynchronize(ctx->stream));
     int rc = drop_graph(ctx);
-    if (!rc) rc = dfree(ctx, ctx->adj_own_off);
-    if (!rc) rc = dfree(ctx, ctx->adj_own_items);
-    if (!rc) rc = dalloc(ctx, &ctx->adj_own_off, tb.off.size());
-    if (!rc) rc = dalloc(ctx, &ctx->adj_own_items, ni);
-    if (!rc) rc = upload(ctx, ctx->adj_own_off, tb.off.data(), tb.off.size());
-    if (!rc) rc = upload(ctx, ctx->adj_own_items, recs.data(), ni);
+    if (!rc) rc = free_items(ctx, ctx->adj_own);
+    if (!rc) rc = upload_items(ctx, tb, ctx->adj_own);
     if (rc) return rc;
     ctx->adj_own_T = T;
     ctx->adj_own_ntiles = int32_t((N + T - 1) / T);
   }
   ctx->adj_T = T;
   ctx->adj_ntiles = ctx->adj_own_ntiles;
-  ctx->adj_item_off = ctx->adj_own_off;
-  ctx->adj_items = ctx->adj_own_items;
+  ctx->adj_items = ctx->adj_own;
   return HG_OK;
 }
 
@@ -2737,7 +2771,6 @@ void launch_adj_tile_t(hg_ctx* ctx, const hg::AdjArgs& a) {
 
 void launch_adj_tile(hg_ctx* ctx, hg::AdjArgs a) {
   if (ctx->deg_full) a.deg = nullptr;  // every cell has tileMD faces
-  a.item_off = ctx->adj_item_off;
   a.items = ctx->adj_items;
   a.xcd_chunk = ctx->xcd_chunk;
   if (ctx->adj_T == 128)
@@ -3215,8 +3248,8 @@ int setup_tiles(hg_ctx* ctx, int T) {
   const std::vector<int32_t>& h_deg = ctx->h_deg;
   TileBuild tb = build_tiles(N, T, h_deg, ctx->h_nbr, ctx->h_nx, ctx->h_ny, ctx->h_len, ctx->h_gkind, ctx->h_gbc);
   const int64_t ntiles = (N + T - 1) / T;
-  const size_t ni = tb.code.size();
-  {  // every (cell, local face) slot must receive exactly one flux
+  {  // every (cell, local face) slot must receive exactly one flux, and only
+     // the items before x0 (cross-tile and boundary) may need their ref
     std::vector<uint8_t> hits(size_t(maxdeg) * N, 0);
     bool ok = true;
     for (int64_t t = 0; t < ntiles && ok; ++t)
@@ -3226,6 +3259,7 @@ int setup_tiles(hg_ctx* ctx, int T) {
         const int32_t type = (code >> 28) & 3;
         const int64_t ca = t * T + ia;
         if (ia >= T || ca >= N || ka >= h_deg[ca]) { ok = false; break; }
+        if ((type == 0) != (it >= tb.x0[t])) { ok = false; break; }
         hits[size_t(ka) * N + ca]++;
         if (type == 0) {
           const int64_t cb = t * T + ib;
@@ -3241,8 +3275,6 @@ int setup_tiles(hg_ctx* ctx, int T) {
       return HG_ERR_MESH;
     }
   }
-  std::vector<hg::ItemRec> recs(ni);
-  for (size_t k = 0; k < ni; ++k) recs[k] = hg::ItemRec{tb.code[k], tb.ref[k], tb.nx[k], tb.ny[k], tb.len[k]};
   // release the previous tiling (and the graph that captured it)
   HG_TRY(hipStreamSynchronize(ctx->stream));
   int rc = drop_graph(ctx);
@@ -3253,13 +3285,10 @@ int setup_tiles(hg_ctx* ctx, int T) {
   }
   ctx->tile_allocs.clear();
   const size_t nalloc0 = ctx->allocs.size();
-  rc = dalloc(ctx, &ctx->item_off, tb.off.size());
-  if (!rc) rc = dalloc(ctx, &ctx->items, ni);
+  rc = upload_items(ctx, tb, ctx->items);
   if (!rc) rc = dalloc(ctx, &ctx->tile_in_off, tb.in_off.size());
   if (!rc) rc = dalloc(ctx, &ctx->tile_in_list, tb.in_list.size());
   ctx->tile_allocs.assign(ctx->allocs.begin() + nalloc0, ctx->allocs.end());
-  if (!rc) rc = upload(ctx, ctx->item_off, tb.off.data(), tb.off.size());
-  if (!rc) rc = upload(ctx, ctx->items, recs.data(), ni);
   if (!rc) rc = upload(ctx, ctx->tile_in_off, tb.in_off.data(), tb.in_off.size());
   if (!rc) rc = upload(ctx, ctx->tile_in_list, tb.in_list.data(), tb.in_list.size());
   if (rc) return rc;
