@@ -12,7 +12,8 @@
 * Batched instances with one weight set each equal separate runs, forward
   and in the weight gradient of each set; weight sets interleaved within
   blocks: equal sets sum to the one-set gradient, distinct sets against
-  central differences.
+  central differences; the same two checks on meshes of more than 3,072
+  chunks, where a reverse-mode block takes a run of chunks.
 """
 import sys
 
@@ -344,6 +345,68 @@ def test_unused_weight_set_has_zero_cotangent():
     k = capi.Context(c)
     k.set_ude(W2, set_of_cell=soc, precision=64)
     assert np.array_equal(k.step(c.Q0, c.dt, nsteps), ctx.step(c.Q0, c.dt, nsteps))
+
+
+def test_chunk_run_weight_gradients_equal_separate_runs():
+    """Above 3,072 64-cell chunks a reverse-mode block of the closure takes a
+    run of consecutive chunks and keeps adding each set's weight cotangents
+    across them.  Seven instances of 29,336 cells (3,208 chunks: two per
+    block, some blocks across two instances) give each set the gradient of its own
+    instance's loss, which a run alone (one chunk per block) groups
+    differently: equal to round-off in f64, and the f32 network tracks it."""
+    inst = [small_case(seed=s, n=30_000) for s in range(1, 8)]
+    B = cases.concat(inst)
+    assert (B.N + 63) // 64 > 3072
+    Ws = ude.perturbed_sets(ude.xavier_weights(21), len(inst), scale=0.1)
+    set_of_cell = np.repeat(np.arange(len(inst)), [c.N for c in inst]).astype(np.int32)
+    nsteps, kck = 12, 4
+    probes, targets = [], []
+    for r, c in enumerate(inst):
+        p, t = _probe(c, 0, seed=40 + r, k=15)
+        probes.append(p)
+        targets.append(t)
+    first = np.cumsum([0] + [c.N for c in inst[:-1]])
+    bprobe = np.concatenate([p + o for p, o in zip(probes, first)]).astype(np.int32)
+    g = {}
+    for prec in (64, 32):
+        ctx = capi.Context(B)
+        ctx.set_ude(Ws, set_of_cell=set_of_cell, precision=prec)
+        ctx.grad_probes(B.Q0, B.dt, nsteps, kck, bprobe, np.concatenate(targets))
+        g[prec] = ctx.ude_grad()
+    for r, c in enumerate(inst):
+        k = capi.Context(c)
+        k.set_ude(Ws[r], precision=64)
+        k.grad_probes(c.Q0, B.dt, nsteps, kck, probes[r], targets[r])
+        g1 = k.ude_grad()
+        rel = float(np.max(np.abs(g[64][r] - g1)) / np.max(np.abs(g1)))
+        rel32 = float(np.linalg.norm(g[32][r] - g1) / np.linalg.norm(g1))
+        say(f"  set {r}: runs of chunks vs alone max rel {rel:.2e}; f32 vs f64 {rel32:.2e}")
+        assert np.linalg.norm(g1) > 0.0
+        assert rel <= 1e-12, r
+        assert rel32 <= 1e-3, r
+
+
+def test_chunk_run_interleaved_sets_sum_to_single_set_gradient():
+    """Runs of chunks with four equal sets interleaved cell by cell: every
+    pass switches set, so each block adds into each of its four slots many
+    times; the four gradients sum to the one-set gradient of the same mesh."""
+    c = small_case(seed=6, n=210_000)
+    assert (c.N + 63) // 64 > 3072
+    nsteps, kck = 8, 4
+    probe, target = _probe(c, 0, seed=11, k=30)
+    W = ude.xavier_weights(22)
+    one = capi.Context(c)
+    one.set_ude(W, precision=64)
+    one.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g1 = one.ude_grad()
+    many = capi.Context(c)
+    many.set_ude(np.stack([W] * 4), set_of_cell=_interleaved(c, 4), precision=64)
+    many.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    g4 = many.ude_grad()
+    assert all(np.linalg.norm(g) > 0.0 for g in g4)
+    rel = float(np.max(np.abs(g4.sum(axis=0) - g1)) / np.max(np.abs(g1)))
+    say(f"  runs of chunks, 4 interleaved equal sets: sum vs one set max rel {rel:.2e}")
+    assert rel <= 1e-12
 
 
 def test_closure_rejects_n_as_active_parameter():
