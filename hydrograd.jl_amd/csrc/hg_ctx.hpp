@@ -447,6 +447,7 @@ struct hg_ctx {
 
   // events attached to timed step dispatches (hg_advance_timed), reused
   std::vector<hipEvent_t> ev_pool;
+  hipEvent_t ev_xfer[2] = {nullptr, nullptr};  // chunk events of the pinned host transfers
 
   // cached graph of kGraphSteps steps (buffer 0 -> ... -> buffer 0) per dt
   hipGraphExec_t gexec = nullptr;

@@ -24,7 +24,11 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <unistd.h>
+
+#include <condition_variable>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -3096,6 +3100,158 @@ int check_flags(hg_ctx* ctx) {
 // Host 3N state (caller order) -> device state `slot` (library order).
 // The staging buffer and device permutation of the transfers below
 // (allocated on first use).
+// ---- host <-> device copies of caller arrays.  A caller's array is
+// ordinary (pageable) host memory, which the DMA engine reads or writes only
+// through the runtime's own staging, slowly for device -> host.  Instead the
+// array moves in 4 MiB chunks through two pinned buffers: host threads copy
+// one chunk between the caller's array and a pinned buffer while the DMA
+// engine moves the other.  The pinned buffers and the threads are the
+// process's, shared by every context (one transfer at a time).
+
+constexpr size_t kXferChunk = size_t(1) << 19;  // doubles per chunk (4 MiB)
+
+// memcpy split over a fixed set of host threads (the caller takes part 0).
+struct HostCopyPool {
+  std::vector<std::thread> th;
+  std::mutex m;
+  std::condition_variable go, done;
+  uint64_t gen = 0;
+  int left = 0, parts = 1;
+  char* dst = nullptr;
+  const char* src = nullptr;
+  size_t bytes = 0;
+  explicit HostCopyPool(int workers) : parts(workers + 1) {
+    for (int k = 1; k <= workers; ++k) th.emplace_back([this, k] { worker(k); });
+  }
+  void part(int p) const {
+    const size_t per = ((bytes + size_t(parts) - 1) / size_t(parts) + 63) & ~size_t(63);
+    const size_t a = std::min(bytes, per * size_t(p)), b = std::min(bytes, a + per);
+    if (b > a) std::memcpy(dst + a, src + a, b - a);
+  }
+  void worker(int p) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+      go.wait(lk, [&] { return gen != seen; });
+      seen = gen;
+      lk.unlock();
+      part(p);
+      lk.lock();
+      if (--left == 0) done.notify_one();
+    }
+  }
+  void copy(void* d, const void* s, size_t n) {
+    {
+      std::lock_guard<std::mutex> g(m);
+      dst = static_cast<char*>(d);
+      src = static_cast<const char*>(s);
+      bytes = n;
+      left = parts - 1;
+      ++gen;
+    }
+    go.notify_all();
+    part(0);
+    std::unique_lock<std::mutex> lk(m);
+    done.wait(lk, [&] { return left == 0; });
+  }
+};
+
+struct HostXfer {
+  std::mutex m;            // one transfer at a time
+  double* pin[2] = {nullptr, nullptr};
+  HostCopyPool* pool = nullptr;  // threads live until the process ends
+  pid_t pid = 0;                 // the process that made them (a forked child makes its own)
+  bool ok = false;
+};
+
+HostXfer& host_xfer() {
+  static HostXfer x;
+  return x;
+}
+
+// The pinned buffers and threads, made on first use; false if pinned memory
+// is not available (the plain copies are used then).
+bool host_xfer_ready(HostXfer& x) {
+  if (x.ok && x.pid == getpid()) return true;
+  if (x.ok) {  // a fork: the parent's threads did not come along (its pool is left as it is)
+    x.pool = nullptr;
+    x.ok = false;
+  }
+  if (!x.pin[0] && hipHostMalloc(reinterpret_cast<void**>(&x.pin[0]), kXferChunk * sizeof(double)) != hipSuccess)
+    x.pin[0] = nullptr;
+  if (!x.pin[1] && hipHostMalloc(reinterpret_cast<void**>(&x.pin[1]), kXferChunk * sizeof(double)) != hipSuccess)
+    x.pin[1] = nullptr;
+  if (!x.pin[0] || !x.pin[1]) return false;
+  if (!x.pool) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    x.pool = new HostCopyPool(int(std::min(7u, hw > 1 ? hw / 2 : 0u)));
+  }
+  x.pid = getpid();
+  x.ok = true;
+  return true;
+}
+
+int ensure_xfer_events(hg_ctx* ctx) {
+  for (hipEvent_t& e : ctx->ev_xfer)
+    if (!e) HG_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return HG_OK;
+}
+
+// n doubles: caller's host array src -> device dst (ctx->stream; returns
+// with the copies enqueued after the last host chunk was staged).
+int host_to_device(hg_ctx* ctx, double* dst, const double* src, size_t n) {
+  HostXfer& x = host_xfer();
+  std::lock_guard<std::mutex> g(x.m);
+  if (!host_xfer_ready(x)) {
+    HG_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    return HG_OK;
+  }
+  if (int rc = ensure_xfer_events(ctx)) return rc;
+  for (size_t off = 0, k = 0; off < n; off += kXferChunk, ++k) {
+    const int b = int(k & 1);
+    const size_t len = std::min(kXferChunk, n - off);
+    if (k >= 2) HG_TRY(hipEventSynchronize(ctx->ev_xfer[b]));  // the DMA that last read pin[b]
+    x.pool->copy(x.pin[b], src + off, len * sizeof(double));
+    HG_TRY(hipMemcpyAsync(dst + off, x.pin[b], len * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HG_TRY(hipEventRecord(ctx->ev_xfer[b], ctx->stream));
+  }
+  HG_TRY(hipStreamSynchronize(ctx->stream));  // the pinned buffers are free for the next transfer
+  return HG_OK;
+}
+
+// n doubles: device src -> caller's host array dst, after the work already
+// on ctx->stream; synchronous.
+int device_to_host(hg_ctx* ctx, double* dst, const double* src, size_t n) {
+  HostXfer& x = host_xfer();
+  std::lock_guard<std::mutex> g(x.m);
+  if (!host_xfer_ready(x)) {
+    HG_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HG_TRY(hipStreamSynchronize(ctx->stream));
+    return HG_OK;
+  }
+  if (int rc = ensure_xfer_events(ctx)) return rc;
+  const size_t nch = (n + kXferChunk - 1) / kXferChunk;
+  auto issue = [&](size_t k) -> int {
+    const int b = int(k & 1);
+    const size_t off = k * kXferChunk, len = std::min(kXferChunk, n - off);
+    HG_TRY(hipMemcpyAsync(x.pin[b], src + off, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HG_TRY(hipEventRecord(ctx->ev_xfer[b], ctx->stream));
+    return HG_OK;
+  };
+  for (size_t k = 0; k < std::min<size_t>(2, nch); ++k)
+    if (int rc = issue(k)) return rc;
+  for (size_t k = 0; k < nch; ++k) {
+    const int b = int(k & 1);
+    const size_t off = k * kXferChunk, len = std::min(kXferChunk, n - off);
+    HG_TRY(hipEventSynchronize(ctx->ev_xfer[b]));
+    x.pool->copy(dst + off, x.pin[b], len * sizeof(double));
+    if (k + 2 < nch)
+      if (int rc = issue(k + 2)) return rc;
+  }
+  return HG_OK;
+}
+
 int ensure_xfer(hg_ctx* ctx) {
   if (ctx->stage) return HG_OK;
   int rc = dalloc(ctx, &ctx->d_perm, size_t(ctx->N));
@@ -3110,7 +3266,8 @@ int perm_in(hg_ctx* ctx, const double* src, int ncomp, double* const d[3]) {
   const int64_t N = ctx->N;
   int rc = ensure_xfer(ctx);
   if (rc) return rc;
-  HG_TRY(hipMemcpyAsync(ctx->stage, src, size_t(ncomp) * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  rc = host_to_device(ctx, ctx->stage, src, size_t(ncomp) * N);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_perm_in, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, ncomp, ctx->d_perm, ctx->stage,
                      d[0], d[ncomp > 1 ? 1 : 0], d[ncomp > 1 ? 2 : 0]);
   HG_TRY(hipGetLastError());
@@ -3126,9 +3283,7 @@ int perm_out(hg_ctx* ctx, const double* const s[3], int ncomp, double* dst) {
   hipLaunchKernelGGL(k_perm_out, dim3(grid_of(N)), dim3(kBlock), 0, ctx->stream, N, ncomp, ctx->d_perm, s[0],
                      s[ncomp > 1 ? 1 : 0], s[ncomp > 1 ? 2 : 0], ctx->stage);
   HG_TRY(hipGetLastError());
-  HG_TRY(hipMemcpyAsync(dst, ctx->stage, size_t(ncomp) * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HG_TRY(hipStreamSynchronize(ctx->stream));
-  return HG_OK;
+  return device_to_host(ctx, dst, ctx->stage, size_t(ncomp) * N);
 }
 
 int state_in(hg_ctx* ctx, const double* Q, int slot) {
@@ -3866,7 +4021,8 @@ int hg_destroy(hg_ctx* ctx) {
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   for (auto& kv : ctx->seg_graphs) (void)hipGraphExecDestroy(kv.second);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd})
+  for (hipEvent_t e : {ctx->ev_seg[0], ctx->ev_seg[1], ctx->ev_rev[0], ctx->ev_rev[1], ctx->ev_fwd, ctx->ev_xfer[0],
+                       ctx->ev_xfer[1]})
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream2) {
     (void)hipStreamSynchronize(ctx->stream2);
