@@ -440,6 +440,74 @@ def measure_fields(args, capi):
     return out
 
 
+def measure_paths(args, capi, case, tile):
+    """The other device paths on the forward workload's 1M-triangle mesh, each
+    timed as the difference of two runs (n and 2n steps) so that the host
+    transfers at the call's ends cancel:
+      - adaptive Tsit5 (hg_tsit5_solve, abstol 1e-6, reltol 1e-3): 6 RHS
+        evaluations, 7 stage combinations and the error norm per accepted step;
+        algorithmic bytes per cell and accepted step 6 x 180 (RHS) + 42 x 24
+        (the combinations read y and 1..6 stage derivatives and write 3N each,
+        the error norm reads y, y_new and 7 derivatives);
+      - forward mode (hg_jvp): the Euler step on dual numbers, one tangent;
+      - the step with the reference's UDE network (hg_set_lux_ude: ManningN_h,
+        hidden layers [3, 3], tanh, the bump example's bounds) re-evaluating n
+        per cell before every step;
+      - the step with the variable-n power law re-evaluated before every step."""
+    from hgswe import cases  # noqa: F401  (the case is the forward leg's)
+
+    c = case
+    out = {}
+    n = 20
+
+    def diff(run, k):
+        run(k)  # warm-up
+        t0 = time.perf_counter()
+        a = run(k)
+        t1 = time.perf_counter()
+        b = run(2 * k)
+        t2 = time.perf_counter()
+        return (t2 - t1) - (t1 - t0), a, b
+
+    ctx = capi.Context(c)
+    ctx.set_tile_config(*tile)
+    span = 100 * c.dt  # about 15 accepted Tsit5 steps, 30 for the doubled span
+
+    def tsit(k):
+        _, st = ctx.tsit5(c.Q0, 0.0, span * k / n, 1, abstol=1e-6, reltol=1e-3, dt0=c.dt)
+        return st
+    el, sa, sb = diff(tsit, n)
+    acc = sb[0] - sa[0]
+    nrhs = sb[2] - sa[2]
+    bpc = 6 * bytes_per_cell_step(c.mesh) + 42 * 24
+    out["tsit5"] = {"ms_per_accepted_step": el * 1e3 / acc, "accepted_steps": [sa[0], sb[0]],
+                    "rejected_steps": [sa[1], sb[1]], "cell_rhs_per_s": c.N * nrhs / el,
+                    "bytes_per_cell_step": bpc, "achieved_GBps": bpc * c.N * acc / el / 1e9,
+                    "frac": bpc * c.N * acc / el / 1e9 / HBM_PEAK_GBS,
+                    "config": {"span_s": [span, 2 * span], "abstol": 1e-6, "reltol": 1e-3}}
+    rng = np.random.default_rng(12)
+    dQ = rng.normal(size=3 * c.N) * 1e-3
+    el, _, _ = diff(lambda k: ctx.jvp(c.Q0, c.dt, 5 * k, dQ0=dQ), n)
+    out["jvp"] = {"value": c.N * 5 * n / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / (5 * n)}
+    del ctx
+
+    ctx = capi.Context(c)
+    ctx.set_tile_config(*tile)
+    net = capi.lux_net("ManningN_h", [3, 3], ["tanh", "tanh"], (0.03, 0.06), (0.1, 0.5))
+    w = rng.normal(size=capi.lux_n_params(net)) * 0.5
+    ctx.set_lux_ude(net, w)
+    el, _, _ = diff(lambda k: ctx.step(c.Q0, c.dt, 5 * k), n)
+    out["lux_network_step"] = {"value": c.N * 5 * n / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / (5 * n),
+                               "network": "ManningN_h, hidden [3, 3], tanh, FP64"}
+    ctx.set_lux_ude(None)
+    ctx.set_n_law("power_law", n_lower=0.02, n_upper=0.06, k=0.3, h_mid=1.0)
+    el, _, _ = diff(lambda k: ctx.step(c.Q0, c.dt, 5 * k), n)
+    out["n_law_step"] = {"value": c.N * 5 * n / el, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / (5 * n),
+                         "law": "power law"}
+    del ctx
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,6 +536,7 @@ def main():
     ap.add_argument("--no-ude", action="store_true")
     ap.add_argument("--fields", type=int, default=32, help="Manning-n fields of the shared-mesh leg (0: skip it)")
     ap.add_argument("--tile-config", default="", help="T,G: fixed launch shape (tile cells, XCD chunk); no autotune")
+    ap.add_argument("--no-paths", action="store_true", help="skip the Tsit5 / forward-mode / network / n-law leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (256-cell tiles, identity tile mapping)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the timed region, run one device-to-device copy of this many MiB "
@@ -542,6 +611,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     tile = ctx.tile_config()
     del ctx
+    if not args.no_paths:
+        try:  # an auxiliary leg: a failure is reported in the line, not fatal to it
+            out["paths"] = measure_paths(args, capi, case, tile)
+        except Exception as e:
+            out["paths"] = {"error": repr(e)}
     if not args.no_wet_dry:
         out["wet_dry"] = measure_wet_dry(args, capi, tile)
     if not args.no_grad:
