@@ -127,7 +127,9 @@ def cpu_gradient_baseline(case, probe, target, seconds):
     """CPU cost of a gradient: the oracle's forward Euler steps plus a
     forward-mode (Dual) tangent sweep of the same steps, one thread.  A CPU
     reverse-mode gradient needs at least one forward and one sweep of
-    tangent-linear cost, so this is a lower bound on its time."""
+    tangent-linear cost, so this is a lower bound on its time; the oracle's
+    own reverse mode (a tape, as an operator-overloading AD tool builds) is
+    timed beside it as `reverse_tape`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -148,6 +150,15 @@ def cpu_gradient_baseline(case, probe, target, seconds):
     n, el = timed_sample(run, seconds / 4, 200)
     pyoracle.set_threads(1)
     out["all_cores"] = {"value": case.N * n / el, "cores": nt, "sample": f"{n} steps, {nt} threads"}
+    # a measured CPU reverse-mode gradient of the same loss: the oracle's tape
+    # (operator-overloading) reverse mode -- the forward run keeping every
+    # state, then each step recorded from its stored input and swept
+    # backwards -- over a bounded number of steps, 1 thread
+    n, el = timed_sample(lambda k: orc.grad_run(case.Q0, case.dt, k, probe, target, wrt=("n",)), seconds / 2, 16)
+    orc.release_tape()
+    out["reverse_tape"] = {"value": case.N * n / el, "cores": 1,
+                           "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle tape reverse mode "
+                                     "(forward run + per-step recording and reverse sweep), 1 thread"}
     return out
 
 
@@ -179,6 +190,7 @@ def measure_gradient(args, capi):
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
         out["speedup_vs_cpu_all_cores"] = out["value"] / cb["all_cores"]["value"]
+        out["speedup_vs_cpu_reverse_tape"] = out["value"] / cb["reverse_tape"]["value"]
     return out
 
 

@@ -10,10 +10,11 @@
 // left-to-right association, no fused multiply-add, the same branch tests,
 // eps(Float64)-smoothed primitives, and the reference's strong-zero Bool*x.
 //
-// The scalar type T is either double or Dual<K> (forward-mode derivative with
-// K tangent directions).  Dual follows ForwardDiff's "active branch"
-// semantics: comparisons and branch tests read the value part only, and a
-// branch that substitutes a constant carries a zero tangent.
+// The scalar type T is double, Dual<K> (forward-mode derivative with K
+// tangent directions) or Rev (reverse mode on a tape).  Dual follows
+// ForwardDiff's "active branch" semantics: comparisons and branch tests read
+// the value part only, and a branch that substitutes a constant carries a
+// zero tangent; Rev follows the same branches.
 //
 // Parity status of each function is tracked in the test-suite
 // (tests/test_oracle_*.py); see the header comment of each function for
@@ -83,6 +84,79 @@ template <int K> inline Dual<K> dpow(const Dual<K>& a, double p) {
   const double dd = p * std::pow(a.v, p - 1.0);
   for (int k = 0; k < K; ++k) r.d[k] = dd * a.d[k];
   return r;
+}
+
+// Reverse mode by operator overloading (the role Zygote plays in the
+// reference's inversion, swe_2D_inversion.jl:247-483): every operation with
+// an active operand appends one record to a tape -- the tape indices of its
+// operands (-1: a constant) and its partial derivatives with respect to them
+// -- and the reverse sweep walks the records backwards, adding
+// adjoint * partial into each operand.  Branch tests read the value only, as
+// for Dual, so a Rev run differentiates the same branches a Dual run does.
+// The tape is per thread; a computation recorded on it must run on one
+// thread (the OpenMP loops over cells with one thread).
+struct Tape {
+  struct Rec {
+    int32_t a, b;    // operand records, -1: a constant
+    double da, db;   // partials of the result with respect to them
+  };
+  std::vector<Rec> rec;
+  std::vector<double> adj;
+  size_t size() const { return rec.size(); }
+  void clear() { rec.clear(); }
+  int32_t push(int32_t ia, double pa, int32_t ib, double pb) {
+    if (rec.size() >= size_t(INT32_MAX)) throw std::runtime_error("reverse-mode tape: more than 2^31 records");
+    rec.push_back(Rec{ia, ib, pa, pb});
+    return int32_t(rec.size() - 1);
+  }
+  // adj[k] for every record given the output seeds already placed in adj
+  void sweep() {
+    for (size_t k = rec.size(); k-- > 0;) {
+      const double w = adj[k];
+      if (w == 0.0) continue;
+      const Rec& r = rec[k];
+      if (r.a >= 0) adj[r.a] += r.da * w;
+      if (r.b >= 0) adj[r.b] += r.db * w;
+    }
+  }
+};
+inline Tape& tape() { static thread_local Tape t; return t; }
+
+struct Rev {
+  double v;
+  int32_t i;  // tape record, -1 for a constant
+  Rev() : v(0.0), i(-1) {}
+  Rev(double x) : v(x), i(-1) {}  // NOLINT
+  static Rev input(double x) { Rev r(x); r.i = tape().push(-1, 0.0, -1, 0.0); return r; }
+};
+inline Rev rev_op(double v, int32_t ia, double pa, int32_t ib, double pb) {
+  Rev r(v);
+  if (ia >= 0 || ib >= 0) r.i = tape().push(ia, pa, ib, pb);
+  return r;
+}
+inline double val(const Rev& x) { return x.v; }
+inline Rev operator-(const Rev& a) { return rev_op(-a.v, a.i, -1.0, -1, 0.0); }
+inline Rev operator+(const Rev& a, const Rev& b) { return rev_op(a.v + b.v, a.i, 1.0, b.i, 1.0); }
+inline Rev operator-(const Rev& a, const Rev& b) { return rev_op(a.v - b.v, a.i, 1.0, b.i, -1.0); }
+inline Rev operator*(const Rev& a, const Rev& b) { return rev_op(a.v * b.v, a.i, b.v, b.i, a.v); }
+inline Rev operator/(const Rev& a, const Rev& b) {
+  const double r = a.v / b.v;
+  return rev_op(r, a.i, 1.0 / b.v, b.i, -r / b.v);
+}
+inline Rev operator+(const Rev& a, double b) { return a + Rev(b); }
+inline Rev operator+(double a, const Rev& b) { return Rev(a) + b; }
+inline Rev operator-(const Rev& a, double b) { return a - Rev(b); }
+inline Rev operator-(double a, const Rev& b) { return Rev(a) - b; }
+inline Rev operator*(const Rev& a, double b) { return a * Rev(b); }
+inline Rev operator*(double a, const Rev& b) { return Rev(a) * b; }
+inline Rev operator/(const Rev& a, double b) { return a / Rev(b); }
+inline Rev operator/(double a, const Rev& b) { return Rev(a) / b; }
+inline Rev dsqrt(const Rev& a) {
+  const double r = std::sqrt(a.v);
+  return rev_op(r, a.i, 1.0 / (2.0 * r), -1, 0.0);
+}
+inline Rev dpow(const Rev& a, double p) {
+  return rev_op(std::pow(a.v, p), a.i, p * std::pow(a.v, p - 1.0), -1, 0.0);
 }
 
 // Julia `Bool * x`: ifelse(b, x, copysign(zero(x), x)) -- a strong zero.

@@ -15,6 +15,7 @@ using orc::Consts;
 using orc::Dual;
 using orc::Fields;
 using orc::Mesh;
+using orc::Rev;
 
 namespace {
 thread_local std::string g_err;
@@ -217,6 +218,74 @@ int orc_jvp_state(const Mesh* m, const Consts* c, const FieldPtrs* f, const doub
       dQout[k] = q[k].d[0];
     }
   });
+}
+
+// Frees the calling thread's tape (orc_grad_run keeps its memory for the
+// next call, as a tape library keeps its tape between gradients).
+void orc_release_tape() { orc::tape() = orc::Tape(); }
+
+// Reverse-mode (tape) gradient of the probe loss
+//   L = sum_p (xi_p + hstill_p + zb_p - target_p)^2
+// at the final state of nsteps Euler steps from Q0, with respect to the
+// per-cell Manning n, the inlet discharges and the initial state: the forward
+// run keeps every state, then each step, last to first, is recorded on the
+// tape from its stored input and swept backwards from the next state's
+// adjoint.  NULL outputs are not formed.  Runs on one thread (the tape is per
+// thread); the OpenMP thread count is restored afterwards.
+int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const double* Q0, double dt, int64_t nsteps,
+                 int64_t n_probe, const int32_t* probe, const double* target, double* loss, double* gn,
+                 double* gQin, double* gQ0) {
+  const int nt = omp_get_max_threads();
+  omp_set_num_threads(1);
+  const int rc = guarded([&] {
+    const int64_t N = m->N, M3 = 3 * N;
+    const Fields<double> pd = make_fields<double>(*m, *f);
+    std::vector<std::vector<double>> traj(size_t(nsteps) + 1);
+    traj[0].assign(Q0, Q0 + M3);
+    for (int64_t s = 0; s < nsteps; ++s) {
+      traj[s + 1] = traj[s];
+      orc::custom_ODE_update_cells(*m, *c, pd, traj[s + 1], dt);
+    }
+    std::vector<double> lam(M3, 0.0);
+    double L = 0.0;
+    for (int64_t k = 0; k < n_probe; ++k) {
+      const int32_t i = probe[k];
+      const double r = traj[nsteps][i] + pd.hstill[i] + pd.zb[i] - target[k];
+      L = L + r * r;
+      lam[i] += 2.0 * r;
+    }
+    *loss = L;
+    if (gn) std::memset(gn, 0, sizeof(double) * N);
+    if (gQin) std::memset(gQin, 0, sizeof(double) * m->n_inlet);
+    Fields<Rev> p = make_fields<Rev>(*m, *f);
+    orc::Tape& t = orc::tape();
+    std::vector<Rev> q(M3);
+    std::vector<int32_t> qin(M3);
+    for (int64_t s = nsteps - 1; s >= 0; --s) {
+      t.clear();
+      for (int64_t k = 0; k < M3; ++k) {
+        q[k] = Rev::input(traj[s][k]);
+        qin[k] = q[k].i;
+      }
+      if (gn)
+        for (int64_t i = 0; i < N; ++i) p.n[i] = Rev::input(pd.n[i]);
+      if (gQin)
+        for (int b = 0; b < m->n_inlet; ++b) p.inletQ[b] = Rev::input(pd.inletQ[b]);
+      orc::custom_ODE_update_cells(*m, *c, p, q, dt);
+      t.adj.assign(t.size(), 0.0);
+      for (int64_t k = 0; k < M3; ++k)
+        if (q[k].i >= 0) t.adj[q[k].i] += lam[k];
+      t.sweep();
+      for (int64_t k = 0; k < M3; ++k) lam[k] = t.adj[qin[k]];
+      if (gn)
+        for (int64_t i = 0; i < N; ++i) gn[i] += t.adj[p.n[i].i];
+      if (gQin)
+        for (int b = 0; b < m->n_inlet; ++b) gQin[b] += t.adj[p.inletQ[b].i];
+    }
+    if (gQ0) std::memcpy(gQ0, lam.data(), sizeof(double) * M3);
+  });
+  omp_set_num_threads(nt);
+  return rc;
 }
 
 }  // extern "C"

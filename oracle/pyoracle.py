@@ -161,6 +161,30 @@ class Oracle:
         return Qout, dQout, loss.value, dloss.value
 
 
+    def grad_run(self, Q0, dt, nsteps, probe, target, wrt=("n", "Q", "Q0")):
+        """Reverse-mode (tape) gradient of the final-state probe loss
+        sum_p (xi_p + hstill_p + zb_p - target_p)^2 after nsteps Euler steps:
+        (loss, d/dn, d/dQ_inlet, d/dQ0); a gradient not named in `wrt` is
+        None.  One thread."""
+        N = self.case.N
+        Q0 = np.ascontiguousarray(Q0, dtype=D)
+        probe = np.ascontiguousarray(probe, dtype=I32)
+        target = np.ascontiguousarray(target, dtype=D)
+        gn = np.empty(N) if "n" in wrt else None
+        gq = np.empty(max(1, self.case.mesh.n_inlet)) if "Q" in wrt else None
+        g0 = np.empty(3 * N) if "Q0" in wrt else None
+        loss = C.c_double()
+        self._check(lib().orc_grad_run(
+            C.byref(self.mesh), C.byref(self.consts), C.byref(self.fields), _p(Q0), C.c_double(dt),
+            C.c_int64(nsteps), C.c_int64(probe.size), _p(probe), _p(target), C.byref(loss),
+            *[(_p(a) if a is not None else C.c_void_p(0)) for a in (gn, gq, g0)]))
+        return loss.value, gn, (None if gq is None else gq[: self.case.mesh.n_inlet]), g0
+
+    @staticmethod
+    def release_tape():
+        """Frees the tape grad_run keeps between calls."""
+        lib().orc_release_tape()
+
     def jvp_state(self, Q0, dQ0, dt, nsteps):
         """Dual run of nsteps Euler steps from (Q0, dQ0): (QN, dQN)."""
         N = self.case.N

@@ -6,8 +6,9 @@ Pins:
   per-cell Manning n, the bed zb (bed data rebuilt from zb) and the inlet
   discharge -- exact identity up to rounding (tolerance 1e-10 relative);
 * the checkpointed gradient of the probe WSE loss after N steps: g . v
-  against the oracle's Dual run of the same N steps (1e-8 relative) and
-  against central finite differences of the GPU forward solve;
+  against the oracle's Dual run of the same N steps (1e-8 relative), every
+  component against the oracle's tape reverse mode (1e-9 of the largest),
+  and against central finite differences of the GPU forward solve;
 * checkpoint-interval invariance and run-to-run determinism (bitwise).
 """
 import sys
@@ -153,6 +154,26 @@ def test_gradient_matches_oracle_dual_run(maker):
         say(f"  grad {c.name} [{name}]: loss gpu {loss:.12e} oracle {L:.12e}; g.v {gv:.12e} oracle JVP {dL:.12e}")
         assert abs(loss - L) <= 1e-10 * abs(L)
         assert abs(gv - dL) <= 1e-8 * abs(dL)
+
+
+@pytest.mark.parametrize("maker", [small_channel, wet_dry_case])
+def test_gradient_matches_oracle_reverse_mode(maker):
+    """Every component of the gradient w.r.t. the per-cell n and the inlet
+    discharge against the oracle's tape reverse mode of the same 30 steps
+    (the oracle's gradient is itself pinned against its Dual run in
+    test_oracle_reverse.py): max |difference| <= 1e-9 x max |gradient|."""
+    c = maker()
+    nsteps, kck = 30, 7
+    probe, target = _probe_problem(c, nsteps)
+    loss, gn, _, gq = capi.Context(c).grad_probes(c.Q0, c.dt, nsteps, kck, probe, target, wrt=("n", "Q"))
+    L, on, oq, _ = pyoracle.Oracle(c).grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("n", "Q"))
+    scale = float(np.max(np.abs(on)))
+    err = float(np.max(np.abs(gn - on)))
+    say(f"  grad {c.name}: loss gpu {loss:.12e} oracle {L:.12e}; max |dn diff| {err:.3e} of max {scale:.3e}; "
+        f"inlet {gq} vs {oq}")
+    assert abs(loss - L) <= 1e-10 * abs(L)
+    assert scale > 0.0 and err <= 1e-9 * scale
+    assert np.all(np.abs(gq - oq) <= 1e-9 * np.abs(oq))
 
 
 @pytest.mark.parametrize("maker", [small_channel, wet_dry_case, corner_inlets_case])
