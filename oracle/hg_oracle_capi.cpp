@@ -227,19 +227,22 @@ void orc_release_tape() { orc::tape() = orc::Tape(); }
 // Reverse-mode (tape) gradient of the probe loss
 //   L = sum_p (xi_p + hstill_p + zb_p - target_p)^2
 // at the final state of nsteps Euler steps from Q0, with respect to the
-// per-cell Manning n, the inlet discharges and the initial state: the forward
+// per-cell Manning n, the inlet discharges, the initial state and the bed zb
+// (a non-NULL gzb takes the active-zb path: zb_ghost and S0 rebuilt from zb,
+// as orc_jvp_run does for a non-NULL dzb): the forward
 // run keeps every state, then each step, last to first, is recorded on the
 // tape from its stored input and swept backwards from the next state's
 // adjoint.  NULL outputs are not formed.  Runs on one thread (the tape is per
 // thread); the OpenMP thread count is restored afterwards.
 int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const double* Q0, double dt, int64_t nsteps,
                  int64_t n_probe, const int32_t* probe, const double* target, double* loss, double* gn,
-                 double* gQin, double* gQ0) {
+                 double* gQin, double* gQ0, double* gzb) {
   const int nt = omp_get_max_threads();
   omp_set_num_threads(1);
   const int rc = guarded([&] {
     const int64_t N = m->N, M3 = 3 * N;
-    const Fields<double> pd = make_fields<double>(*m, *f);
+    Fields<double> pd = make_fields<double>(*m, *f);
+    if (gzb) orc::update_bed_data(*m, pd.zb, pd.zb_ghost, pd.S0x, pd.S0y);
     std::vector<std::vector<double>> traj(size_t(nsteps) + 1);
     traj[0].assign(Q0, Q0 + M3);
     for (int64_t s = 0; s < nsteps; ++s) {
@@ -255,6 +258,13 @@ int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const doubl
       lam[i] += 2.0 * r;
     }
     *loss = L;
+    if (gzb) {  // the loss reads zb directly as well
+      std::memset(gzb, 0, sizeof(double) * N);
+      for (int64_t k = 0; k < n_probe; ++k) {
+        const int32_t i = probe[k];
+        gzb[i] += 2.0 * (traj[nsteps][i] + pd.hstill[i] + pd.zb[i] - target[k]);
+      }
+    }
     if (gn) std::memset(gn, 0, sizeof(double) * N);
     if (gQin) std::memset(gQin, 0, sizeof(double) * m->n_inlet);
     Fields<Rev> p = make_fields<Rev>(*m, *f);
@@ -271,6 +281,10 @@ int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const doubl
         for (int64_t i = 0; i < N; ++i) p.n[i] = Rev::input(pd.n[i]);
       if (gQin)
         for (int b = 0; b < m->n_inlet; ++b) p.inletQ[b] = Rev::input(pd.inletQ[b]);
+      if (gzb) {  // the active-zb path: the bed data rebuilt from zb
+        for (int64_t i = 0; i < N; ++i) p.zb[i] = Rev::input(pd.zb[i]);
+        orc::update_bed_data(*m, p.zb, p.zb_ghost, p.S0x, p.S0y);
+      }
       orc::custom_ODE_update_cells(*m, *c, p, q, dt);
       t.adj.assign(t.size(), 0.0);
       for (int64_t k = 0; k < M3; ++k)
@@ -281,6 +295,8 @@ int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const doubl
         for (int64_t i = 0; i < N; ++i) gn[i] += t.adj[p.n[i].i];
       if (gQin)
         for (int b = 0; b < m->n_inlet; ++b) gQin[b] += t.adj[p.inletQ[b].i];
+      if (gzb)
+        for (int64_t i = 0; i < N; ++i) gzb[i] += t.adj[p.zb[i].i];
     }
     if (gQ0) std::memcpy(gQ0, lam.data(), sizeof(double) * M3);
   });

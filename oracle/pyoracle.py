@@ -164,8 +164,9 @@ class Oracle:
     def grad_run(self, Q0, dt, nsteps, probe, target, wrt=("n", "Q", "Q0")):
         """Reverse-mode (tape) gradient of the final-state probe loss
         sum_p (xi_p + hstill_p + zb_p - target_p)^2 after nsteps Euler steps:
-        (loss, d/dn, d/dQ_inlet, d/dQ0); a gradient not named in `wrt` is
-        None.  One thread."""
+        (loss, d/dn, d/dQ_inlet, d/dQ0) and, when "zb" is in `wrt`, d/dzb
+        last (the bed data then rebuilt from zb, the active-zb path); a
+        gradient not named in `wrt` is None.  One thread."""
         N = self.case.N
         Q0 = np.ascontiguousarray(Q0, dtype=D)
         probe = np.ascontiguousarray(probe, dtype=I32)
@@ -173,12 +174,14 @@ class Oracle:
         gn = np.empty(N) if "n" in wrt else None
         gq = np.empty(max(1, self.case.mesh.n_inlet)) if "Q" in wrt else None
         g0 = np.empty(3 * N) if "Q0" in wrt else None
+        gz = np.empty(N) if "zb" in wrt else None
         loss = C.c_double()
         self._check(lib().orc_grad_run(
             C.byref(self.mesh), C.byref(self.consts), C.byref(self.fields), _p(Q0), C.c_double(dt),
             C.c_int64(nsteps), C.c_int64(probe.size), _p(probe), _p(target), C.byref(loss),
-            *[(_p(a) if a is not None else C.c_void_p(0)) for a in (gn, gq, g0)]))
-        return loss.value, gn, (None if gq is None else gq[: self.case.mesh.n_inlet]), g0
+            *[(_p(a) if a is not None else C.c_void_p(0)) for a in (gn, gq, g0, gz)]))
+        out = (loss.value, gn, (None if gq is None else gq[: self.case.mesh.n_inlet]), g0)
+        return out + (gz,) if "zb" in wrt else out
 
     @staticmethod
     def release_tape():

@@ -158,21 +158,25 @@ def test_gradient_matches_oracle_dual_run(maker):
 
 @pytest.mark.parametrize("maker", [small_channel, wet_dry_case])
 def test_gradient_matches_oracle_reverse_mode(maker):
-    """Every component of the gradient w.r.t. the per-cell n and the inlet
-    discharge against the oracle's tape reverse mode of the same 30 steps
+    """Every component of the gradient w.r.t. the per-cell n, the bed zb
+    and the inlet discharge against the oracle's tape reverse mode of the
+    same 30 steps
     (the oracle's gradient is itself pinned against its Dual run in
     test_oracle_reverse.py): max |difference| <= 1e-9 x max |gradient|."""
     c = maker()
     nsteps, kck = 30, 7
     probe, target = _probe_problem(c, nsteps)
-    loss, gn, _, gq = capi.Context(c).grad_probes(c.Q0, c.dt, nsteps, kck, probe, target, wrt=("n", "Q"))
-    L, on, oq, _ = pyoracle.Oracle(c).grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("n", "Q"))
-    scale = float(np.max(np.abs(on)))
-    err = float(np.max(np.abs(gn - on)))
-    say(f"  grad {c.name}: loss gpu {loss:.12e} oracle {L:.12e}; max |dn diff| {err:.3e} of max {scale:.3e}; "
-        f"inlet {gq} vs {oq}")
+    loss, gn, gz, gq = capi.Context(c).grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    orc = pyoracle.Oracle(c)
+    L, on, oq, _ = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("n", "Q"))
+    _, _, _, _, oz = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("zb",))
+    for name, g, o in (("n", gn, on), ("zb", gz, oz)):
+        scale = float(np.max(np.abs(o)))
+        err = float(np.max(np.abs(g - o)))
+        say(f"  grad {c.name} [{name}]: loss gpu {loss:.12e} oracle {L:.12e}; max |diff| {err:.3e} of max {scale:.3e}")
+        assert scale > 0.0 and err <= 1e-9 * scale
+    say(f"  grad {c.name} [Qin]: {gq} vs {oq}")
     assert abs(loss - L) <= 1e-10 * abs(L)
-    assert scale > 0.0 and err <= 1e-9 * scale
     assert np.all(np.abs(gq - oq) <= 1e-9 * np.abs(oq))
 
 

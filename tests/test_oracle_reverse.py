@@ -5,8 +5,8 @@ sweeps it backwards; orc_jvp_run / orc_jvp_state push one tangent through the
 same steps.  Both differentiate the same arithmetic along the same branches,
 so for any direction v, g . v from the reverse sweep equals the Dual run's
 derivative of the loss up to rounding (tolerance 1e-10 relative), for the
-per-cell Manning n, the inlet discharges and the initial state, on a wet
-channel and on one with wet/dry fronts.  A step with its adjoint dropped
+per-cell Manning n, the inlet discharges, the initial state and the bed zb
+(bed data rebuilt from zb), on a wet channel and on one with wet/dry fronts.  A step with its adjoint dropped
 (the reverse sweep seeded from the wrong state) or a partial with a wrong
 sign fails these by orders of magnitude; the mistake test below plants one.
 """
@@ -46,15 +46,19 @@ def test_reverse_gradient_matches_forward_mode(maker):
     probe, target = probe_problem(c, nsteps)
     orc = pyoracle.Oracle(c)
     loss, gn, gq, g0 = orc.grad_run(c.Q0, c.dt, nsteps, probe, target)
+    _, _, _, _, gz = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("zb",))
     rng = np.random.default_rng(11)
     for _ in range(2):
         dn = rng.normal(size=c.N) * 1e-3
         dq = rng.normal(size=c.mesh.n_inlet)
+        dz = rng.normal(size=c.N) * 1e-2
         _, _, L, dLn = orc.jvp_run(c.Q0, c.dt, nsteps, dn=dn, probe=probe, target=target)
         _, _, _, dLq = orc.jvp_run(c.Q0, c.dt, nsteps, dQin=dq, probe=probe, target=target)
+        _, _, _, dLz = orc.jvp_run(c.Q0, c.dt, nsteps, dzb=dz, probe=probe, target=target)
         assert loss == L  # the same forward arithmetic
         assert abs(np.dot(gn, dn) - dLn) <= 1e-10 * abs(dLn)
         assert abs(np.dot(gq, dq) - dLq) <= 1e-10 * abs(dLq)
+        assert abs(np.dot(gz, dz) - dLz) <= 1e-10 * abs(dLz)
         dQ0 = rng.normal(size=3 * c.N) * 1e-3
         QN, dQN = orc.jvp_state(c.Q0, dQ0, c.dt, nsteps)
         r = QN[probe] + c.hstill[probe] + c.zb[probe] - target
