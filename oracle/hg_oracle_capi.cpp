@@ -241,6 +241,9 @@ int orc_grad_run(const Mesh* m, const Consts* c, const FieldPtrs* f, const doubl
   omp_set_num_threads(1);
   const int rc = guarded([&] {
     const int64_t N = m->N, M3 = 3 * N;
+    if (nsteps < 0 || n_probe < 0) throw std::invalid_argument("orc_grad_run: negative nsteps or n_probe");
+    for (int64_t k = 0; k < n_probe; ++k)
+      if (probe[k] < 0 || probe[k] >= N) throw std::invalid_argument("orc_grad_run: probe cell out of range");
     Fields<double> pd = make_fields<double>(*m, *f);
     if (gzb) orc::update_bed_data(*m, pd.zb, pd.zb_ghost, pd.S0x, pd.S0y);
     std::vector<std::vector<double>> traj(size_t(nsteps) + 1);

@@ -82,3 +82,23 @@ def test_planted_mistake_is_caught():
     _, _, _, dL = orc.jvp_run(c.Q0, c.dt, nsteps, dn=dn, probe=probe, target=target)
     assert abs(np.dot(gn, dn) - dL) <= 1e-10 * abs(dL)
     assert abs(np.dot(gn_short, dn) - dL) > 1e-4 * abs(dL)
+
+
+def test_zero_steps_closed_form():
+    """With no steps the loss reads the initial state directly: dL/dQ0 is
+    2 (xi + hstill + zb - target) at each probe's xi and zero elsewhere,
+    dL/dzb the same at the probes, dL/dn zero; a probe outside the mesh is
+    refused."""
+    c = small_channel()
+    probe = np.array([5, 17, 400], np.int32)
+    target = np.array([0.5, 1.0, 2.0])
+    L, gn, _, g0, gz = pyoracle.Oracle(c).grad_run(c.Q0, c.dt, 0, probe, target, wrt=("n", "Q0", "zb"))
+    r = c.Q0[probe] + c.hstill[probe] + c.zb[probe] - target
+    assert L == pytest.approx(float(np.sum(r * r)), rel=1e-15)
+    want = np.zeros(3 * c.N)
+    want[probe] = 2.0 * r
+    assert np.array_equal(g0, want)
+    assert np.array_equal(gz, want[: c.N])
+    assert not np.any(gn)
+    with pytest.raises(RuntimeError, match="out of range"):
+        pyoracle.Oracle(c).grad_run(c.Q0, c.dt, 1, np.array([c.N], np.int32), np.zeros(1))
