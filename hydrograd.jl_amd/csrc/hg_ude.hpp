@@ -45,10 +45,12 @@ struct Mfma<float> {
   // bit against the reference's Float64 arithmetic, which the solver keeps.
   static __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
   // tanh in f32 as 1 - 2 / (e^(2|x|) + 1) with the sign of x, from the
-  // hardware exp and reciprocal: absolute error about 6e-8 over the whole
-  // range (checked against float64 tanh on the host), the resolution of an
-  // f32 activation near 1; near 0 the relative error grows, which the next
-  // layer's f32 sums do not see.  A Taylor branch for |x| < 0.4 that kept the
+  // hardware exp and reciprocal: with every step rounded as f32, the exp off
+  // by up to three ulp and the reciprocal by one, the absolute error against
+  // float64 tanh stays below 5e-7 (largest near 0, where 1 - 2r cancels;
+  // below 1e-7 for |x| >= 0.5 with correctly rounded steps), a few ulp of an
+  // f32 activation, which the next layer's f32 sums carry at the same
+  // magnitude.  A Taylor branch for |x| < 0.4 that kept the
   // relative error to 4 ulp cost 6 % of the 32-set closure gradient and left
   // the closure's n against a float64 evaluation unchanged (1.0e-6).  The
   // closure's 32 activations per cell dominate its forward and reverse kernels.
