@@ -158,19 +158,26 @@ def test_gradient_matches_oracle_dual_run(maker):
 
 @pytest.mark.parametrize("maker", [small_channel, wet_dry_case])
 def test_gradient_matches_oracle_reverse_mode(maker):
-    """Every component of the gradient w.r.t. the per-cell n, the bed zb
-    and the inlet discharge against the oracle's tape reverse mode of the
-    same 30 steps
+    """Every component of the gradient w.r.t. the per-cell n, the bed zb,
+    the inlet discharge and the initial state against the oracle's tape
+    reverse mode of the same 30 steps
     (the oracle's gradient is itself pinned against its Dual run in
     test_oracle_reverse.py): max |difference| <= 1e-9 x max |gradient|."""
     c = maker()
     nsteps, kck = 30, 7
     probe, target = _probe_problem(c, nsteps)
-    loss, gn, gz, gq = capi.Context(c).grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
+    ctx = capi.Context(c)
+    loss, gn, gz, gq = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target)
     orc = pyoracle.Oracle(c)
-    L, on, oq, _ = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("n", "Q"))
+    L, on, oq, o0 = orc.grad_run(c.Q0, c.dt, nsteps, probe, target)
     _, _, _, _, oz = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("zb",))
-    for name, g, o in (("n", gn, on), ("zb", gz, oz)):
+    # the initial-state cotangent through hg_adjoint, seeded with the loss's
+    # final-state cotangent 2 r at the probes
+    QN = orc.run(c.Q0, c.dt, nsteps)
+    QN_bar = np.zeros(3 * c.N)
+    QN_bar[probe] = 2.0 * (QN[probe] + c.hstill[probe] + c.zb[probe] - target)
+    g0, gn_adj, _, _ = ctx.adjoint(c.Q0, c.dt, nsteps, kck, QN_bar)
+    for name, g, o in (("n", gn, on), ("zb", gz, oz), ("Q0", g0, o0), ("n via hg_adjoint", gn_adj, on)):
         scale = float(np.max(np.abs(o)))
         err = float(np.max(np.abs(g - o)))
         say(f"  grad {c.name} [{name}]: loss gpu {loss:.12e} oracle {L:.12e}; max |diff| {err:.3e} of max {scale:.3e}")
