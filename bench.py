@@ -154,11 +154,15 @@ def cpu_gradient_baseline(case, probe, target, seconds):
     # (operator-overloading) reverse mode -- the forward run keeping every
     # state, then each step recorded from its stored input and swept
     # backwards -- over a bounded number of steps, 1 thread
-    n, el = timed_sample(lambda k: orc.grad_run(case.Q0, case.dt, k, probe, target, wrt=("n",)), seconds / 2, 16)
-    orc.release_tape()
-    out["reverse_tape"] = {"value": case.N * n / el, "cores": 1,
-                           "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle tape reverse mode "
-                                     "(forward run + per-step recording and reverse sweep), 1 thread"}
+    try:  # an auxiliary figure: a failure is reported in the line, not fatal to it
+        n, el = timed_sample(lambda k: orc.grad_run(case.Q0, case.dt, k, probe, target, wrt=("n",)), seconds / 2, 16)
+        out["reverse_tape"] = {"value": case.N * n / el, "cores": 1,
+                               "sample": f"{n} steps of the {case.N}-cell gradient mesh: C++ oracle tape reverse "
+                                         "mode (forward run + per-step recording and reverse sweep), 1 thread"}
+    except Exception as e:
+        out["reverse_tape"] = {"error": repr(e)}
+    finally:
+        orc.release_tape()
     return out
 
 
@@ -190,7 +194,8 @@ def measure_gradient(args, capi):
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
         out["speedup_vs_cpu_all_cores"] = out["value"] / cb["all_cores"]["value"]
-        out["speedup_vs_cpu_reverse_tape"] = out["value"] / cb["reverse_tape"]["value"]
+        if "value" in cb["reverse_tape"]:
+            out["speedup_vs_cpu_reverse_tape"] = out["value"] / cb["reverse_tape"]["value"]
     return out
 
 
