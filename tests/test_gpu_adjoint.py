@@ -187,6 +187,31 @@ def test_gradient_matches_oracle_reverse_mode(maker):
     assert np.all(np.abs(gq - oq) <= 1e-9 * np.abs(oq))
 
 
+def test_gradient_matches_oracle_reverse_mode_long_horizon():
+    """The same component-wise check over 300 steps of a 10k-cell channel
+    with a checkpoint every 25 steps (12 recomputed segments): max
+    |difference| <= 1e-9 x max |gradient| for n, and for the initial state
+    through hg_adjoint."""
+    c = cases.channel(10_000, seed=31, Lx=2000.0, Ly=200.0, Q_in=80.0, nsteps=300)
+    nsteps, kck = 300, 25
+    probe, target = _probe_problem(c, nsteps, seed=9, nprobe=40)
+    ctx = capi.Context(c)
+    loss, gn, _, _ = ctx.grad_probes(c.Q0, c.dt, nsteps, kck, probe, target, wrt=("n",))
+    orc = pyoracle.Oracle(c)
+    L, on, _, o0 = orc.grad_run(c.Q0, c.dt, nsteps, probe, target, wrt=("n", "Q0"))
+    pyoracle.Oracle.release_tape()
+    QN = orc.run(c.Q0, c.dt, nsteps)
+    QN_bar = np.zeros(3 * c.N)
+    QN_bar[probe] = 2.0 * (QN[probe] + c.hstill[probe] + c.zb[probe] - target)
+    g0, _, _, _ = ctx.adjoint(c.Q0, c.dt, nsteps, kck, QN_bar)
+    assert abs(loss - L) <= 1e-10 * abs(L)
+    for name, g, o in (("n", gn, on), ("Q0", g0, o0)):
+        scale = float(np.max(np.abs(o)))
+        err = float(np.max(np.abs(g - o)))
+        say(f"  grad {c.name} 300 steps [{name}]: max |diff| {err:.3e} of max {scale:.3e}")
+        assert scale > 0.0 and err <= 1e-9 * scale
+
+
 @pytest.mark.parametrize("maker", [small_channel, wet_dry_case, corner_inlets_case])
 def test_tiled_reverse_step_matches_cell_reverse_step(maker):
     """The tiled reverse step (each face's VJP formed once for both cells,
